@@ -1,0 +1,225 @@
+/*
+ * sdr_amd.h — C ABI of the MI355X-native FLIP/PIC substep library (libsdr_amd.so).
+ *
+ * The boundary mirrors the reference crate's `Solver` / `pressure` surface
+ * (uselessgoddess/sdr).  Every entry point below names the Rust item it replaces.
+ * A Rust crate binds this header with `extern "C"` blocks;
+ * Python binds it with ctypes (sdr_amd/_abi.py).
+ *
+ * Conventions
+ *   - Return value: 0 on success, negative SDR_E_* on failure; the message of the
+ *     last failure on the calling thread is available from sdr_last_error().
+ *   - The caller owns every host buffer (copied in / out).  The library owns the
+ *     device buffers of a solver.  One solver = one HIP stream; a solver is not
+ *     thread-safe, distinct solvers may be driven from distinct threads.
+ *   - Vectors are passed as `double[3]` triples (AoS, x,y,z), exactly the memory
+ *     layout of the reference's `Vec<Vec3>`.
+ *   - Cell kinds use the reference enum's discriminant order
+ *     (`pressure.rs:24-32`): 0 = Solid, 1 = Fluid, 2 = Air.
+ */
+#ifndef SDR_AMD_H
+#define SDR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDR_OK 0
+#define SDR_E_INVALID (-1)  /* bad argument / size mismatch (reference: assert_eq! panics) */
+#define SDR_E_OOM (-2)      /* device allocation failed */
+#define SDR_E_HIP (-3)      /* a HIP runtime call failed */
+#define SDR_E_NODEV (-4)    /* no usable gfx950 device */
+#define SDR_E_STATE (-5)    /* call not valid in the current state */
+#define SDR_E_NUMERIC (-6)  /* NaN residual or persistent-kernel wait limit hit */
+
+/* Arithmetic of the device path.
+ *   SDR_PRECISION_F64: the plain drop-in; every field in f64, reference operand order.
+ *   SDR_PRECISION_F32: f32 particle/grid/PCG storage and sweeps, f64 dots, norms and
+ *                      mass sums, f64 outer iterative refinement of the pressure solve. */
+#define SDR_PRECISION_F64 0
+#define SDR_PRECISION_F32 1
+
+/* Dot-product reduction of the f64 path.
+ *   SDR_DOT_EXACT: the reference's 4096-element chunk tree (`pressure.rs:588-603`),
+ *                  bit-identical to the CPU path.
+ *   SDR_DOT_FAST:  wavefront-shuffle tree in f64, deterministic but not the reference tree. */
+#define SDR_DOT_EXACT 0
+#define SDR_DOT_FAST 1
+
+typedef struct sdr_vec3 {
+  double x, y, z;
+} sdr_vec3;
+
+/* `Aabb` (math.rs:198-201); `contains` is inclusive on both ends. */
+typedef struct sdr_aabb {
+  sdr_vec3 min, max;
+} sdr_aabb;
+
+/* `Sdf` (sdf.rs:23-34): node-grid signed distance, x-fastest, phi < 0 inside the cavity. */
+typedef struct sdr_sdf {
+  sdr_vec3 origin;
+  double dx;
+  uint64_t nx, ny, nz;
+  const double* data; /* nx*ny*nz values, host memory, copied at solver creation */
+} sdr_sdf;
+
+/* `FluidParams` (solver.rs:41-85). */
+typedef struct sdr_fluid_params {
+  double density;
+  sdr_vec3 gravity;
+  double flip_ratio;
+  uint64_t particles_per_cell;
+  double volume_stiffness;
+  int32_t redistribute; /* bool */
+  double redist_cap_factor;
+} sdr_fluid_params;
+
+/* `Needle` (solver.rs:90-104). */
+typedef struct sdr_needle {
+  sdr_vec3 tip;
+  sdr_vec3 axis;
+  double radius;
+  double flow_rate;
+  double inject_until; /* +inf = inject for the whole run */
+} sdr_needle;
+
+/* `SolveParams` (pressure.rs:36-50): defaults {200, 1e-6}. */
+typedef struct sdr_solve_params {
+  uint64_t max_iters;
+  double tol;
+} sdr_solve_params;
+
+/* `SolveReport` (pressure.rs:54-57). */
+typedef struct sdr_solve_report {
+  uint64_t iters;
+  double residual;
+} sdr_solve_report;
+
+/* Device-path options (no reference counterpart: the reference is CPU-only). */
+typedef struct sdr_options {
+  int32_t precision; /* SDR_PRECISION_* */
+  int32_t dot_mode;  /* SDR_DOT_* (f64 path only) */
+  int32_t device;    /* HIP device ordinal */
+  double refine_tol; /* f32 path: relative residual target of the f64 refinement (1e-8) */
+} sdr_options;
+
+/* Scalar solver state: the reference's pub fields `time, steps, last_pcg_iters,
+ * last_substeps, drained` (solver.rs:152-165) plus the private emission state
+ * `rng, emit_accum` (solver.rs:177-179) and the particle-conservation counters the
+ * build adds (emitted = resident + drained + culled). */
+typedef struct sdr_state {
+  double time;
+  uint64_t steps;
+  uint64_t last_pcg_iters;
+  uint32_t last_substeps;
+  uint32_t _pad;
+  uint64_t drained;
+  uint64_t rng;
+  double emit_accum;
+  uint64_t emitted;
+  uint64_t culled;
+  double last_residual;
+} sdr_state;
+
+typedef struct sdr_dims {
+  uint64_t nx, ny, nz;
+  double dx;
+  sdr_vec3 origin;
+} sdr_dims;
+
+/* Stages of `Solver::substep` (solver.rs:284-310), for stage-isolated runs. */
+#define SDR_STAGE_EMIT 0           /* solver.rs:313-339 */
+#define SDR_STAGE_CLASSIFY 1       /* solver.rs:342-371 */
+#define SDR_STAGE_P2G 2            /* solver.rs:560-585 */
+#define SDR_STAGE_SAVE 3           /* solver.rs:587-591 */
+#define SDR_STAGE_FORCES 4         /* solver.rs:593-604 */
+#define SDR_STAGE_PROJECT 5        /* volume_source + project_capturing_with_source, solver.rs:290-299 */
+#define SDR_STAGE_EXTRAPOLATE 6    /* solver.rs:608-647 */
+#define SDR_STAGE_G2P 7            /* solver.rs:650-677 */
+#define SDR_STAGE_ADVECT 8         /* solver.rs:693-711 */
+#define SDR_STAGE_BOUNDS 9         /* solver.rs:736-755 */
+#define SDR_STAGE_REDISTRIBUTE 10  /* solver.rs:454-557 (runs only if fluid.redistribute) */
+#define SDR_STAGE_FINISH 11        /* time += dt; steps += 1 (solver.rs:308-309) */
+#define SDR_STAGE_COUNT 12
+
+typedef struct sdr_solver sdr_solver;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* sdr_last_error(void);
+
+/* Library / device query: number of visible devices and gfx name of device 0. */
+int sdr_device_info(int32_t* count, char* arch, size_t arch_len);
+
+/* `Solver::new(solid, sim_dx, fluid, needle)` (solver.rs:184-222).  `opts` may be
+ * NULL (f64, exact dots, device 0). */
+int sdr_solver_new(const sdr_sdf* solid, double sim_dx, const sdr_fluid_params* fluid,
+                   const sdr_needle* needle, const sdr_options* opts, sdr_solver** out);
+int sdr_solver_free(sdr_solver* s);
+
+/* `solver.outlet = Some(aabb)` / `None` (solver.rs:150). */
+int sdr_solver_set_outlet(sdr_solver* s, const sdr_aabb* outlet /* nullable */);
+/* `solver.solve = params` (solver.rs:151). */
+int sdr_solver_set_solve_params(sdr_solver* s, const sdr_solve_params* params);
+/* `solver.needle = needle` (solver.rs:148). */
+int sdr_solver_set_needle(sdr_solver* s, const sdr_needle* needle);
+
+/* `Solver::step(dt)` (solver.rs:237-259): CFL sub-stepping of one frame. */
+int sdr_solver_step(sdr_solver* s, double dt);
+/* `Solver::substep(dt)` (solver.rs:284-310). */
+int sdr_solver_substep(sdr_solver* s, double dt);
+/* One stage of `Solver::substep` (SDR_STAGE_*), for stage-isolated parity runs. */
+int sdr_solver_run_stage(sdr_solver* s, int32_t stage, double dt);
+/* `Solver::redistribute()` (solver.rs:454-557), independent of fluid.redistribute. */
+int sdr_solver_redistribute(sdr_solver* s);
+
+/* `solver.particles` (particles.rs:13-16): count, read (reference order), replace. */
+int sdr_solver_particle_count(const sdr_solver* s, uint64_t* n);
+int sdr_solver_get_particles(sdr_solver* s, double* pos_xyz, double* vel_xyz, uint64_t n);
+int sdr_solver_set_particles(sdr_solver* s, const double* pos_xyz, const double* vel_xyz,
+                             uint64_t n);
+
+int sdr_solver_get_dims(const sdr_solver* s, sdr_dims* out);
+/* `solver.grid.{u,v,w}` (grid.rs:92-101), face arrays of (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1). */
+int sdr_solver_get_grid(sdr_solver* s, double* u, double* v, double* w);
+int sdr_solver_set_grid(sdr_solver* s, const double* u, const double* v, const double* w);
+/* private `saved_{u,v,w}` (solver.rs:174-176): the post-P2G grid FLIP differences against. */
+int sdr_solver_get_saved(sdr_solver* s, double* u, double* v, double* w);
+int sdr_solver_set_saved(sdr_solver* s, const double* u, const double* v, const double* w);
+/* private `cells` (solver.rs:167): 0 Solid, 1 Fluid, 2 Air. */
+int sdr_solver_get_cells(sdr_solver* s, uint8_t* cells);
+int sdr_solver_set_cells(sdr_solver* s, const uint8_t* cells);
+/* `solver.pressure` (solver.rs:163). */
+int sdr_solver_get_pressure(sdr_solver* s, double* p);
+int sdr_solver_get_state(const sdr_solver* s, sdr_state* st);
+int sdr_solver_set_state(sdr_solver* s, const sdr_state* st);
+
+/* Per-kernel timing of the last substep, filled when enabled (HIP events on the
+ * solver's stream).  names/ms arrays of length cap; *count receives the number written. */
+int sdr_solver_enable_timing(sdr_solver* s, int32_t on);
+int sdr_solver_get_timing(sdr_solver* s, char* names, size_t name_len, double* ms,
+                          uint64_t* calls, uint64_t cap, uint64_t* count);
+
+/* Append `per_cell` particles to every cavity cell whose centre has y_lo <= y <= y_hi
+ * (centre + splitmix64 jitter of (seed, index)), all with velocity `vel` — the synthetic
+ * initial fill of the benchmark cavities (no reference counterpart; the reference only
+ * emits from the needle).  Runs on the device. */
+int sdr_solver_seed_fill(sdr_solver* s, uint32_t per_cell, double y_lo, double y_hi,
+                         uint64_t seed, const sdr_vec3* vel, uint64_t* n_out);
+
+/* `pressure::project_capturing_with_source(grid, cells, rho, dt, params, source)`
+ * (pressure.rs:123-239) on host buffers.  u,v,w are updated in place; pressure_out
+ * (nullable) receives the per-cell pressure; source is nullable (None). */
+int sdr_project_capturing_with_source(const sdr_dims* grid, double* u, double* v, double* w,
+                                      const uint8_t* cells, double rho, double dt,
+                                      const sdr_solve_params* params, const double* source,
+                                      double* pressure_out, sdr_solve_report* report,
+                                      const sdr_options* opts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDR_AMD_H */

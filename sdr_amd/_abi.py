@@ -1,0 +1,187 @@
+"""ctypes declarations of the C ABI in include/sdr_amd.h.
+
+The same declarations bind any library that exports the ABI under a prefix, so the
+Python mirror of the reference `Solver` API (sdr_amd.solver) drives the device
+library; test code binds the CPU oracle through the same declarations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+c_double_p = C.POINTER(C.c_double)
+c_u8_p = C.POINTER(C.c_uint8)
+c_u64_p = C.POINTER(C.c_uint64)
+
+SDR_OK = 0
+SDR_E_INVALID = -1
+SDR_E_OOM = -2
+SDR_E_HIP = -3
+SDR_E_NODEV = -4
+SDR_E_STATE = -5
+SDR_E_NUMERIC = -6
+
+PRECISION_F64 = 0
+PRECISION_F32 = 1
+DOT_EXACT = 0
+DOT_FAST = 1
+
+STAGES = [
+    "emit", "classify", "p2g", "save", "forces", "project", "extrapolate", "g2p", "advect",
+    "bounds", "redistribute", "finish",
+]
+STAGE = {name: i for i, name in enumerate(STAGES)}
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class Aabb(C.Structure):
+    _fields_ = [("min", Vec3), ("max", Vec3)]
+
+
+class SdfDesc(C.Structure):
+    _fields_ = [("origin", Vec3), ("dx", C.c_double), ("nx", C.c_uint64), ("ny", C.c_uint64),
+                ("nz", C.c_uint64), ("data", c_double_p)]
+
+
+class FluidParams(C.Structure):
+    _fields_ = [("density", C.c_double), ("gravity", Vec3), ("flip_ratio", C.c_double),
+                ("particles_per_cell", C.c_uint64), ("volume_stiffness", C.c_double),
+                ("redistribute", C.c_int32), ("redist_cap_factor", C.c_double)]
+
+
+class NeedleC(C.Structure):
+    _fields_ = [("tip", Vec3), ("axis", Vec3), ("radius", C.c_double), ("flow_rate", C.c_double),
+                ("inject_until", C.c_double)]
+
+
+class SolveParamsC(C.Structure):
+    _fields_ = [("max_iters", C.c_uint64), ("tol", C.c_double)]
+
+
+class SolveReportC(C.Structure):
+    _fields_ = [("iters", C.c_uint64), ("residual", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("dot_mode", C.c_int32), ("device", C.c_int32),
+                ("refine_tol", C.c_double)]
+
+
+class State(C.Structure):
+    _fields_ = [("time", C.c_double), ("steps", C.c_uint64), ("last_pcg_iters", C.c_uint64),
+                ("last_substeps", C.c_uint32), ("_pad", C.c_uint32), ("drained", C.c_uint64),
+                ("rng", C.c_uint64), ("emit_accum", C.c_double), ("emitted", C.c_uint64),
+                ("culled", C.c_uint64), ("last_residual", C.c_double)]
+
+
+class Dims(C.Structure):
+    _fields_ = [("nx", C.c_uint64), ("ny", C.c_uint64), ("nz", C.c_uint64), ("dx", C.c_double),
+                ("origin", Vec3)]
+
+
+P = C.c_void_p
+# name -> argtypes (restype is always c_int unless listed in _RESTYPE)
+SIGNATURES = {
+    "last_error": [],
+    "solver_new": [C.POINTER(SdfDesc), C.c_double, C.POINTER(FluidParams), C.POINTER(NeedleC),
+                   C.POINTER(Options), C.POINTER(P)],
+    "solver_free": [P],
+    "solver_set_outlet": [P, C.POINTER(Aabb)],
+    "solver_set_solve_params": [P, C.POINTER(SolveParamsC)],
+    "solver_set_needle": [P, C.POINTER(NeedleC)],
+    "solver_step": [P, C.c_double],
+    "solver_substep": [P, C.c_double],
+    "solver_run_stage": [P, C.c_int32, C.c_double],
+    "solver_redistribute": [P],
+    "solver_particle_count": [P, c_u64_p],
+    "solver_get_particles": [P, c_double_p, c_double_p, C.c_uint64],
+    "solver_set_particles": [P, c_double_p, c_double_p, C.c_uint64],
+    "solver_get_dims": [P, C.POINTER(Dims)],
+    "solver_get_grid": [P, c_double_p, c_double_p, c_double_p],
+    "solver_set_grid": [P, c_double_p, c_double_p, c_double_p],
+    "solver_get_saved": [P, c_double_p, c_double_p, c_double_p],
+    "solver_set_saved": [P, c_double_p, c_double_p, c_double_p],
+    "solver_get_cells": [P, c_u8_p],
+    "solver_set_cells": [P, c_u8_p],
+    "solver_get_pressure": [P, c_double_p],
+    "solver_get_state": [P, C.POINTER(State)],
+    "solver_set_state": [P, C.POINTER(State)],
+    "project_capturing_with_source": [C.POINTER(Dims), c_double_p, c_double_p, c_double_p, c_u8_p,
+                                      C.c_double, C.c_double, C.POINTER(SolveParamsC), c_double_p,
+                                      c_double_p, C.POINTER(SolveReportC), C.POINTER(Options)],
+}
+# Entry points only the device library has.
+DEVICE_ONLY = {
+    "device_info": [C.POINTER(C.c_int32), C.c_char_p, C.c_size_t],
+    "solver_enable_timing": [P, C.c_int32],
+    "solver_get_timing": [P, C.c_char_p, C.c_size_t, c_double_p, c_u64_p, C.c_uint64, c_u64_p],
+    "solver_seed_fill": [P, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Vec3),
+                         c_u64_p],
+}
+
+
+class AbiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Lib:
+    """A loaded library exporting the ABI under `prefix` (``sdr_`` or the oracle's)."""
+
+    def __init__(self, path: str | Path, prefix: str, device: bool):
+        self.path = str(path)
+        self.prefix = prefix
+        self.device = device
+        self.dll = C.CDLL(self.path)
+        sigs = dict(SIGNATURES)
+        if device:
+            sigs.update(DEVICE_ONLY)
+        self._fns = {}
+        for name, args in sigs.items():
+            fn = getattr(self.dll, prefix + name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name == "last_error" else C.c_int
+            self._fns[name] = fn
+
+    def has(self, name: str) -> bool:
+        return name in self._fns
+
+    def __getattr__(self, name):
+        fns = self.__dict__.get("_fns")
+        if fns is None or name not in fns:
+            raise AttributeError(name)
+        fn = fns[name]
+        if name == "last_error":
+            return fn
+
+        def call(*args):
+            rc = fn(*args)
+            if rc != SDR_OK:
+                raise AbiError(rc, fns["last_error"]().decode(errors="replace"))
+            return rc
+
+        return call
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_double_p)
+
+
+def u8ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_u8_p)
+
+
+def vec3(v) -> Vec3:
+    return Vec3(float(v[0]), float(v[1]), float(v[2]))

@@ -1,0 +1,1227 @@
+// kernels.hip — gfx950 kernels of the FLIP/PIC substep (uselessgoddess/sdr hot path).
+//
+// Each kernel names the reference loop it replaces.  No kernel uses floating-point
+// atomics: every float sum is computed by one thread in a fixed order, so results are
+// bitwise reproducible; integer atomics (histograms, counters, max on non-negative
+// bit patterns) are order-independent.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace sdr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+template <class T>
+__device__ inline V3<T> load_pos(const Parts<T>& p, int64_t t) {
+  return mk3(p.px[t], p.py[t], p.pz[t]);
+}
+template <class T>
+__device__ inline V3<T> load_vel(const Parts<T>& p, int64_t t) {
+  return mk3(p.vx[t], p.vy[t], p.vz[t]);
+}
+
+// ------------------------------------------------------------------ cell keys + sort glue
+// Key = the clamped cell (every particle scatters through it, grid.rs:52-88); the exact
+// mode appends the reference index so each cell's segment is in reference order.
+template <class T>
+__global__ void k_keys(Parts<T> p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
+                       uint32_t* vals, int exact) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t c = g.clamped_cell(load_pos(p, t));
+  if (exact)
+    keys64[t] = ((uint64_t)c << 32) | (uint64_t)p.rank[t];
+  else
+    keys[t] = (uint32_t)c;
+  vals[t] = (uint32_t)t;
+}
+
+template <class T>
+__global__ void k_gather_parts(Parts<T> s, Parts<T> d, const uint32_t* __restrict__ perm,
+                               int64_t n) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t q = perm[t];
+  d.px[t] = s.px[q];
+  d.py[t] = s.py[q];
+  d.pz[t] = s.pz[q];
+  d.vx[t] = s.vx[q];
+  d.vy[t] = s.vy[q];
+  d.vz[t] = s.vz[q];
+  d.rank[t] = s.rank[q];
+}
+
+__global__ void k_cell_ranges(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ keys64,
+                              int64_t n, uint32_t* cstart, uint32_t* cend) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  auto cell = [&](int64_t q) -> uint32_t {
+    return keys64 ? (uint32_t)(keys64[q] >> 32) : keys[q];
+  };
+  uint32_t c = cell(t);
+  if (t == 0 || cell(t - 1) != c) cstart[c] = (uint32_t)t;
+  if (t == n - 1 || cell(t + 1) != c) cend[c] = (uint32_t)(t + 1);
+}
+
+// ------------------------------------------------------------------ classify (solver.rs:342-371)
+// Per cell: the static class (outlet -> Air, !mask -> Solid, else Air) then Fluid if a
+// particle whose `cell_of` is this cell lies outside the outlet.  Also counts the
+// particles whose `cell_of` is this cell (volume_source, solver.rs:400-409).
+template <class T>
+__global__ void k_classify(const uint8_t* __restrict__ static_cls, const uint32_t* __restrict__ cstart,
+                           const uint32_t* __restrict__ cend, Parts<T> p, GridD<T> g, BoxD<T> outlet,
+                           uint8_t* cells, uint32_t* count) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+  if (c >= nc) return;
+  uint8_t cls = static_cls[c];
+  uint32_t s = cstart[c], e = cend[c];
+  uint32_t cnt = 0;
+  bool fluid = false;
+  for (uint32_t t = s; t < e; ++t) {
+    V3<T> q = load_pos(p, t);
+    if (g.cell_of(q) != c) continue;
+    ++cnt;
+    if (!outlet.contains(q)) fluid = true;
+  }
+  cells[c] = (cls == kAir && fluid) ? (uint8_t)kFluid : cls;
+  count[c] = cnt;
+}
+
+// ------------------------------------------------------------------ P2G (solver.rs:560-585)
+// One thread per face gathers every particle whose clamped trilinear stencil
+// (grid.rs:52-88) covers it, from the cell segments that can reach the face: the two
+// cells straddling it along the component axis and the 3x3 cells around it across.
+// The fast mode visits segments in a fixed cell order; the exact mode merges the
+// segments by reference index, reproducing the reference's per-face summation order.
+// The weight sum and the normalisation (solver.rs:944-952) follow in the same thread.
+template <class T, int COMP, bool EXACT>
+__global__ void k_p2g(GridD<T> g, Parts<T> p, const uint32_t* __restrict__ cstart,
+                      const uint32_t* __restrict__ cend, T* __restrict__ out) {
+  const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t nf = (int64_t)fnx * fny * fnz;
+  if (f >= nf) return;
+  const int fi = (int)(f % fnx), fj = (int)((f / fnx) % fny), fk = (int)(f / ((int64_t)fnx * fny));
+  const T* vel = COMP == 0 ? p.vx : (COMP == 1 ? p.vy : p.vz);
+  // Candidate cell ranges per axis: along the component axis {f-1, f}, across {f-1..f+1}.
+  int lo[3], hi[3];
+  const int fc[3] = {fi, fj, fk};
+  const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = fc[a] - 1;
+    hi[a] = (a == COMP) ? fc[a] : fc[a] + 1;
+    if (lo[a] < 0) lo[a] = 0;
+    if (hi[a] > nn[a] - 1) hi[a] = nn[a] - 1;
+  }
+  T acc = T(0), wsum = T(0);
+  auto visit = [&](uint32_t t) {
+    V3<T> l = g.local(load_pos(p, t));
+    T gx = l.x - (COMP == 0 ? T(0) : T(0.5));
+    T gy = l.y - (COMP == 1 ? T(0) : T(0.5));
+    T gz = l.z - (COMP == 2 ? T(0) : T(0.5));
+    int bi, bj, bk;
+    T ffx, ffy, ffz;
+    clamp_base(gx, fnx, bi, ffx);
+    clamp_base(gy, fny, bj, ffy);
+    clamp_base(gz, fnz, bk, ffz);
+    int di = fi - bi, dj = fj - bj, dk = fk - bk;
+    if ((unsigned)di > 1u || (unsigned)dj > 1u || (unsigned)dk > 1u) return;
+    T wx = di ? ffx : T(1) - ffx;
+    T wy = dj ? ffy : T(1) - ffy;
+    T wz = dk ? ffz : T(1) - ffz;
+    T w = wx * wy * wz;
+    acc += w * vel[t];
+    wsum += w;
+  };
+  if (!EXACT) {
+    for (int ck = lo[2]; ck <= hi[2]; ++ck)
+      for (int cj = lo[1]; cj <= hi[1]; ++cj)
+        for (int ci = lo[0]; ci <= hi[0]; ++ci) {
+          int64_t c = ((int64_t)ck * g.ny + cj) * g.nx + ci;
+          uint32_t e = cend[c];
+          for (uint32_t t = cstart[c]; t < e; ++t) visit(t);
+        }
+  } else {
+    // k-way merge of <= 18 rank-sorted segments, fully unrolled to stay in registers.
+    constexpr int S = (COMP == 0 ? 2 : 3) * (COMP == 1 ? 2 : 3) * (COMP == 2 ? 2 : 3);
+    uint32_t cur[S], end[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      int ci, cj, ck;
+      if (COMP == 0) {
+        ci = lo[0] + s % 2;
+        cj = lo[1] + (s / 2) % 3;
+        ck = lo[2] + s / 6;
+      } else if (COMP == 1) {
+        ci = lo[0] + s % 3;
+        cj = lo[1] + (s / 3) % 2;
+        ck = lo[2] + s / 6;
+      } else {
+        ci = lo[0] + s % 3;
+        cj = lo[1] + (s / 3) % 3;
+        ck = lo[2] + s / 9;
+      }
+      bool ok = ci <= hi[0] && cj <= hi[1] && ck <= hi[2];
+      int64_t c = ((int64_t)ck * g.ny + cj) * g.nx + ci;
+      cur[s] = ok ? cstart[c] : 0u;
+      end[s] = ok ? cend[c] : 0u;
+    }
+    while (true) {
+      uint32_t best = 0xffffffffu;
+      int bs = -1;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        if (cur[s] < end[s]) {
+          uint32_t r = p.rank[cur[s]];
+          if (r < best) {
+            best = r;
+            bs = s;
+          }
+        }
+      }
+      if (bs < 0) break;
+      uint32_t t = 0;
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (s == bs) {
+          t = cur[s];
+          cur[s] = t + 1;
+        }
+      visit(t);
+    }
+  }
+  out[f] = wsum > T(1e-12) ? acc / wsum : T(0);
+}
+
+template <class T>
+__global__ void k_add_const(T* a, int64_t n, T c) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] += c;
+}
+
+// ------------------------------------------------------------------ enforce_solid_faces (pressure.rs:243-281)
+template <class T, int COMP>
+__global__ void k_enforce_solid(GridD<T> g, T* __restrict__ a, const uint8_t* __restrict__ cells) {
+  const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (int64_t)fnx * fny * fnz) return;
+  int i = (int)(f % fnx), j = (int)((f / fnx) % fny), k = (int)(f / ((int64_t)fnx * fny));
+  auto solid = [&](int a_, int b_, int c_) {
+    if (a_ < 0 || b_ < 0 || c_ < 0 || a_ >= g.nx || b_ >= g.ny || c_ >= g.nz) return true;
+    return cells[((int64_t)c_ * g.ny + b_) * g.nx + a_] == kSolid;
+  };
+  bool s = COMP == 0 ? (solid(i - 1, j, k) || solid(i, j, k))
+                     : (COMP == 1 ? (solid(i, j - 1, k) || solid(i, j, k))
+                                  : (solid(i, j, k - 1) || solid(i, j, k)));
+  if (s) a[f] = T(0);
+}
+
+// ------------------------------------------------------------------ RHS + Laplacian flags
+// pressure.rs:152-223 with the volume-control source of solver.rs:392-423, written in
+// the sheared solve layout.  One thread per sheared slot; slots outside the grid get
+// b = 0 and flags = 0.
+template <class T>
+__global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* __restrict__ cells,
+                            const uint32_t* __restrict__ count, const double* __restrict__ srcarr,
+                            double inv_scale, int use_source, double rest, double k_vol,
+                            double unit, double cap, T* __restrict__ b,
+                            uint16_t* __restrict__ flags) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sh.size()) return;
+  int i = (int)(t % sh.nx);
+  int J = (int)((t / sh.nx) % sh.NJ);
+  int k = (int)(t / ((int64_t)sh.nx * sh.NJ));
+  int j = J - i;
+  if (j < 0 || j >= sh.ny) {
+    b[t] = T(0);
+    flags[t] = 0;
+    return;
+  }
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  auto cell = [&](int a_, int b_, int c_) -> uint8_t {
+    if (a_ < 0 || b_ < 0 || c_ < 0 || a_ >= nx || b_ >= ny || c_ >= nz) return kSolid;
+    return cells[((int64_t)c_ * ny + b_) * nx + a_];
+  };
+  int64_t c = ((int64_t)k * ny + j) * nx + i;
+  if (cells[c] != kFluid) {
+    b[t] = T(0);
+    flags[t] = 0;
+    return;
+  }
+  uint16_t fl = kFlFluid;
+  int diag = 0;
+  const int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k},
+                        {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+  const uint16_t bit[6] = {kFlPx, kFlMx, kFlPy, kFlMy, kFlPz, kFlMz};
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    uint8_t kind = cell(nb[d][0], nb[d][1], nb[d][2]);
+    if (kind != kSolid) ++diag;
+    if (kind == kFluid) fl |= bit[d];
+  }
+  fl |= (uint16_t)(diag << 8);
+  const int64_t ux = nx + 1;
+  T du = f.u[((int64_t)k * ny + j) * ux + i + 1] - f.u[((int64_t)k * ny + j) * ux + i];
+  T dv = f.v[((int64_t)k * (ny + 1) + j + 1) * nx + i] - f.v[((int64_t)k * (ny + 1) + j) * nx + i];
+  T dw = f.w[((int64_t)(k + 1) * ny + j) * nx + i] - f.w[((int64_t)k * ny + j) * nx + i];
+  T div = du + dv + dw;
+  T s = T(0);
+  if (srcarr) {
+    s = (T)srcarr[c];
+  } else if (use_source) {
+    double over = (double)count[c] / rest - 1.0;
+    if (over > 0.0) s = (T)fmin(k_vol * over * unit, cap);
+  }
+  b[t] = T(-inv_scale) * (div - s);
+  flags[t] = fl;
+}
+
+// ------------------------------------------------------------------ apply_a (pressure.rs:400-439)
+// Matrix-free: diag and the -1 couplings come from the flags.  In-bounds neighbours
+// without coupling contribute (+0 * x) exactly as the reference's stored 0.0 does.
+template <class T>
+__global__ void k_spmv(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ x,
+                       T* __restrict__ out, const PcgScalars* sc, int it) {
+  if (sc->done) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
+  if (t >= sh.size()) return;
+  uint16_t f = flags[t];
+  if (!(f & kFlFluid)) {
+    out[t] = T(0);
+    return;
+  }
+  int i = (int)(t % sh.nx);
+  int J = (int)((t / sh.nx) % sh.NJ);
+  int k = (int)(t / ((int64_t)sh.nx * sh.NJ));
+  int j = J - i;
+  const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
+  T v = T(fl_diag(f)) * x[t];
+  // (i+1, j) -> (i+1, J+1); (i, j+1) -> (i, J+1); (i, j, k+1) -> next plane.
+  if (i + 1 < sh.nx) v += ((f & kFlPx) ? T(-1) : T(0)) * x[t + row + 1];
+  if (i >= 1) v += ((f & kFlMx) ? T(-1) : T(0)) * x[t - row - 1];
+  if (j + 1 < sh.ny) v += ((f & kFlPy) ? T(-1) : T(0)) * x[t + row];
+  if (j >= 1) v += ((f & kFlMy) ? T(-1) : T(0)) * x[t - row];
+  if (k + 1 < sh.nz) v += ((f & kFlPz) ? T(-1) : T(0)) * x[t + plane];
+  if (k >= 1) v += ((f & kFlMz) ? T(-1) : T(0)) * x[t - plane];
+  out[t] = v;
+}
+
+// ------------------------------------------------------------------ dot (pressure.rs:588-603)
+// Exact mode: one wavefront per 4096-element chunk of the reference cell order; each
+// lane forms one product and lane order is folded sequentially into one accumulator,
+// so acc = ((0 + x0*y0) + x1*y1) + ... exactly as the reference chunk loop.
+__device__ inline double readlane_f64(double v, int lane) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, lane);
+  hi = __builtin_amdgcn_readlane(hi, lane);
+  return __hiloint2double(hi, lo);
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_dot_exact(Shear sh, const T* __restrict__ a,
+                                                  const T* __restrict__ b, double* partials,
+                                                  const PcgScalars* sc) {
+  if (sc->done) return;
+  const int64_t n = (int64_t)sh.nx * sh.ny * sh.nz;
+  const int64_t c0 = (int64_t)blockIdx.x * 4096;
+  const int lane = threadIdx.x;
+  double acc = 0.0;
+  for (int m = 0; m < 64; ++m) {
+    int64_t r = c0 + m * 64 + lane;
+    double prod = 0.0;
+    bool ok = r < n;
+    if (ok) {
+      int i = (int)(r % sh.nx);
+      int j = (int)((r / sh.nx) % sh.ny);
+      int k = (int)(r / ((int64_t)sh.nx * sh.ny));
+      int64_t s = sh.at(i, j, k);
+      prod = (double)a[s] * (double)b[s];
+    }
+    int live = (int)((n - (c0 + m * 64)) < 64 ? (n - (c0 + m * 64)) : 64);
+    if (live <= 0) break;
+    if (live == 64) {
+#pragma unroll
+      for (int l = 0; l < 64; ++l) acc += readlane_f64(prod, l);
+    } else {
+      for (int l = 0; l < live; ++l) acc += readlane_f64(prod, l);
+    }
+  }
+  if (lane == 0) partials[blockIdx.x] = acc;
+}
+
+// Fast mode: deterministic fixed tree in f64 (per-thread strided sum, then a wavefront
+// butterfly, then the waves of the block in order).
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dot_fast(Shear sh, const T* __restrict__ a,
+                                                  const T* __restrict__ b, double* partials,
+                                                  const PcgScalars* sc) {
+  if (sc->done) return;
+  __shared__ double red[4];
+  const int64_t n = sh.size();
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    acc += (double)a[t] * (double)b[t];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Combine partials and apply the scalar step of the PCG loop (pressure.rs:345-389).
+__global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblocks, int exact, int op,
+                                                    PcgScalars* sc, double tol) {
+  if (sc->done) return;
+  __shared__ double red[4];
+  double v;
+  if (op == kOpResidual || op == kOpRhsNorm) {
+    double m = 0.0;
+    for (int q = threadIdx.x; q < nblocks; q += 256) m = fmax(m, partials[q]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    v = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+  } else if (exact) {
+    // Iterator::sum over the chunk partials, in chunk order.
+    if (threadIdx.x != 0) return;
+    v = -0.0;
+    for (int q = 0; q < nblocks; ++q) v += partials[q];
+  } else {
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nblocks; q += 256) acc += partials[q];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    v = ((red[0] + red[1]) + red[2]) + red[3];
+  }
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case kOpSigma0:
+      sc->sigma = v;
+      break;
+    case kOpDenom:
+      sc->denom = v;
+      if (fabs(v) < 1e-30) {
+        sc->done = 1;
+      } else {
+        sc->alpha = sc->sigma / v;
+      }
+      break;
+    case kOpSigmaNew:
+      sc->beta = v / sc->sigma;
+      sc->sigma = v;
+      break;
+    case kOpResidual:
+      sc->residual = v;
+      if (v != v) sc->nan_seen = 1;
+      if (v <= tol) sc->done = 1;
+      break;
+    case kOpRhsNorm:
+      sc->residual = v;
+      sc->rhs_norm = v;
+      if (v <= tol) sc->done = 1;
+      break;
+  }
+}
+
+// inf_norm (pressure.rs:605-609): max |x| ignoring NaN (f64::max semantics).
+template <class T>
+__global__ void __launch_bounds__(256) k_maxabs(Shear sh, const T* __restrict__ a, double* partials,
+                                                const PcgScalars* sc) {
+  if (sc->done) return;
+  __shared__ double red[4];
+  const int64_t n = sh.size();
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    m = fmax(m, fabs((double)a[t]));
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+}
+
+// p += alpha s ; r -= alpha As (pressure.rs:371-375), fused with max|r| partials.
+template <class T>
+__global__ void __launch_bounds__(256) k_axpy_pr(Shear sh, T* __restrict__ p, T* __restrict__ r,
+                                                 const T* __restrict__ s, const T* __restrict__ as,
+                                                 const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  __shared__ double red[4];
+  const T alpha = (T)sc->alpha;
+  const int64_t n = sh.size();
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    p[t] += alpha * s[t];
+    T rv = r[t] - alpha * as[t];
+    r[t] = rv;
+    m = fmax(m, fabs((double)rv));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+}
+
+// s = z + beta s (pressure.rs:386-388).
+template <class T>
+__global__ void k_update_s(Shear sh, T* __restrict__ s, const T* __restrict__ z,
+                           const PcgScalars* sc) {
+  if (sc->done) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= sh.size()) return;
+  const T beta = (T)sc->beta;
+  s[t] = z[t] + beta * s[t];
+}
+
+// ------------------------------------------------------------------ MIC(0) (pressure.rs:442-570)
+// One launch per hyperplane sigma = J + k of the sheared layout: every cell of the
+// plane reads only cells of earlier planes, in the reference's operand order.
+//   build: e = diag - tx^2 - ty^2 - tz^2 - tau*(mx + my + mz); precon = 1/sqrt(e)
+//   fwd:   q = (((r - Tx) - Ty) - Tz) * precon, T = (plus[cm]*precon[cm])*q[cm]
+//   bwd:   z = (((q - Ux) - Uy) - Uz) * precon, U = (plus[c]*precon[c])*z[cp]
+// Non-coupled in-grid neighbours contribute (0*0)*0 = +0 in the reference, and
+// subtracting +0 is the identity, so they are skipped here.
+template <class T, int OP>
+__global__ void k_mic_plane(Shear sh, const uint16_t* __restrict__ flags, T* __restrict__ precon,
+                            const T* __restrict__ in, T* __restrict__ out, int sigma,
+                            const PcgScalars* sc) {
+  if (sc->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (OP == kMicBwd) k = sh.nz - 1 - k;
+  if (i >= sh.nx) return;
+  int J = OP == kMicBwd ? (sh.NJ - 1) - (sigma - (sh.nz - 1 - k)) : sigma - k;
+  if (J < 0 || J >= sh.NJ) return;
+  int j = J - i;
+  if (j < 0 || j >= sh.ny) return;
+  const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
+  const int64_t t = ((int64_t)k * sh.NJ + J) * sh.nx + i;
+  const uint16_t f = flags[t];
+  if (!(f & kFlFluid)) return;  // precon / q / z stay 0 (zero-initialised)
+  if (OP == kMicBuild) {
+    const double TAU = 0.97, SIG = 0.25;
+    double ax = 0, px = 0, axy = 0, axz = 0;
+    double ay = 0, py = 0, ayx = 0, ayz = 0;
+    double az = 0, pz = 0, azx = 0, azy = 0;
+    if (i >= 1) {
+      int64_t m = t - row - 1;
+      uint16_t g = flags[m];
+      ax = (f & kFlMx) ? -1.0 : 0.0;
+      px = precon[m];
+      axy = (g & kFlPy) ? -1.0 : 0.0;
+      axz = (g & kFlPz) ? -1.0 : 0.0;
+    }
+    if (j >= 1) {
+      int64_t m = t - row;
+      uint16_t g = flags[m];
+      ay = (f & kFlMy) ? -1.0 : 0.0;
+      py = precon[m];
+      ayx = (g & kFlPx) ? -1.0 : 0.0;
+      ayz = (g & kFlPz) ? -1.0 : 0.0;
+    }
+    if (k >= 1) {
+      int64_t m = t - plane;
+      uint16_t g = flags[m];
+      az = (f & kFlMz) ? -1.0 : 0.0;
+      pz = precon[m];
+      azx = (g & kFlPx) ? -1.0 : 0.0;
+      azy = (g & kFlPy) ? -1.0 : 0.0;
+    }
+    double tx = ax * px, ty = ay * py, tz = az * pz;
+    double diag = (double)fl_diag(f);
+    double e = diag - tx * tx - ty * ty - tz * tz -
+               TAU * (ax * (axy + axz) * px * px + ay * (ayx + ayz) * py * py +
+                      az * (azx + azy) * pz * pz);
+    if (e < SIG * diag) e = diag;
+    precon[t] = e > 0.0 ? (T)(1.0 / sqrt(e)) : T(0);
+  } else if (OP == kMicFwd) {
+    T v = in[t];
+    if (f & kFlMx) v -= (-precon[t - row - 1]) * out[t - row - 1];
+    if (f & kFlMy) v -= (-precon[t - row]) * out[t - row];
+    if (f & kFlMz) v -= (-precon[t - plane]) * out[t - plane];
+    out[t] = v * precon[t];
+  } else {
+    const T pc = precon[t];
+    T v = in[t];
+    if (f & kFlPx) v -= (-pc) * out[t + row + 1];
+    if (f & kFlPy) v -= (-pc) * out[t + row];
+    if (f & kFlPz) v -= (-pc) * out[t + plane];
+    out[t] = v * pc;
+  }
+}
+
+// ------------------------------------------------------------------ apply_pressure_gradient (pressure.rs:285-332)
+template <class T, int COMP>
+__global__ void k_gradient(GridD<T> g, Shear sh, T* __restrict__ a, const uint8_t* __restrict__ cells,
+                           const T* __restrict__ p, T coef) {
+  const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (int64_t)fnx * fny * fnz) return;
+  int i = (int)(f % fnx), j = (int)((f / fnx) % fny), k = (int)(f / ((int64_t)fnx * fny));
+  int li = i - (COMP == 0), lj = j - (COMP == 1), lk = k - (COMP == 2);
+  if (li < 0 || lj < 0 || lk < 0) return;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  uint8_t l = cells[((int64_t)lk * g.ny + lj) * g.nx + li];
+  uint8_t r = cells[((int64_t)k * g.ny + j) * g.nx + i];
+  if (l == kSolid || r == kSolid) return;
+  if (l == kFluid || r == kFluid) a[f] -= coef * (p[sh.at(i, j, k)] - p[sh.at(li, lj, lk)]);
+}
+
+template <class T>
+__global__ void k_unshear(Shear sh, const T* __restrict__ p, T* __restrict__ out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)sh.nx * sh.ny * sh.nz) return;
+  int i = (int)(c % sh.nx), j = (int)((c / sh.nx) % sh.ny), k = (int)(c / ((int64_t)sh.nx * sh.ny));
+  out[c] = p[sh.at(i, j, k)];
+}
+
+// ------------------------------------------------------------------ extrapolate (solver.rs:608-647, 960-1009)
+template <class T, int COMP>
+__global__ void k_extrap_init(GridD<T> g, const uint8_t* __restrict__ cells, uint8_t* known) {
+  const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (int64_t)fnx * fny * fnz) return;
+  int i = (int)(f % fnx), j = (int)((f / fnx) % fny), k = (int)(f / ((int64_t)fnx * fny));
+  auto fluid = [&](int a_, int b_, int c_) {
+    return cells[((int64_t)c_ * g.ny + b_) * g.nx + a_] == kFluid;
+  };
+  bool v;
+  if (COMP == 0) v = (i > 0 && fluid(i - 1, j, k)) || (i < g.nx && fluid(i, j, k));
+  else if (COMP == 1) v = (j > 0 && fluid(i, j - 1, k)) || (j < g.ny && fluid(i, j, k));
+  else v = (k > 0 && fluid(i, j, k - 1)) || (k < g.nz && fluid(i, j, k));
+  known[f] = v;
+}
+
+template <class T>
+__global__ void k_extrap_iter(int nx, int ny, int nz, const T* __restrict__ src, T* __restrict__ dst,
+                              const uint8_t* __restrict__ ksrc, uint8_t* __restrict__ kdst) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)nx * ny * nz) return;
+  if (ksrc[c]) {
+    dst[c] = src[c];
+    kdst[c] = 1;
+    return;
+  }
+  int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((int64_t)nx * ny));
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  T sum = T(0);
+  int cnt = 0;
+  if (i > 0 && ksrc[c - 1]) { sum += src[c - 1]; ++cnt; }
+  if (i + 1 < nx && ksrc[c + 1]) { sum += src[c + 1]; ++cnt; }
+  if (j > 0 && ksrc[c - sy]) { sum += src[c - sy]; ++cnt; }
+  if (j + 1 < ny && ksrc[c + sy]) { sum += src[c + sy]; ++cnt; }
+  if (k > 0 && ksrc[c - sz]) { sum += src[c - sz]; ++cnt; }
+  if (k + 1 < nz && ksrc[c + sz]) { sum += src[c + sz]; ++cnt; }
+  if (cnt > 0) {
+    dst[c] = sum / T(cnt);
+    kdst[c] = 1;
+  } else {
+    dst[c] = src[c];
+    kdst[c] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ G2P (solver.rs:650-677)
+template <class T>
+__global__ void k_g2p(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  V3<T> x = load_pos(p, t);
+  V3<T> pic = velocity_at(g, f, x);
+  V3<T> l = g.local(x);
+  T dvx = trilinear_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5));
+  T dvy = trilinear_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5));
+  T dvz = trilinear_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z);
+  V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
+  V3<T> nv = fv * flip + pic * (T(1) - flip);
+  p.vx[t] = nv.x;
+  p.vy[t] = nv.y;
+  p.vz[t] = nv.z;
+}
+
+// ------------------------------------------------------------------ advect (solver.rs:693-711)
+template <class T>
+__global__ void k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  V3<T> p0 = load_pos(p, t);
+  V3<T> v0 = velocity_at(g, f, p0);
+  V3<T> mid = p0 + v0 * (T(0.5) * dt);
+  V3<T> vm = velocity_at(g, f, mid);
+  V3<T> st = vm * dt;
+  T len = len3(st);
+  if (len > g.dx) st = st * (g.dx / len);
+  V3<T> q = project_inside(sdf, g.dx, p0 + st);
+  V3<T> r = sdf.sample(q) <= T(0) ? q : p0;
+  p.px[t] = r.x;
+  p.py[t] = r.y;
+  p.pz[t] = r.z;
+}
+
+// ------------------------------------------------------------------ enforce_particle_bounds (solver.rs:736-755)
+template <class T>
+__global__ void k_bounds(Parts<T> p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
+                         uint32_t* cull, unsigned long long* counters) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  V3<T> q = load_pos(p, t);
+  bool k_ = false;
+  int why = 0;  // 1 drained, 2 culled
+  if (!sdf.in_bounds(q)) {
+    why = 2;
+  } else if (outlet.contains(q)) {
+    why = 1;
+  } else {
+    k_ = sdf.sample(q) < T(0.5) * sdf.dx;
+    if (!k_) why = 2;
+  }
+  keep[t] = k_;
+  cull[t] = !k_;
+  if (why) atomicAdd(&counters[why - 1], 1ull);
+}
+
+template <class T>
+__global__ void k_compact(Parts<T> s, Parts<T> d, const uint32_t* __restrict__ keep,
+                          const uint32_t* __restrict__ pos, int64_t n, uint32_t* culled_ranks) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t q = pos[t];
+  if (keep[t]) {
+    d.px[q] = s.px[t];
+    d.py[q] = s.py[t];
+    d.pz[q] = s.pz[t];
+    d.vx[q] = s.vx[t];
+    d.vy[q] = s.vy[t];
+    d.vz[q] = s.vz[t];
+    d.rank[q] = s.rank[t];
+  } else {
+    culled_ranks[(uint32_t)t - q] = s.rank[t];
+  }
+}
+
+// The reference compacts in place (particles.rs:38-49), so a survivor's index drops by
+// the number of removed particles before it.
+__global__ void k_rank_fix(uint32_t* rank, int64_t n, const uint32_t* __restrict__ culled,
+                           int64_t m) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t r = rank[t];
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (culled[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  rank[t] = r - (uint32_t)lo;
+}
+
+// ------------------------------------------------------------------ redistribute (solver.rs:454-557)
+template <class T>
+__global__ void k_hist(Parts<T> p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t c = g.cell_of(load_pos(p, t));
+  cellkey[t] = (int32_t)c;
+  if (c >= 0) atomicAdd(&count[c], 1u);
+}
+
+__global__ void k_active_flags(const int32_t* __restrict__ cellkey, const uint32_t* __restrict__ count,
+                               int64_t n, uint32_t cap, uint32_t* flag) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int32_t c = cellkey[t];
+  flag[t] = (c >= 0 && count[c] > cap) ? 1u : 0u;
+}
+
+__global__ void k_active_compact(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                 int64_t n, const uint32_t* __restrict__ rank, uint32_t* act_rank,
+                                 uint32_t* act_slot) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !flag[t]) return;
+  uint32_t q = pos[t];
+  act_rank[q] = rank[t];
+  act_slot[q] = (uint32_t)t;
+}
+
+// The order-dependent decision pass: visits the particles of over-cap cells in
+// reference order and mutates `count` as it goes, exactly as the reference loop does.
+// Particles outside over-cap cells can never move during a sweep (a receiver only
+// rises to the cap), so this set is the whole sweep.  Single thread on purpose.
+__global__ void k_redist_decide(const uint32_t* __restrict__ act, int64_t nact,
+                                const int32_t* __restrict__ cellkey, uint32_t* count,
+                                const uint8_t* __restrict__ mask, int nx, int ny, int nz,
+                                uint32_t cap, double gx, double gy, double gz, int32_t* dest) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  // dir . gdir for -x,+x,-y,+y,-z,+z (math.rs:42-44 with the unit dirs of solver.rs:492-509)
+  double dn[6];
+  dn[0] = -1.0 * gx + 0.0 * gy + 0.0 * gz;
+  dn[1] = 1.0 * gx + 0.0 * gy + 0.0 * gz;
+  dn[2] = 0.0 * gx + -1.0 * gy + 0.0 * gz;
+  dn[3] = 0.0 * gx + 1.0 * gy + 0.0 * gz;
+  dn[4] = 0.0 * gx + 0.0 * gy + -1.0 * gz;
+  dn[5] = 0.0 * gx + 0.0 * gy + 1.0 * gz;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  for (int64_t a = 0; a < nact; ++a) {
+    uint32_t slot = act[a];
+    int64_t c = cellkey[slot];
+    dest[a] = -1;
+    if (count[c] <= cap) continue;
+    int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+    int64_t nb[6];
+    double dd[6];
+    int nn = 0;
+    if (i > 0) { nb[nn] = c - 1; dd[nn++] = dn[0]; }
+    if (i + 1 < nx) { nb[nn] = c + 1; dd[nn++] = dn[1]; }
+    if (j > 0) { nb[nn] = c - sy; dd[nn++] = dn[2]; }
+    if (j + 1 < ny) { nb[nn] = c + sy; dd[nn++] = dn[3]; }
+    if (k > 0) { nb[nn] = c - sz; dd[nn++] = dn[4]; }
+    if (k + 1 < nz) { nb[nn] = c + sz; dd[nn++] = dn[5]; }
+    bool have = false;
+    int64_t best = 0;
+    double best_down = -INFINITY;
+    uint32_t best_room = 0;
+    for (int q = 0; q < nn; ++q) {
+      int64_t nc = nb[q];
+      if (!mask[nc] || count[nc] >= cap) continue;
+      uint32_t room = cap - count[nc];
+      bool better = !have || dd[q] > best_down || (dd[q] == best_down && room > best_room);
+      if (better) {
+        have = true;
+        best = nc;
+        best_down = dd[q];
+        best_room = room;
+      }
+    }
+    if (have) {
+      count[c] -= 1;
+      count[best] += 1;
+      dest[a] = (int32_t)best;
+    }
+  }
+}
+
+template <class T>
+__global__ void k_redist_apply(Parts<T> p, const uint32_t* __restrict__ act,
+                               const int32_t* __restrict__ dest, int64_t nact, GridD<T> g,
+                               CFacesD<T> f, SdfD<T> sdf) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact) return;
+  int32_t nc = dest[a];
+  if (nc < 0) return;
+  uint32_t slot = act[a];
+  uint64_t key = p.rank[slot];
+  int ci = nc % g.nx, cj = (nc / g.nx) % g.ny, ck = nc / (g.nx * g.ny);
+  V3<T> jit = mk3((T)jitter_mix(key * 3), (T)jitter_mix(key * 3 + 1), (T)jitter_mix(key * 3 + 2));
+  V3<T> q = g.cell_center(ci, cj, ck) + jit * g.dx;
+  q = project_inside(sdf, g.dx, q);
+  V3<T> v = velocity_at(g, f, q);
+  p.px[slot] = q.x;
+  p.py[slot] = q.y;
+  p.pz[slot] = q.z;
+  p.vx[slot] = v.x;
+  p.vy[slot] = v.y;
+  p.vz[slot] = v.z;
+}
+
+// ------------------------------------------------------------------ CFL (grid.rs:202-206)
+template <class T>
+__global__ void k_maxabs_faces(const T* __restrict__ a, int64_t n, unsigned long long* out) {
+  __shared__ double red[4];
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs((double)a[t]));
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ------------------------------------------------------------------ host <-> device particle layout
+template <class T>
+__global__ void k_to_aos(Parts<T> p, int64_t n, double* pos, double* vel) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t r = p.rank[t];
+  if ((int64_t)r >= n) return;  // ranks are dense 0..n-1; never scatter outside the buffer
+  pos[3 * (int64_t)r] = p.px[t];
+  pos[3 * (int64_t)r + 1] = p.py[t];
+  pos[3 * (int64_t)r + 2] = p.pz[t];
+  vel[3 * (int64_t)r] = p.vx[t];
+  vel[3 * (int64_t)r + 1] = p.vy[t];
+  vel[3 * (int64_t)r + 2] = p.vz[t];
+}
+
+template <class T>
+__global__ void k_from_aos(Parts<T> p, int64_t n, int64_t base, const double* pos,
+                           const double* vel) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t d = base + t;
+  p.px[d] = (T)pos[3 * t];
+  p.py[d] = (T)pos[3 * t + 1];
+  p.pz[d] = (T)pos[3 * t + 2];
+  p.vx[d] = vel ? (T)vel[3 * t] : T(0);
+  p.vy[d] = vel ? (T)vel[3 * t + 1] : T(0);
+  p.vz[d] = vel ? (T)vel[3 * t + 2] : T(0);
+  p.rank[d] = (uint32_t)d;
+}
+
+// Initial fill of the benchmark cavities: `per_cell` particles in each listed cell at
+// centre + jitter, jitter from the same splitmix64 finaliser the reference uses for
+// redistribution, keyed by (seed, cell, slot).
+template <class T>
+__global__ void k_seed_fill(Parts<T> p, int64_t base, GridD<T> g, const uint32_t* __restrict__ cells,
+                            int64_t ncells, uint32_t per_cell, uint64_t seed, T vx, T vy, T vz) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * per_cell) return;
+  int64_t q = t / per_cell;
+  uint32_t c = cells[q];
+  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  uint64_t key = seed * 0x100000000ull + (uint64_t)t;
+  double jx = jitter_mix(key * 3) / 0.6 * 0.9999, jy = jitter_mix(key * 3 + 1) / 0.6 * 0.9999,
+         jz = jitter_mix(key * 3 + 2) / 0.6 * 0.9999;
+  V3<T> x = g.cell_center(ci, cj, ck) + mk3((T)jx, (T)jy, (T)jz) * g.dx;
+  int64_t d = base + t;
+  p.px[d] = x.x;
+  p.py[d] = x.y;
+  p.pz[d] = x.z;
+  p.vx[d] = vx;
+  p.vy[d] = vy;
+  p.vz[d] = vz;
+  p.rank[d] = (uint32_t)d;
+}
+
+template <class T>
+__global__ void k_static_cls(GridD<T> g, const uint8_t* __restrict__ mask, BoxD<T> outlet,
+                             uint8_t* out) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)g.nx * g.ny * g.nz) return;
+  int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  bool o = outlet.contains(g.cell_center(i, j, k));
+  out[c] = o ? kAir : (!mask[c] ? kSolid : kAir);
+}
+
+// solver.rs:194-199: the static cavity mask from the SDF at each cell centre.
+template <class T>
+__global__ void k_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)g.nx * g.ny * g.nz) return;
+  int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((int64_t)g.nx * g.ny));
+  mask[c] = sdf.sample(g.cell_center(i, j, k)) < T(0);
+}
+
+}  // namespace
+
+// ================================================================== launch wrappers
+template <class T>
+void launch_keys(const Parts<T>& p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
+                 uint32_t* vals, bool exact, hipStream_t st) {
+  if (n <= 0) return;
+  k_keys<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, keys, keys64, vals, exact ? 1 : 0);
+}
+template <class T>
+void launch_gather_parts(const Parts<T>& s, const Parts<T>& d, const uint32_t* perm, int64_t n,
+                         hipStream_t st) {
+  if (n <= 0) return;
+  k_gather_parts<T><<<grid_for(n), kBlock, 0, st>>>(s, d, perm, n);
+}
+void launch_cell_ranges(const uint32_t* keys, const uint64_t* keys64, int64_t n, int64_t ncells,
+                        uint32_t* cstart, uint32_t* cend, hipStream_t st) {
+  (void)ncells;
+  if (n <= 0) return;
+  k_cell_ranges<<<grid_for(n), kBlock, 0, st>>>(keys, keys64, n, cstart, cend);
+}
+template <class T>
+void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
+                     const Parts<T>& p, GridD<T> g, BoxD<T> outlet, uint8_t* cells,
+                     uint32_t* count, hipStream_t st) {
+  int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+  k_classify<T><<<grid_for(nc), kBlock, 0, st>>>(static_cls, cstart, cend, p, g, outlet, cells,
+                                                  count);
+}
+template <class T>
+void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
+                const uint32_t* cend, T* out, bool exact, hipStream_t st) {
+  int64_t nf = (int64_t)(g.nx + (comp == 0)) * (g.ny + (comp == 1)) * (g.nz + (comp == 2));
+  unsigned gr = grid_for(nf);
+  if (exact) {
+    if (comp == 0) k_p2g<T, 0, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+    if (comp == 1) k_p2g<T, 1, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+    if (comp == 2) k_p2g<T, 2, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+  } else {
+    if (comp == 0) k_p2g<T, 0, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+    if (comp == 1) k_p2g<T, 1, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+    if (comp == 2) k_p2g<T, 2, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
+  }
+}
+template <class T>
+void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st) {
+  k_add_const<T><<<grid_for(n), kBlock, 0, st>>>(a, n, c);
+}
+template <class T>
+void launch_enforce_solid(GridD<T> g, FacesD<T> f, const uint8_t* cells, hipStream_t st) {
+  int64_t nu = (int64_t)(g.nx + 1) * g.ny * g.nz, nv = (int64_t)g.nx * (g.ny + 1) * g.nz,
+          nw = (int64_t)g.nx * g.ny * (g.nz + 1);
+  k_enforce_solid<T, 0><<<grid_for(nu), kBlock, 0, st>>>(g, f.u, cells);
+  k_enforce_solid<T, 1><<<grid_for(nv), kBlock, 0, st>>>(g, f.v, cells);
+  k_enforce_solid<T, 2><<<grid_for(nw), kBlock, 0, st>>>(g, f.w, cells);
+}
+template <class T>
+void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
+                      const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
+                      double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
+                      hipStream_t st) {
+  k_rhs_flags<T><<<grid_for(sh.size()), kBlock, 0, st>>>(g, sh, f, cells, count, srcarr, inv_scale,
+                                                          use_source, rest, k_vol, unit, cap, b,
+                                                          flags);
+}
+template <class T>
+void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
+                 hipStream_t st) {
+  k_spmv<T><<<grid_for(sh.size()), kBlock, 0, st>>>(sh, flags, x, out, sc, it);
+}
+int dot_blocks(Shear sh, bool exact) {
+  if (exact) {
+    int64_t n = (int64_t)sh.nx * sh.ny * sh.nz;
+    return (int)((n + 4095) / 4096);
+  }
+  int64_t b = (sh.size() + 256 * 8 - 1) / (256 * 8);
+  if (b > 2048) b = 2048;
+  return (int)(b < 1 ? 1 : b);
+}
+int maxabs_blocks(Shear sh) { return dot_blocks(sh, false); }
+template <class T>
+void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,
+                         const PcgScalars* sc, hipStream_t st) {
+  if (exact)
+    k_dot_exact<T><<<nblocks, 64, 0, st>>>(sh, a, b, partials, sc);
+  else
+    k_dot_fast<T><<<nblocks, 256, 0, st>>>(sh, a, b, partials, sc);
+}
+void launch_dot_finish(double* partials, int nblocks, bool exact, int op, PcgScalars* sc,
+                       double tol, hipStream_t st) {
+  k_dot_finish<<<1, 256, 0, st>>>(partials, nblocks, exact ? 1 : 0, op, sc, tol);
+}
+template <class T>
+void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
+                            const PcgScalars* sc, hipStream_t st) {
+  k_maxabs<T><<<nblocks, 256, 0, st>>>(sh, a, partials, sc);
+}
+template <class T>
+void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScalars* sc,
+                    double* partials, int nblocks, hipStream_t st) {
+  k_axpy_pr<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials);
+}
+template <class T>
+void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st) {
+  k_update_s<T><<<grid_for(sh.size()), kBlock, 0, st>>>(sh, s, z, sc);
+}
+template <class T>
+void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
+                      int sigma, const PcgScalars* sc, hipStream_t st) {
+  dim3 gr((unsigned)((sh.nx + 63) / 64), (unsigned)sh.nz);
+  if (op == kMicBuild) k_mic_plane<T, kMicBuild><<<gr, 64, 0, st>>>(sh, flags, precon, in, out, sigma, sc);
+  if (op == kMicFwd) k_mic_plane<T, kMicFwd><<<gr, 64, 0, st>>>(sh, flags, precon, in, out, sigma, sc);
+  if (op == kMicBwd) k_mic_plane<T, kMicBwd><<<gr, 64, 0, st>>>(sh, flags, precon, in, out, sigma, sc);
+}
+template <class T>
+void launch_gradient(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, const T* p, T coef,
+                     hipStream_t st) {
+  int64_t nu = (int64_t)(g.nx + 1) * g.ny * g.nz, nv = (int64_t)g.nx * (g.ny + 1) * g.nz,
+          nw = (int64_t)g.nx * g.ny * (g.nz + 1);
+  k_gradient<T, 0><<<grid_for(nu), kBlock, 0, st>>>(g, sh, f.u, cells, p, coef);
+  k_gradient<T, 1><<<grid_for(nv), kBlock, 0, st>>>(g, sh, f.v, cells, p, coef);
+  k_gradient<T, 2><<<grid_for(nw), kBlock, 0, st>>>(g, sh, f.w, cells, p, coef);
+}
+template <class T>
+void launch_unshear(Shear sh, const T* p, T* out, hipStream_t st) {
+  k_unshear<T><<<grid_for((int64_t)sh.nx * sh.ny * sh.nz), kBlock, 0, st>>>(sh, p, out);
+}
+template <class T>
+void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st) {
+  int64_t nf = (int64_t)(g.nx + (comp == 0)) * (g.ny + (comp == 1)) * (g.nz + (comp == 2));
+  if (comp == 0) k_extrap_init<T, 0><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+  if (comp == 1) k_extrap_init<T, 1><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+  if (comp == 2) k_extrap_init<T, 2><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+}
+template <class T>
+void launch_extrap_iter(int nx, int ny, int nz, const T* src, T* dst, const uint8_t* ksrc,
+                        uint8_t* kdst, hipStream_t st) {
+  k_extrap_iter<T><<<grid_for((int64_t)nx * ny * nz), kBlock, 0, st>>>(nx, ny, nz, src, dst, ksrc,
+                                                                        kdst);
+}
+template <class T>
+void launch_g2p(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> saved, T flip,
+                hipStream_t st) {
+  if (n <= 0) return;
+  k_g2p<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, f, saved, flip);
+}
+template <class T>
+void launch_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  k_advect<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, f, sdf, dt);
+}
+template <class T>
+void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
+                   uint32_t* cull, unsigned long long* counters, hipStream_t st) {
+  if (n <= 0) return;
+  k_bounds<T><<<grid_for(n), kBlock, 0, st>>>(p, n, sdf, outlet, keep, cull, counters);
+}
+template <class T>
+void launch_compact(const Parts<T>& s, const Parts<T>& d, const uint32_t* keep, const uint32_t* pos,
+                    int64_t n, uint32_t* culled_ranks, hipStream_t st) {
+  if (n <= 0) return;
+  k_compact<T><<<grid_for(n), kBlock, 0, st>>>(s, d, keep, pos, n, culled_ranks);
+}
+void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* culled, int64_t m, hipStream_t st) {
+  if (n <= 0 || m <= 0) return;
+  k_rank_fix<<<grid_for(n), kBlock, 0, st>>>(rank, n, culled, m);
+}
+template <class T>
+void launch_hist(const Parts<T>& p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count,
+                 hipStream_t st) {
+  if (n <= 0) return;
+  k_hist<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, cellkey, count);
+}
+void launch_active_flags(const int32_t* cellkey, const uint32_t* count, int64_t n, uint32_t cap,
+                         uint32_t* flag, hipStream_t st) {
+  if (n <= 0) return;
+  k_active_flags<<<grid_for(n), kBlock, 0, st>>>(cellkey, count, n, cap, flag);
+}
+void launch_active_compact(const uint32_t* flag, const uint32_t* pos, int64_t n,
+                           const uint32_t* rank, uint32_t* act_rank, uint32_t* act_slot,
+                           hipStream_t st) {
+  if (n <= 0) return;
+  k_active_compact<<<grid_for(n), kBlock, 0, st>>>(flag, pos, n, rank, act_rank, act_slot);
+}
+void launch_redist_decide(const uint32_t* act, int64_t nact, const int32_t* cellkey,
+                          uint32_t* count, const uint8_t* mask, int nx, int ny, int nz, uint32_t cap,
+                          double gx, double gy, double gz, int32_t* dest, hipStream_t st) {
+  if (nact <= 0) return;
+  k_redist_decide<<<1, 1, 0, st>>>(act, nact, cellkey, count, mask, nx, ny, nz, cap, gx, gy, gz,
+                                   dest);
+}
+template <class T>
+void launch_redist_apply(const Parts<T>& p, const uint32_t* act, const int32_t* dest, int64_t nact,
+                         GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st) {
+  if (nact <= 0) return;
+  k_redist_apply<T><<<grid_for(nact), kBlock, 0, st>>>(p, act, dest, nact, g, f, sdf);
+}
+template <class T>
+void launch_maxabs_faces(const T* a, int64_t n, unsigned long long* out, hipStream_t st) {
+  int64_t b = (n + 256 * 8 - 1) / (256 * 8);
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  k_maxabs_faces<T><<<(unsigned)b, 256, 0, st>>>(a, n, out);
+}
+template <class T>
+void launch_to_aos(const Parts<T>& p, int64_t n, double* pos, double* vel, hipStream_t st) {
+  if (n <= 0) return;
+  k_to_aos<T><<<grid_for(n), kBlock, 0, st>>>(p, n, pos, vel);
+}
+template <class T>
+void launch_from_aos(const Parts<T>& p, int64_t n, int64_t base, const double* pos,
+                     const double* vel, hipStream_t st) {
+  if (n <= 0) return;
+  k_from_aos<T><<<grid_for(n), kBlock, 0, st>>>(p, n, base, pos, vel);
+}
+template <class T>
+void launch_seed_fill(const Parts<T>& p, int64_t base, GridD<T> g, const uint8_t* /*mask*/,
+                      const uint32_t* cells, int64_t ncells, uint32_t per_cell, uint64_t seed, T vx,
+                      T vy, T vz, hipStream_t st) {
+  int64_t n = ncells * per_cell;
+  if (n <= 0) return;
+  k_seed_fill<T><<<grid_for(n), kBlock, 0, st>>>(p, base, g, cells, ncells, per_cell, seed, vx, vy,
+                                                  vz);
+}
+template <class T>
+void launch_static_cls(GridD<T> g, const uint8_t* mask, BoxD<T> outlet, uint8_t* out, hipStream_t st) {
+  k_static_cls<T><<<grid_for((int64_t)g.nx * g.ny * g.nz), kBlock, 0, st>>>(g, mask, outlet, out);
+}
+template <class T>
+void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) {
+  k_cavity_mask<T><<<grid_for((int64_t)g.nx * g.ny * g.nz), kBlock, 0, st>>>(g, sdf, mask);
+}
+
+// ------------------------------------------------------------------ instantiations
+#define SDR_INST(T)                                                                                \
+  template void launch_keys<T>(const Parts<T>&, int64_t, GridD<T>, uint32_t*, uint64_t*, uint32_t*, \
+                               bool, hipStream_t);                                                 \
+  template void launch_gather_parts<T>(const Parts<T>&, const Parts<T>&, const uint32_t*, int64_t,  \
+                                       hipStream_t);                                               \
+  template void launch_classify<T>(const uint8_t*, const uint32_t*, const uint32_t*, const Parts<T>&, \
+                                   GridD<T>, BoxD<T>, uint8_t*, uint32_t*, hipStream_t);           \
+  template void launch_p2g<T>(int, GridD<T>, const Parts<T>&, const uint32_t*, const uint32_t*, T*, \
+                              bool, hipStream_t);                                                  \
+  template void launch_axpy_const<T>(T*, int64_t, T, hipStream_t);                                 \
+  template void launch_enforce_solid<T>(GridD<T>, FacesD<T>, const uint8_t*, hipStream_t);         \
+  template void launch_rhs_flags<T>(GridD<T>, Shear, CFacesD<T>, const uint8_t*, const uint32_t*,  \
+                                    const double*, double, int, double, double, double, double,    \
+                                    T*, uint16_t*, hipStream_t);                                   \
+  template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
+                               hipStream_t);                                                       \
+  template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
+                                       const PcgScalars*, hipStream_t);                            \
+  template void launch_maxabs_partials<T>(Shear, const T*, double*, int, const PcgScalars*,        \
+                                          hipStream_t);                                            \
+  template void launch_axpy_pr<T>(Shear, T*, T*, const T*, const T*, const PcgScalars*, double*,   \
+                                  int, hipStream_t);                                               \
+  template void launch_update_s<T>(Shear, T*, const T*, const PcgScalars*, hipStream_t);           \
+  template void launch_mic_plane<T>(int, Shear, const uint16_t*, T*, const T*, T*, int,            \
+                                    const PcgScalars*, hipStream_t);                               \
+  template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \
+                                   hipStream_t);                                                   \
+  template void launch_unshear<T>(Shear, const T*, T*, hipStream_t);                               \
+  template void launch_extrap_init<T>(GridD<T>, int, const uint8_t*, uint8_t*, hipStream_t);       \
+  template void launch_extrap_iter<T>(int, int, int, const T*, T*, const uint8_t*, uint8_t*,       \
+                                      hipStream_t);                                                \
+  template void launch_g2p<T>(const Parts<T>&, int64_t, GridD<T>, CFacesD<T>, CFacesD<T>, T,       \
+                              hipStream_t);                                                        \
+  template void launch_advect<T>(const Parts<T>&, int64_t, GridD<T>, CFacesD<T>, SdfD<T>, T,       \
+                                 hipStream_t);                                                     \
+  template void launch_bounds<T>(const Parts<T>&, int64_t, SdfD<T>, BoxD<T>, uint32_t*, uint32_t*, \
+                                 unsigned long long*, hipStream_t);                                \
+  template void launch_compact<T>(const Parts<T>&, const Parts<T>&, const uint32_t*,               \
+                                  const uint32_t*, int64_t, uint32_t*, hipStream_t);               \
+  template void launch_hist<T>(const Parts<T>&, int64_t, GridD<T>, int32_t*, uint32_t*,            \
+                               hipStream_t);                                                       \
+  template void launch_redist_apply<T>(const Parts<T>&, const uint32_t*, const int32_t*, int64_t,  \
+                                       GridD<T>, CFacesD<T>, SdfD<T>, hipStream_t);                \
+  template void launch_maxabs_faces<T>(const T*, int64_t, unsigned long long*, hipStream_t);       \
+  template void launch_to_aos<T>(const Parts<T>&, int64_t, double*, double*, hipStream_t);         \
+  template void launch_from_aos<T>(const Parts<T>&, int64_t, int64_t, const double*, const double*, \
+                                   hipStream_t);                                                   \
+  template void launch_seed_fill<T>(const Parts<T>&, int64_t, GridD<T>, const uint8_t*,            \
+                                    const uint32_t*, int64_t, uint32_t, uint64_t, T, T, T,         \
+                                    hipStream_t);                                                  \
+  template void launch_static_cls<T>(GridD<T>, const uint8_t*, BoxD<T>, uint8_t*, hipStream_t);    \
+  template void launch_cavity_mask<T>(GridD<T>, SdfD<T>, uint8_t*, hipStream_t);
+
+SDR_INST(double)
+SDR_INST(float)
+
+}  // namespace sdr

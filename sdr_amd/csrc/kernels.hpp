@@ -1,0 +1,150 @@
+// kernels.hpp — host-side launch interface of the substep kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace sdr {
+
+// Particle storage: structure of arrays, kept sorted by cell between the sort in
+// `classify` and the first kernel that moves particles.  `rank` is the particle's
+// index in the reference's `ParticleSet` order (particles.rs:13-16).
+template <class T>
+struct Parts {
+  T *px, *py, *pz, *vx, *vy, *vz;
+  uint32_t* rank;
+};
+
+// Scalars of the PCG loop (pressure.rs:341-397), resident on the device.
+struct PcgScalars {
+  double sigma;
+  double denom;
+  double alpha;
+  double beta;
+  double residual;
+  double rhs_norm;
+  int32_t iters;
+  int32_t done;
+  int32_t nan_seen;
+  int32_t pad;
+};
+
+enum DotOp : int32_t {
+  kOpSigma0 = 0,    // sigma = dot(z, r)
+  kOpDenom = 1,     // denom = dot(s, As); break if |denom| < 1e-30; alpha = sigma/denom
+  kOpSigmaNew = 2,  // beta = dot(z, r)/sigma; sigma = dot(z, r)
+  kOpResidual = 3,  // residual = max|r|; done if residual <= tol   (max partials)
+  kOpRhsNorm = 4,   // residual = max|b|; done (0 iterations) if <= tol
+};
+
+enum MicOp : int32_t { kMicBuild = 0, kMicFwd = 1, kMicBwd = 2 };
+
+struct SolveCfg {
+  double tol;
+  double rho, dt, dx;
+};
+
+// ---- launches (all asynchronous on `st`)
+template <class T>
+void launch_keys(const Parts<T>& p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
+                 uint32_t* vals, bool exact, hipStream_t st);
+template <class T>
+void launch_gather_parts(const Parts<T>& src, const Parts<T>& dst, const uint32_t* perm, int64_t n,
+                         hipStream_t st);
+void launch_cell_ranges(const uint32_t* keys, const uint64_t* keys64, int64_t n, int64_t ncells,
+                        uint32_t* cstart, uint32_t* cend, hipStream_t st);
+template <class T>
+void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
+                     const Parts<T>& p, GridD<T> g, BoxD<T> outlet, uint8_t* cells,
+                     uint32_t* count, hipStream_t st);
+template <class T>
+void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
+                const uint32_t* cend, T* out, bool exact, hipStream_t st);
+template <class T>
+void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st);
+template <class T>
+void launch_enforce_solid(GridD<T> g, FacesD<T> f, const uint8_t* cells, hipStream_t st);
+template <class T>
+void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
+                      const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
+                      double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
+                      hipStream_t st);
+template <class T>
+void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
+                 int it, hipStream_t st);
+template <class T>
+void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,
+                         const PcgScalars* sc, hipStream_t st);
+int dot_blocks(Shear sh, bool exact);
+void launch_dot_finish(double* partials, int nblocks, bool exact, int op, PcgScalars* sc,
+                       double tol, hipStream_t st);
+template <class T>
+void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
+                            const PcgScalars* sc, hipStream_t st);
+int maxabs_blocks(Shear sh);
+template <class T>
+void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScalars* sc,
+                    double* partials, int nblocks, hipStream_t st);
+template <class T>
+void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st);
+template <class T>
+void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
+                      int sigma, const PcgScalars* sc, hipStream_t st);
+template <class T>
+void launch_gradient(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, const T* p, T coef,
+                     hipStream_t st);
+template <class T>
+void launch_unshear(Shear sh, const T* p, T* out, hipStream_t st);
+template <class T>
+void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st);
+template <class T>
+void launch_extrap_iter(int nx, int ny, int nz, const T* src, T* dst, const uint8_t* ksrc,
+                        uint8_t* kdst, hipStream_t st);
+template <class T>
+void launch_g2p(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> saved, T flip,
+                hipStream_t st);
+template <class T>
+void launch_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt,
+                   hipStream_t st);
+template <class T>
+void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
+                   uint32_t* cull_flag, unsigned long long* counters, hipStream_t st);
+template <class T>
+void launch_compact(const Parts<T>& src, const Parts<T>& dst, const uint32_t* keep,
+                    const uint32_t* pos, int64_t n, uint32_t* culled_ranks, hipStream_t st);
+void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* culled_sorted, int64_t nculled,
+                     hipStream_t st);
+template <class T>
+void launch_hist(const Parts<T>& p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count,
+                 hipStream_t st);
+void launch_active_flags(const int32_t* cellkey, const uint32_t* count, int64_t n, uint32_t cap,
+                         uint32_t* flag, hipStream_t st);
+void launch_active_compact(const uint32_t* flag, const uint32_t* pos, int64_t n,
+                           const uint32_t* rank, uint32_t* act_rank, uint32_t* act_slot,
+                           hipStream_t st);
+void launch_redist_decide(const uint32_t* act_slot_sorted, int64_t nact, const int32_t* cellkey,
+                          uint32_t* count, const uint8_t* mask, int nx, int ny, int nz, uint32_t cap,
+                          double gx, double gy, double gz, int32_t* dest, hipStream_t st);
+template <class T>
+void launch_redist_apply(const Parts<T>& p, const uint32_t* act_slot_sorted, const int32_t* dest,
+                         int64_t nact, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st);
+template <class T>
+void launch_maxabs_faces(const T* a, int64_t n, unsigned long long* out_bits, hipStream_t st);
+template <class T>
+void launch_to_aos(const Parts<T>& p, int64_t n, double* pos, double* vel, hipStream_t st);
+template <class T>
+void launch_from_aos(const Parts<T>& p, int64_t n, int64_t base, const double* pos,
+                     const double* vel, hipStream_t st);
+template <class T>
+void launch_seed_fill(const Parts<T>& p, int64_t base, GridD<T> g, const uint8_t* mask,
+                      const uint32_t* fill_cells, int64_t nfill, uint32_t per_cell, uint64_t seed,
+                      T vx, T vy, T vz, hipStream_t st);
+template <class T>
+void launch_static_cls(GridD<T> g, const uint8_t* mask, BoxD<T> outlet, uint8_t* static_cls,
+                       hipStream_t st);
+template <class T>
+void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st);
+
+}  // namespace sdr

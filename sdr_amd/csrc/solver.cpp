@@ -1,0 +1,1031 @@
+// solver.cpp — host side of the MI355X substep: the reference's `Solver` and
+// `pressure::project_capturing_with_source` surface (uselessgoddess/sdr
+// src/solver.rs:143-310, src/pressure.rs:123-239) driving the gfx950 kernels, and the
+// C ABI of include/sdr_amd.h on top of it.
+//
+// Layout on the device (one HIP stream per solver):
+//   particles  SoA px,py,pz,vx,vy,vz (T) + rank (u32, the reference index), double
+//              buffered; cell-sorted from `classify` until the first moving kernel
+//   grid       u,v,w and saved_u,v,w (T), MAC layout of grid.rs:128-142
+//   cells      u8 classification (0 Solid, 1 Fluid, 2 Air), static cavity mask
+//   solve      b/r, z, s, p, As, q, precon (T) + u16 flags in the sheared layout
+//              (device.hpp Shear): row J = i + j of plane k
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sdr_amd.h"
+#include "kernels.hpp"
+#include "sort.hpp"
+
+namespace sdr {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                            \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess)                                                                    \
+      throw Error(e_ == hipErrorOutOfMemory ? SDR_E_OOM : SDR_E_HIP,                         \
+                  std::string(#x) + ": " + hipGetErrorString(e_));                           \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffer
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 256;
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void ensure(size_t b) {
+    if (b > bytes) alloc(b);
+  }
+  template <class U>
+  U* as() const {
+    return (U*)p;
+  }
+};
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    bytes = b;
+  }
+};
+
+// ------------------------------------------------------------------ host math mirror
+// Host-side pieces of the substep (emission, CFL) restate the reference in f64 with the
+// same libm, so the device path consumes the same particles and timesteps.
+struct H3 {
+  double x, y, z;
+};
+static inline H3 operator+(H3 a, H3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline H3 operator*(H3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline H3 operator/(H3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline double hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline H3 hcross(H3 a, H3 o) {
+  return {a.y * o.z - a.z * o.y, a.z * o.x - a.x * o.z, a.x * o.y - a.y * o.x};
+}
+static inline double hlen(H3 a) { return std::sqrt(hdot(a, a)); }
+static inline H3 hnorm(H3 a) {  // math.rs:73-80
+  double l = hlen(a);
+  if (l > 1e-12) return a / l;
+  return {0, 0, 0};
+}
+static inline H3 h3(sdr_vec3 v) { return {v.x, v.y, v.z}; }
+
+static uint64_t xorshift(uint64_t& s) {  // solver.rs:127-134
+  uint64_t x = s;
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  s = x;
+  return x;
+}
+static double unit(uint64_t& s) { return (double)(xorshift(s) >> 11) / (double)(1ull << 53); }
+
+static int bits_for(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+// ------------------------------------------------------------------ timing
+struct Timer {
+  bool on = false;
+  hipStream_t st = nullptr;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, uint64_t>> acc;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  size_t begin(const char* name) {
+    if (!on) return 0;
+    Pending p{name, get(), get()};
+    HIPCHK(hipEventRecord(p.a, st));
+    pending.push_back(p);
+    return pending.size();
+  }
+  void end(size_t id) {
+    if (!on || id == 0) return;
+    HIPCHK(hipEventRecord(pending[id - 1].b, st));
+  }
+  void collect() {
+    if (pending.empty()) return;
+    HIPCHK(hipStreamSynchronize(st));
+    for (auto& p : pending) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+      auto& e = acc[p.name];
+      e.first += ms;
+      e.second += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Timer() {
+    for (auto& p : pending) pool.push_back(p.a), pool.push_back(p.b);
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+struct Scope {
+  Timer& t;
+  size_t id;
+  Scope(Timer& t_, const char* n) : t(t_), id(t_.begin(n)) {}
+  ~Scope() { t.end(id); }
+};
+
+// ------------------------------------------------------------------ pressure projection
+// `pressure::project_capturing_with_source` (pressure.rs:123-239) on device buffers.
+template <class T>
+struct Projector {
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0;
+  Shear sh{};
+  bool exact_dot = true;
+  hipStream_t st = nullptr;
+  Timer* timer = nullptr;
+  Buf b, z, s, p, as, q, precon, flags, partials, scal;
+  PinnedBuf host_scal;
+  int64_t ncells() const { return (int64_t)nx * ny * nz; }
+
+  void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
+    nx = x, ny = y, nz = zz, dx = h, st = stream, timer = t;
+    sh = Shear{nx, ny, nz, nx + ny - 1};
+    size_t n = (size_t)sh.size();
+    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
+    flags.alloc(n * sizeof(uint16_t));
+    int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
+    partials.alloc((size_t)nb * sizeof(double));
+    scal.alloc(sizeof(PcgScalars));
+    host_scal.ensure(sizeof(PcgScalars));
+  }
+
+  PcgScalars* sc() const { return scal.as<PcgScalars>(); }
+
+  void mic_sweep(int op, const T* in, T* out) {
+    const int planes = sh.NJ + sh.nz - 1;
+    for (int sgm = 0; sgm < planes; ++sgm)
+      launch_mic_plane<T>(op, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sgm, sc(), st);
+  }
+
+  void dot(const T* a, const T* bb, int op, double tol) {
+    int nb = dot_blocks(sh, exact_dot);
+    launch_dot_partials<T>(sh, a, bb, partials.as<double>(), nb, exact_dot, sc(), st);
+    launch_dot_finish(partials.as<double>(), nb, exact_dot, op, sc(), tol, st);
+  }
+
+  PcgScalars read_scalars() {
+    HIPCHK(hipMemcpyAsync(host_scal.p, scal.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    PcgScalars r;
+    memcpy(&r, host_scal.p, sizeof r);
+    return r;
+  }
+
+  // pcg (pressure.rs:341-397) on b (becomes r); the solution is left in p.
+  PcgScalars pcg(uint64_t max_iters, double tol) {
+    const size_t n = (size_t)sh.size();
+    {
+      Scope sc_(*timer, "pcg_setup");
+      for (Buf* bb : {&z, &s, &p, &as, &q, &precon}) HIPCHK(hipMemsetAsync(bb->p, 0, n * sizeof(T), st));
+      HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(PcgScalars), st));
+      int mb = maxabs_blocks(sh);
+      launch_maxabs_partials<T>(sh, b.as<T>(), partials.as<double>(), mb, sc(), st);
+      launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), tol, st);
+    }
+    {
+      Scope sc_(*timer, "mic_build");
+      mic_sweep(kMicBuild, nullptr, nullptr);
+    }
+    T* r = b.as<T>();
+    {
+      Scope sc_(*timer, "mic_apply");
+      mic_sweep(kMicFwd, r, q.as<T>());
+      mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
+    }
+    HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    dot(z.as<T>(), r, kOpSigma0, tol);
+    const int mb = maxabs_blocks(sh);
+    const uint64_t check_every = 8;
+    for (uint64_t it = 0; it < max_iters; ++it) {
+      {
+        Scope sc_(*timer, "spmv_dot");
+        launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, st);
+        dot(s.as<T>(), as.as<T>(), kOpDenom, tol);
+      }
+      {
+        Scope sc_(*timer, "axpy_norm");
+        launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb,
+                          st);
+        launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
+      }
+      {
+        Scope sc_(*timer, "mic_apply");
+        mic_sweep(kMicFwd, r, q.as<T>());
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
+      }
+      {
+        Scope sc_(*timer, "dot_update");
+        dot(z.as<T>(), r, kOpSigmaNew, tol);
+        launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
+      }
+      if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
+        if (read_scalars().done) break;
+      }
+    }
+    return read_scalars();
+  }
+
+  // Steps 1-5 of project_capturing_with_source on the device faces (MAC layout).
+  sdr_solve_report project(GridD<T> g, FacesD<T> f, const uint8_t* cells, const uint32_t* count,
+                           const double* srcarr, double rho, double dt, uint64_t max_iters,
+                           double tol, bool use_count_source, double rest, double k_vol,
+                           T* pressure_out, int* nan_flag) {
+    {
+      Scope sc_(*timer, "rhs");
+      launch_enforce_solid<T>(g, f, cells, st);
+      double inv_scale = rho * g.dx / dt;
+      double unit_ = g.dx / dt;
+      launch_rhs_flags<T>(g, sh, CFacesD<T>{f.u, f.v, f.w}, cells, count, srcarr, inv_scale,
+                          use_count_source ? 1 : 0, rest, k_vol, unit_, 0.25 * unit_, b.as<T>(),
+                          flags.as<uint16_t>(), st);
+    }
+    PcgScalars r = pcg(max_iters, tol);
+    {
+      Scope sc_(*timer, "gradient");
+      double coef = dt / (rho * g.dx);
+      launch_gradient<T>(g, sh, f, cells, p.as<T>(), (T)coef, st);
+      if (pressure_out) launch_unshear<T>(sh, p.as<T>(), pressure_out, st);
+    }
+    if (nan_flag) *nan_flag = r.nan_seen;
+    return sdr_solve_report{(uint64_t)r.iters, r.residual};
+  }
+};
+
+// ------------------------------------------------------------------ solver
+struct SolverBase {
+  virtual ~SolverBase() = default;
+};
+
+template <class T>
+struct Solver : SolverBase {
+  // -- configuration (solver.rs:143-166)
+  int nx, ny, nz;
+  double dx;
+  H3 origin;
+  sdr_sdf sdf_desc{};
+  sdr_fluid_params fluid{};
+  sdr_needle needle{};
+  bool has_outlet = false;
+  sdr_aabb outlet{};
+  sdr_solve_params solve{200, 1e-6};
+  sdr_options opts{};
+  // -- scalar state
+  double time = 0;
+  uint64_t steps = 0, last_pcg_iters = 0;
+  uint32_t last_substeps = 0;
+  double last_residual = 0;
+  uint64_t drained = 0, emitted = 0, culled = 0;
+  uint64_t rng = 0x5deece66dull | 1;
+  double emit_accum = 0;
+  int64_t n = 0, cap = 0;
+  bool sorted_valid = false, have_saved = false;
+  int cur = 0;
+  // -- device
+  hipStream_t st = nullptr;
+  Timer timer;
+  Buf part[2][7];
+  Buf keys, keys_alt, vals, vals_alt, keys64, keys64_alt;
+  Buf cstart, cend, count, mask, static_cls, cells;
+  Buf fu, fv, fw, su, sv, sw, ftmp, known0, known1, pressure, sdf_data;
+  Buf temp;
+  Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
+  Buf cellkey, act_rank, act_slot, act_rank_sorted, act_slot_sorted, dest;
+  Buf counters, aos_pos, aos_vel, fill_cells;
+  PinnedBuf pin;
+  Projector<T> proj;
+
+  GridD<T> grid() const { return GridD<T>{nx, ny, nz, (T)dx, (T)origin.x, (T)origin.y, (T)origin.z}; }
+  SdfD<T> sdf() const {
+    return SdfD<T>{(int)sdf_desc.nx, (int)sdf_desc.ny, (int)sdf_desc.nz, (T)sdf_desc.dx,
+                   (T)sdf_desc.origin.x, (T)sdf_desc.origin.y, (T)sdf_desc.origin.z,
+                   sdf_data.as<T>()};
+  }
+  BoxD<T> outlet_box() const {
+    return BoxD<T>{has_outlet ? 1 : 0, (T)outlet.min.x, (T)outlet.min.y, (T)outlet.min.z,
+                   (T)outlet.max.x, (T)outlet.max.y, (T)outlet.max.z};
+  }
+  Parts<T> parts(int which) const {
+    return Parts<T>{part[which][0].as<T>(), part[which][1].as<T>(), part[which][2].as<T>(),
+                    part[which][3].as<T>(), part[which][4].as<T>(), part[which][5].as<T>(),
+                    part[which][6].as<uint32_t>()};
+  }
+  int64_t ncells() const { return (int64_t)nx * ny * nz; }
+  int64_t nu() const { return (int64_t)(nx + 1) * ny * nz; }
+  int64_t nv() const { return (int64_t)nx * (ny + 1) * nz; }
+  int64_t nw() const { return (int64_t)nx * ny * (nz + 1); }
+  FacesD<T> faces() const { return FacesD<T>{fu.as<T>(), fv.as<T>(), fw.as<T>()}; }
+  CFacesD<T> cfaces() const { return CFacesD<T>{fu.as<T>(), fv.as<T>(), fw.as<T>()}; }
+  CFacesD<T> csaved() const { return CFacesD<T>{su.as<T>(), sv.as<T>(), sw.as<T>()}; }
+
+  ~Solver() override {
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // Solver::new (solver.rs:184-222).
+  void create(const sdr_sdf* s, double sim_dx, const sdr_fluid_params* f, const sdr_needle* nd,
+              const sdr_options* o) {
+    opts = o ? *o : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8};
+    HIPCHK(hipSetDevice(opts.device));
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    timer.st = st;
+    sdf_desc = *s;
+    sdf_desc.data = nullptr;
+    fluid = *f;
+    needle = *nd;
+    // bounds = origin .. origin + (n-1)*dx (sdf.rs:119-129); dims = ceil(size/dx).max(1)
+    double mx = s->origin.x + (double)(s->nx - 1) * s->dx;
+    double my = s->origin.y + (double)(s->ny - 1) * s->dx;
+    double mz = s->origin.z + (double)(s->nz - 1) * s->dx;
+    double szx = mx - s->origin.x, szy = my - s->origin.y, szz = mz - s->origin.z;
+    double fx = std::fmax(std::ceil(szx / sim_dx), 1.0), fy = std::fmax(std::ceil(szy / sim_dx), 1.0),
+           fz = std::fmax(std::ceil(szz / sim_dx), 1.0);
+    if (!(fx * fy * fz < 2.0e9)) throw Error(SDR_E_INVALID, "grid too large");
+    nx = (int)fx, ny = (int)fy, nz = (int)fz;
+    if (nx < 2 || ny < 2 || nz < 2) throw Error(SDR_E_INVALID, "grid needs at least 2 cells per axis");
+    dx = sim_dx;
+    origin = {s->origin.x, s->origin.y, s->origin.z};
+    const int64_t nc = ncells();
+    size_t nsdf = (size_t)(s->nx * s->ny * s->nz);
+    sdf_data.alloc(nsdf * sizeof(T));
+    {
+      std::vector<T> tmp(nsdf);
+      for (size_t t = 0; t < nsdf; ++t) tmp[t] = (T)s->data[t];
+      HIPCHK(hipMemcpy(sdf_data.p, tmp.data(), nsdf * sizeof(T), hipMemcpyHostToDevice));
+    }
+    for (Buf* b : {&cstart, &cend, &count}) b->alloc((size_t)nc * 4);
+    for (Buf* b : {&mask, &static_cls, &cells}) b->alloc((size_t)nc);
+    for (Buf* b : {&fu, &su}) b->alloc((size_t)nu() * sizeof(T));
+    for (Buf* b : {&fv, &sv}) b->alloc((size_t)nv() * sizeof(T));
+    for (Buf* b : {&fw, &sw}) b->alloc((size_t)nw() * sizeof(T));
+    int64_t nf = std::max(nu(), std::max(nv(), nw()));
+    ftmp.alloc((size_t)nf * sizeof(T));
+    known0.alloc((size_t)nf);
+    known1.alloc((size_t)nf);
+    pressure.alloc((size_t)nc * sizeof(T));
+    counters.alloc(4 * sizeof(unsigned long long));
+    for (Buf* b : {&fu, &fv, &fw, &su, &sv, &sw, &pressure})
+      HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, st));
+    HIPCHK(hipMemsetAsync(cells.p, kAir, (size_t)nc, st));
+    launch_cavity_mask<T>(grid(), sdf(), mask.as<uint8_t>(), st);
+    launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
+    proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
+    proj.init(nx, ny, nz, dx, st, &timer);
+    reserve(1024);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // Grow particle storage (keeps the live particles of the current buffer).
+  void reserve(int64_t want) {
+    if (want <= cap) return;
+    int64_t nc = std::max<int64_t>(want, cap + cap / 2);
+    if (nc > (int64_t)0x7fffffff) throw Error(SDR_E_INVALID, "too many particles");
+    for (int w = 0; w < 2; ++w)
+      for (int a = 0; a < 7; ++a) {
+        size_t es = a == 6 ? 4 : sizeof(T);
+        Buf nb;
+        nb.alloc((size_t)nc * es);
+        if (w == cur && n > 0) HIPCHK(hipMemcpyAsync(nb.p, part[w][a].p, (size_t)n * es, hipMemcpyDeviceToDevice, st));
+        std::swap(part[w][a].p, nb.p);
+        std::swap(part[w][a].bytes, nb.bytes);
+      }
+    for (Buf* b : {&keys, &keys_alt, &vals, &vals_alt, &keep, &scanpos, &cullflag, &culled_ranks,
+                   &culled_sorted, &cellkey, &act_rank, &act_slot, &act_rank_sorted,
+                   &act_slot_sorted, &dest})
+      b->alloc((size_t)nc * 4);
+    if (opts.dot_mode == SDR_DOT_EXACT) {
+      keys64.alloc((size_t)nc * 8);
+      keys64_alt.alloc((size_t)nc * 8);
+    }
+    size_t tb = std::max({sort_pairs_u32_temp(nc), sort_pairs_u64_temp(nc), sort_keys_u32_temp(nc),
+                          scan_u32_temp(nc)});
+    temp.alloc(tb);
+    cap = nc;
+  }
+
+  bool exact() const { return opts.dot_mode == SDR_DOT_EXACT; }
+
+  // ---------------------------------------------------------------- stages
+  // emit (solver.rs:313-339): host-side (libm cos/sin), appended with rank = index.
+  void emit(double dt) {
+    uint64_t ppc = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
+    double pv = dx * dx * dx / (double)ppc;
+    if (pv <= 0.0 || needle.flow_rate <= 0.0 || time >= needle.inject_until) return;
+    emit_accum += needle.flow_rate * dt / pv;
+    double cnt = std::floor(emit_accum);
+    uint64_t m = cnt > 0 ? (uint64_t)cnt : 0;
+    emit_accum -= cnt;
+    if (m == 0) return;
+    Scope sc_(timer, "emit");
+    H3 axis = hnorm(h3(needle.axis));
+    H3 a = std::fabs(axis.x) < 0.9 ? H3{1, 0, 0} : H3{0, 1, 0};  // solver.rs:933-942
+    H3 e1 = hnorm(hcross(axis, a));
+    H3 e2 = hnorm(hcross(axis, e1));
+    double area = M_PI * needle.radius * needle.radius;
+    double speed = area > 0.0 ? needle.flow_rate / area : 0.0;
+    H3 vel = axis * speed;
+    reserve(n + (int64_t)m);
+    pin.ensure(m * 6 * sizeof(double));
+    double* hp = (double*)pin.p;
+    double* hv = hp + 3 * m;
+    for (uint64_t t = 0; t < m; ++t) {
+      double r = needle.radius * std::sqrt(unit(rng));
+      double theta = 2.0 * M_PI * unit(rng);
+      double along = (unit(rng) - 0.5) * dx;
+      H3 p = h3(needle.tip) + e1 * (r * std::cos(theta)) + e2 * (r * std::sin(theta)) + axis * along;
+      hp[3 * t] = p.x, hp[3 * t + 1] = p.y, hp[3 * t + 2] = p.z;
+      hv[3 * t] = vel.x, hv[3 * t + 1] = vel.y, hv[3 * t + 2] = vel.z;
+    }
+    aos_pos.ensure(m * 3 * sizeof(double));
+    aos_vel.ensure(m * 3 * sizeof(double));
+    HIPCHK(hipMemcpyAsync(aos_pos.p, hp, m * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(aos_vel.p, hv, m * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    launch_from_aos<T>(parts(cur), (int64_t)m, n, aos_pos.as<double>(), aos_vel.as<double>(), st);
+    HIPCHK(hipStreamSynchronize(st));  // the pinned staging buffer is reused next substep
+    n += (int64_t)m;
+    emitted += m;
+    sorted_valid = false;
+  }
+
+  // Stable sort of the particles by (clamped) cell; cell segments [cstart, cend).
+  void sort_particles() {
+    Scope sc_(timer, "sort");
+    const int64_t nc = ncells();
+    HIPCHK(hipMemsetAsync(cstart.p, 0, (size_t)nc * 4, st));
+    HIPCHK(hipMemsetAsync(cend.p, 0, (size_t)nc * 4, st));
+    if (n == 0) {
+      sorted_valid = true;
+      return;
+    }
+    int cbits = bits_for((uint64_t)(nc - 1));
+    if (exact()) {
+      launch_keys<T>(parts(cur), n, grid(), nullptr, keys64.as<uint64_t>(), vals.as<uint32_t>(), true, st);
+      HIPCHK(sort_pairs_u64(temp.p, temp.bytes, keys64.as<uint64_t>(), keys64_alt.as<uint64_t>(),
+                            vals.as<uint32_t>(), vals_alt.as<uint32_t>(), n, 32 + cbits, st));
+      launch_cell_ranges(nullptr, keys64_alt.as<uint64_t>(), n, nc, cstart.as<uint32_t>(),
+                         cend.as<uint32_t>(), st);
+    } else {
+      launch_keys<T>(parts(cur), n, grid(), keys.as<uint32_t>(), nullptr, vals.as<uint32_t>(), false, st);
+      HIPCHK(sort_pairs_u32(temp.p, temp.bytes, keys.as<uint32_t>(), keys_alt.as<uint32_t>(),
+                            vals.as<uint32_t>(), vals_alt.as<uint32_t>(), n, cbits, st));
+      launch_cell_ranges(keys_alt.as<uint32_t>(), nullptr, n, nc, cstart.as<uint32_t>(),
+                         cend.as<uint32_t>(), st);
+    }
+    launch_gather_parts<T>(parts(cur), parts(1 - cur), vals_alt.as<uint32_t>(), n, st);
+    cur = 1 - cur;
+    sorted_valid = true;
+  }
+
+  // classify (solver.rs:342-371) + the per-cell counts of volume_source.
+  void classify() {
+    sort_particles();
+    Scope sc_(timer, "classify");
+    launch_classify<T>(static_cls.as<uint8_t>(), cstart.as<uint32_t>(), cend.as<uint32_t>(),
+                       parts(cur), grid(), outlet_box(), cells.as<uint8_t>(), count.as<uint32_t>(), st);
+  }
+  void ensure_counts() {
+    if (sorted_valid) return;
+    // cells stay as set; only the segments and counts are refreshed
+    Buf keep_cells;
+    keep_cells.alloc((size_t)ncells());
+    HIPCHK(hipMemcpyAsync(keep_cells.p, cells.p, (size_t)ncells(), hipMemcpyDeviceToDevice, st));
+    classify();
+    HIPCHK(hipMemcpyAsync(cells.p, keep_cells.p, (size_t)ncells(), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // p2g (solver.rs:560-585)
+  void p2g() {
+    ensure_counts();
+    Scope sc_(timer, "p2g");
+    launch_p2g<T>(0, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fu.as<T>(), exact(), st);
+    launch_p2g<T>(1, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fv.as<T>(), exact(), st);
+    launch_p2g<T>(2, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fw.as<T>(), exact(), st);
+  }
+  void save_grid() {  // solver.rs:587-591
+    HIPCHK(hipMemcpyAsync(su.p, fu.p, (size_t)nu() * sizeof(T), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv.p, fv.p, (size_t)nv() * sizeof(T), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sw.p, fw.p, (size_t)nw() * sizeof(T), hipMemcpyDeviceToDevice, st));
+    have_saved = true;
+  }
+  void add_forces(double dt) {  // solver.rs:593-604
+    if (fluid.gravity.x != 0.0) launch_axpy_const<T>(fu.as<T>(), nu(), (T)(fluid.gravity.x * dt), st);
+    if (fluid.gravity.y != 0.0) launch_axpy_const<T>(fv.as<T>(), nv(), (T)(fluid.gravity.y * dt), st);
+    if (fluid.gravity.z != 0.0) launch_axpy_const<T>(fw.as<T>(), nw(), (T)(fluid.gravity.z * dt), st);
+  }
+  // volume_source + project_capturing_with_source (solver.rs:290-299)
+  void project(double dt) {
+    ensure_counts();
+    uint64_t ppc = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
+    bool use_src = fluid.volume_stiffness > 0.0 && dt > 0.0;
+    int nanf = 0;
+    sdr_solve_report r = proj.project(grid(), faces(), cells.as<uint8_t>(), count.as<uint32_t>(),
+                                      nullptr, fluid.density, dt, solve.max_iters, solve.tol, use_src,
+                                      (double)ppc, fluid.volume_stiffness, pressure.as<T>(), &nanf);
+    last_pcg_iters = r.iters;
+    last_residual = r.residual;
+  }
+  // extrapolate_velocities (solver.rs:608-647) with 4 passes of extrapolate (960-1009)
+  void extrapolate() {
+    Scope sc_(timer, "extrapolate");
+    T* arr[3] = {fu.as<T>(), fv.as<T>(), fw.as<T>()};
+    for (int c = 0; c < 3; ++c) {
+      int ax = nx + (c == 0), ay = ny + (c == 1), az = nz + (c == 2);
+      launch_extrap_init<T>(grid(), c, cells.as<uint8_t>(), known0.as<uint8_t>(), st);
+      T* a = arr[c];
+      T* b = ftmp.as<T>();
+      uint8_t* k0 = known0.as<uint8_t>();
+      uint8_t* k1 = known1.as<uint8_t>();
+      for (int it = 0; it < 4; ++it) {
+        launch_extrap_iter<T>(ax, ay, az, a, b, k0, k1, st);
+        std::swap(a, b);
+        std::swap(k0, k1);
+      }
+    }
+  }
+  void g2p() {  // solver.rs:650-677
+    if (!have_saved) throw Error(SDR_E_STATE, "g2p needs the saved post-P2G grid");
+    Scope sc_(timer, "g2p");
+    double fr = fluid.flip_ratio < 0.0 ? 0.0 : (fluid.flip_ratio > 1.0 ? 1.0 : fluid.flip_ratio);
+    launch_g2p<T>(parts(cur), n, grid(), cfaces(), csaved(), (T)fr, st);
+  }
+  void advect(double dt) {  // solver.rs:693-711
+    Scope sc_(timer, "advect");
+    launch_advect<T>(parts(cur), n, grid(), cfaces(), sdf(), (T)dt, st);
+    sorted_valid = false;
+  }
+  // enforce_particle_bounds (solver.rs:736-755) + ParticleSet::retain (particles.rs:38-49)
+  void bounds() {
+    if (n == 0) return;
+    Scope sc_(timer, "bounds");
+    HIPCHK(hipMemsetAsync(counters.p, 0, 4 * sizeof(unsigned long long), st));
+    launch_bounds<T>(parts(cur), n, sdf(), outlet_box(), keep.as<uint32_t>(), cullflag.as<uint32_t>(),
+                     counters.as<unsigned long long>(), st);
+    HIPCHK(exclusive_sum_u32(temp.p, temp.bytes, keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, st));
+    pin.ensure(64);
+    unsigned long long* hc = (unsigned long long*)pin.p;
+    HIPCHK(hipMemcpyAsync(hc, counters.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint64_t dr = hc[0], cu = hc[1];
+    drained += dr;
+    culled += cu;
+    int64_t removed = (int64_t)(dr + cu);
+    if (removed == 0) return;
+    launch_compact<T>(parts(cur), parts(1 - cur), keep.as<uint32_t>(), scanpos.as<uint32_t>(), n,
+                      culled_ranks.as<uint32_t>(), st);
+    cur = 1 - cur;
+    int64_t kept = n - removed;
+    HIPCHK(sort_keys_u32(temp.p, temp.bytes, culled_ranks.as<uint32_t>(), culled_sorted.as<uint32_t>(),
+                         removed, st));
+    launch_rank_fix(part[cur][6].as<uint32_t>(), kept, culled_sorted.as<uint32_t>(), removed, st);
+    n = kept;
+    sorted_valid = false;
+  }
+  // redistribute (solver.rs:454-557)
+  void redistribute() {
+    if (n == 0) return;
+    Scope sc_(timer, "redistribute");
+    uint64_t target = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
+    uint32_t capc = (uint32_t)std::ceil(std::fmax(fluid.redist_cap_factor, 1.0) * (double)target);
+    H3 gdir = hnorm(h3(fluid.gravity));
+    pin.ensure(64);
+    uint32_t* hn = (uint32_t*)pin.p;
+    for (int sweep = 0; sweep < 4; ++sweep) {
+      HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
+      launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
+      launch_active_flags(cellkey.as<int32_t>(), count.as<uint32_t>(), n, capc, keep.as<uint32_t>(), st);
+      HIPCHK(exclusive_sum_u32(temp.p, temp.bytes, keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, st));
+      HIPCHK(hipMemcpyAsync(hn, scanpos.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(hn + 1, keep.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      int64_t nact = (int64_t)hn[0] + hn[1];
+      if (nact == 0) break;  // no cell over the cap: later sweeps see the same state
+      launch_active_compact(keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, part[cur][6].as<uint32_t>(),
+                            act_rank.as<uint32_t>(), act_slot.as<uint32_t>(), st);
+      HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(),
+                            act_slot.as<uint32_t>(), act_slot_sorted.as<uint32_t>(), nact, 32, st));
+      launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(),
+                           count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x, gdir.y,
+                           gdir.z, dest.as<int32_t>(), st);
+      launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact,
+                             grid(), cfaces(), sdf(), st);
+    }
+    sorted_valid = false;
+  }
+
+  void run_stage(int stage, double dt) {
+    switch (stage) {
+      case SDR_STAGE_EMIT: emit(dt); break;
+      case SDR_STAGE_CLASSIFY: classify(); break;
+      case SDR_STAGE_P2G: p2g(); break;
+      case SDR_STAGE_SAVE: save_grid(); break;
+      case SDR_STAGE_FORCES: add_forces(dt); break;
+      case SDR_STAGE_PROJECT: project(dt); break;
+      case SDR_STAGE_EXTRAPOLATE: extrapolate(); break;
+      case SDR_STAGE_G2P: g2p(); break;
+      case SDR_STAGE_ADVECT: advect(dt); break;
+      case SDR_STAGE_BOUNDS: bounds(); break;
+      case SDR_STAGE_REDISTRIBUTE:
+        if (fluid.redistribute) redistribute();
+        break;
+      case SDR_STAGE_FINISH:
+        time += dt;
+        steps += 1;
+        break;
+      default: throw Error(SDR_E_INVALID, "unknown stage");
+    }
+  }
+  void substep(double dt) {  // solver.rs:284-310
+    for (int s = 0; s < SDR_STAGE_COUNT; ++s) run_stage(s, dt);
+    timer.collect();
+  }
+
+  // cfl_dt (solver.rs:262-281) with MacGrid::max_speed (grid.rs:202-206) on the device.
+  double cfl_dt() {
+    HIPCHK(hipMemsetAsync(counters.p, 0, 3 * sizeof(unsigned long long), st));
+    unsigned long long* c = counters.as<unsigned long long>();
+    launch_maxabs_faces<T>(fu.as<T>(), nu(), c, st);
+    launch_maxabs_faces<T>(fv.as<T>(), nv(), c + 1, st);
+    launch_maxabs_faces<T>(fw.as<T>(), nw(), c + 2, st);
+    pin.ensure(64);
+    HIPCHK(hipMemcpyAsync(pin.p, counters.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double m[3];
+    memcpy(m, pin.p, sizeof m);
+    double grid_speed = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+    double area = M_PI * needle.radius * needle.radius;
+    double jet = time < needle.inject_until ? (area > 0.0 ? needle.flow_rate / area : 0.0) : 0.0;
+    double g = std::sqrt(hlen(h3(fluid.gravity)) * dx);
+    double speed = std::fmax(std::fmax(std::fmax(grid_speed, jet), g), 1e-6);
+    return std::fmin(dx / speed, 0.01);
+  }
+  void step(double dt) {  // solver.rs:237-259
+    const uint32_t MAX_SUBSTEPS = 512;
+    double floor_dt = dt / (double)MAX_SUBSTEPS;
+    double remaining = dt;
+    uint32_t taken = 0;
+    while (remaining > 1e-12) {
+      double sub = std::fmin(std::fmax(cfl_dt(), floor_dt), remaining);
+      substep(sub);
+      remaining -= sub;
+      taken += 1;
+    }
+    last_substeps = taken;
+  }
+
+  // ---------------------------------------------------------------- host I/O
+  void get_particles(double* pos, double* vel, uint64_t m) {
+    if ((int64_t)m != n) throw Error(SDR_E_INVALID, "particle count mismatch");
+    if (n == 0) return;
+    aos_pos.ensure((size_t)n * 3 * sizeof(double));
+    aos_vel.ensure((size_t)n * 3 * sizeof(double));
+    launch_to_aos<T>(parts(cur), n, aos_pos.as<double>(), aos_vel.as<double>(), st);
+    if (pos) HIPCHK(hipMemcpyAsync(pos, aos_pos.p, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (vel) HIPCHK(hipMemcpyAsync(vel, aos_vel.p, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  void set_particles(const double* pos, const double* vel, uint64_t m) {
+    reserve((int64_t)m);
+    n = (int64_t)m;
+    if (m > 0) {
+      aos_pos.ensure(m * 3 * sizeof(double));
+      aos_vel.ensure(m * 3 * sizeof(double));
+      HIPCHK(hipMemcpyAsync(aos_pos.p, pos, m * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+      if (vel) HIPCHK(hipMemcpyAsync(aos_vel.p, vel, m * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+      launch_from_aos<T>(parts(cur), n, 0, aos_pos.as<double>(), vel ? aos_vel.as<double>() : nullptr, st);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    sorted_valid = false;
+  }
+  void copy_out(const Buf& b, int64_t cnt, double* out) {
+    if (!out) return;
+    if (sizeof(T) == sizeof(double)) {
+      HIPCHK(hipMemcpyAsync(out, b.p, (size_t)cnt * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      std::vector<T> tmp((size_t)cnt);
+      HIPCHK(hipMemcpyAsync(tmp.data(), b.p, (size_t)cnt * sizeof(T), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int64_t t = 0; t < cnt; ++t) out[t] = (double)tmp[(size_t)t];
+    }
+  }
+  void copy_in(Buf& b, int64_t cnt, const double* in) {
+    if (!in) return;
+    std::vector<T> tmp((size_t)cnt);
+    for (int64_t t = 0; t < cnt; ++t) tmp[(size_t)t] = (T)in[t];
+    HIPCHK(hipMemcpyAsync(b.p, tmp.data(), (size_t)cnt * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  void set_outlet(const sdr_aabb* o) {
+    has_outlet = o != nullptr;
+    if (o) outlet = *o;
+    launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  uint64_t seed_fill(uint32_t per_cell, double y_lo, double y_hi, uint64_t seed, const sdr_vec3* v) {
+    std::vector<uint8_t> hm((size_t)ncells());
+    HIPCHK(hipMemcpyAsync(hm.data(), mask.p, hm.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<uint32_t> list;
+    for (int64_t c = 0; c < ncells(); ++c) {
+      int j = (int)((c / nx) % ny);
+      double cy = origin.y + ((double)j + 0.5) * dx;
+      if (hm[(size_t)c] && cy >= y_lo && cy <= y_hi) list.push_back((uint32_t)c);
+    }
+    int64_t add = (int64_t)list.size() * per_cell;
+    reserve(n + add);
+    fill_cells.ensure(list.size() * 4 + 4);
+    HIPCHK(hipMemcpyAsync(fill_cells.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, st));
+    launch_seed_fill<T>(parts(cur), n, grid(), mask.as<uint8_t>(), fill_cells.as<uint32_t>(),
+                        (int64_t)list.size(), per_cell, seed, v ? (T)v->x : T(0), v ? (T)v->y : T(0),
+                        v ? (T)v->z : T(0), st);
+    HIPCHK(hipStreamSynchronize(st));
+    n += add;
+    emitted += (uint64_t)add;
+    sorted_valid = false;
+    return (uint64_t)add;
+  }
+};
+
+// ------------------------------------------------------------------ C ABI plumbing
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return SDR_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return SDR_E_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return SDR_E_INVALID;
+  }
+}
+
+}  // namespace sdr
+
+using sdr::Error;
+using sdr::guard;
+
+struct sdr_solver {
+  int precision = SDR_PRECISION_F64;
+  std::unique_ptr<sdr::Solver<double>> d;
+  std::unique_ptr<sdr::Solver<float>> f;
+};
+
+template <class F>
+static int with(sdr_solver* s, F&& fn) {
+  if (!s) {
+    sdr::g_err = "null solver";
+    return SDR_E_INVALID;
+  }
+  return guard([&] {
+    if (s->d) fn(*s->d);
+    else fn(*s->f);
+  });
+}
+#define WITH(s, ...) with(s, [&](auto& S) { __VA_ARGS__; })
+
+extern "C" {
+
+const char* sdr_last_error(void) { return sdr::g_err.c_str(); }
+
+int sdr_device_info(int32_t* count, char* arch, size_t arch_len) {
+  return guard([&] {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (count) *count = c;
+    if (arch && arch_len) {
+      arch[0] = 0;
+      if (c > 0) {
+        hipDeviceProp_t p;
+        HIPCHK(hipGetDeviceProperties(&p, 0));
+        snprintf(arch, arch_len, "%s", p.gcnArchName);
+      }
+    }
+  });
+}
+
+int sdr_solver_new(const sdr_sdf* solid, double sim_dx, const sdr_fluid_params* fluid,
+                   const sdr_needle* needle, const sdr_options* opts, sdr_solver** out) {
+  return guard([&] {
+    if (!solid || !fluid || !needle || !out || !solid->data)
+      throw Error(SDR_E_INVALID, "null argument");
+    if (solid->nx < 2 || solid->ny < 2 || solid->nz < 2 || !(sim_dx > 0.0))
+      throw Error(SDR_E_INVALID, "invalid SDF dims or sim_dx");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
+    std::unique_ptr<sdr_solver> s(new sdr_solver());
+    s->precision = opts ? opts->precision : SDR_PRECISION_F64;
+    if (s->precision == SDR_PRECISION_F32) {
+      s->f.reset(new sdr::Solver<float>());
+      s->f->create(solid, sim_dx, fluid, needle, opts);
+    } else if (s->precision == SDR_PRECISION_F64) {
+      s->d.reset(new sdr::Solver<double>());
+      s->d->create(solid, sim_dx, fluid, needle, opts);
+    } else {
+      throw Error(SDR_E_INVALID, "unknown precision");
+    }
+    *out = s.release();
+  });
+}
+int sdr_solver_free(sdr_solver* s) {
+  delete s;
+  return SDR_OK;
+}
+int sdr_solver_set_outlet(sdr_solver* s, const sdr_aabb* o) { return WITH(s, S.set_outlet(o)); }
+int sdr_solver_set_solve_params(sdr_solver* s, const sdr_solve_params* p) {
+  return WITH(s, if (!p) throw Error(SDR_E_INVALID, "null params"); S.solve = *p);
+}
+int sdr_solver_set_needle(sdr_solver* s, const sdr_needle* nd) {
+  return WITH(s, if (!nd) throw Error(SDR_E_INVALID, "null needle"); S.needle = *nd);
+}
+int sdr_solver_step(sdr_solver* s, double dt) { return WITH(s, S.step(dt)); }
+int sdr_solver_substep(sdr_solver* s, double dt) { return WITH(s, S.substep(dt)); }
+int sdr_solver_run_stage(sdr_solver* s, int32_t stage, double dt) {
+  return WITH(s, S.run_stage(stage, dt); S.timer.collect());
+}
+int sdr_solver_redistribute(sdr_solver* s) { return WITH(s, S.redistribute()); }
+int sdr_solver_particle_count(const sdr_solver* s, uint64_t* n) {
+  return WITH(const_cast<sdr_solver*>(s), *n = (uint64_t)S.n);
+}
+int sdr_solver_get_particles(sdr_solver* s, double* pos, double* vel, uint64_t n) {
+  return WITH(s, S.get_particles(pos, vel, n));
+}
+int sdr_solver_set_particles(sdr_solver* s, const double* pos, const double* vel, uint64_t n) {
+  return WITH(s, if (n && !pos) throw Error(SDR_E_INVALID, "null positions"); S.set_particles(pos, vel, n));
+}
+int sdr_solver_get_dims(const sdr_solver* s, sdr_dims* d) {
+  return WITH(const_cast<sdr_solver*>(s), d->nx = S.nx; d->ny = S.ny; d->nz = S.nz; d->dx = S.dx;
+              d->origin = sdr_vec3{S.origin.x, S.origin.y, S.origin.z});
+}
+int sdr_solver_get_grid(sdr_solver* s, double* u, double* v, double* w) {
+  return WITH(s, S.copy_out(S.fu, S.nu(), u); S.copy_out(S.fv, S.nv(), v); S.copy_out(S.fw, S.nw(), w));
+}
+int sdr_solver_set_grid(sdr_solver* s, const double* u, const double* v, const double* w) {
+  return WITH(s, S.copy_in(S.fu, S.nu(), u); S.copy_in(S.fv, S.nv(), v); S.copy_in(S.fw, S.nw(), w));
+}
+int sdr_solver_get_saved(sdr_solver* s, double* u, double* v, double* w) {
+  return WITH(s, if (!S.have_saved) throw Error(SDR_E_STATE, "no saved grid yet");
+              S.copy_out(S.su, S.nu(), u); S.copy_out(S.sv, S.nv(), v); S.copy_out(S.sw, S.nw(), w));
+}
+int sdr_solver_set_saved(sdr_solver* s, const double* u, const double* v, const double* w) {
+  return WITH(s, S.copy_in(S.su, S.nu(), u); S.copy_in(S.sv, S.nv(), v); S.copy_in(S.sw, S.nw(), w);
+              S.have_saved = true);
+}
+int sdr_solver_get_cells(sdr_solver* s, uint8_t* c) {
+  return WITH(s, HIPCHK(hipMemcpyAsync(c, S.cells.p, (size_t)S.ncells(), hipMemcpyDeviceToHost, S.st));
+              HIPCHK(hipStreamSynchronize(S.st)));
+}
+int sdr_solver_set_cells(sdr_solver* s, const uint8_t* c) {
+  return WITH(s, HIPCHK(hipMemcpyAsync(S.cells.p, c, (size_t)S.ncells(), hipMemcpyHostToDevice, S.st));
+              HIPCHK(hipStreamSynchronize(S.st)));
+}
+int sdr_solver_get_pressure(sdr_solver* s, double* p) {
+  return WITH(s, S.copy_out(S.pressure, S.ncells(), p));
+}
+int sdr_solver_get_state(const sdr_solver* s, sdr_state* st) {
+  return WITH(const_cast<sdr_solver*>(s), memset(st, 0, sizeof *st); st->time = S.time;
+              st->steps = S.steps; st->last_pcg_iters = S.last_pcg_iters;
+              st->last_substeps = S.last_substeps; st->drained = S.drained; st->rng = S.rng;
+              st->emit_accum = S.emit_accum; st->emitted = S.emitted; st->culled = S.culled;
+              st->last_residual = S.last_residual);
+}
+int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
+  return WITH(s, S.time = st->time; S.steps = st->steps; S.last_pcg_iters = st->last_pcg_iters;
+              S.last_substeps = st->last_substeps; S.drained = st->drained; S.rng = st->rng;
+              S.emit_accum = st->emit_accum; S.emitted = st->emitted; S.culled = st->culled;
+              S.last_residual = st->last_residual);
+}
+int sdr_solver_enable_timing(sdr_solver* s, int32_t on) {
+  return WITH(s, S.timer.collect(); S.timer.on = on != 0; if (on) S.timer.acc.clear());
+}
+int sdr_solver_get_timing(sdr_solver* s, char* names, size_t name_len, double* ms, uint64_t* calls,
+                          uint64_t cap, uint64_t* count) {
+  return WITH(s, S.timer.collect(); uint64_t k = 0; for (auto& e : S.timer.acc) {
+    if (k >= cap) break;
+    if (names) snprintf(names + k * name_len, name_len, "%s", e.first.c_str());
+    if (ms) ms[k] = e.second.first;
+    if (calls) calls[k] = e.second.second;
+    ++k;
+  } if (count) *count = k);
+}
+int sdr_solver_seed_fill(sdr_solver* s, uint32_t per_cell, double y_lo, double y_hi,
+                         uint64_t seed, const sdr_vec3* vel, uint64_t* n_out) {
+  return WITH(s, uint64_t a = S.seed_fill(per_cell, y_lo, y_hi, seed, vel); if (n_out) *n_out = a);
+}
+
+int sdr_project_capturing_with_source(const sdr_dims* gd, double* u, double* v, double* w,
+                                      const uint8_t* cells, double rho, double dt,
+                                      const sdr_solve_params* params, const double* source,
+                                      double* pressure_out, sdr_solve_report* report,
+                                      const sdr_options* opts) {
+  return guard([&] {
+    if (!gd || !u || !v || !w || !cells) throw Error(SDR_E_INVALID, "null argument");
+    if (gd->nx < 2 || gd->ny < 2 || gd->nz < 2) throw Error(SDR_E_INVALID, "grid needs >= 2 cells per axis");
+    sdr_options o = opts ? *opts : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8};
+    if (o.precision != SDR_PRECISION_F64) throw Error(SDR_E_INVALID, "standalone projection is f64");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
+    HIPCHK(hipSetDevice(o.device));
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    sdr::Timer timer;
+    timer.st = st;
+    const int nx = (int)gd->nx, ny = (int)gd->ny, nz = (int)gd->nz;
+    const int64_t nc = (int64_t)nx * ny * nz, nu = (int64_t)(nx + 1) * ny * nz,
+                  nv = (int64_t)nx * (ny + 1) * nz, nw = (int64_t)nx * ny * (nz + 1);
+    sdr::Projector<double> pj;
+    pj.exact_dot = o.dot_mode == SDR_DOT_EXACT;
+    pj.init(nx, ny, nz, gd->dx, st, &timer);
+    sdr::Buf du, dv, dw, dc, ds, dp;
+    du.alloc(nu * 8), dv.alloc(nv * 8), dw.alloc(nw * 8), dc.alloc(nc), dp.alloc(nc * 8);
+    HIPCHK(hipMemcpyAsync(du.p, u, nu * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dv.p, v, nv * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dw.p, w, nw * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dc.p, cells, nc, hipMemcpyHostToDevice, st));
+    if (source) {
+      ds.alloc(nc * 8);
+      HIPCHK(hipMemcpyAsync(ds.p, source, nc * 8, hipMemcpyHostToDevice, st));
+    }
+    sdr::GridD<double> g{nx, ny, nz, gd->dx, gd->origin.x, gd->origin.y, gd->origin.z};
+    sdr_solve_params prm = params ? *params : sdr_solve_params{200, 1e-6};
+    sdr_solve_report r = pj.project(g, sdr::FacesD<double>{du.as<double>(), dv.as<double>(), dw.as<double>()},
+                                    dc.as<uint8_t>(), nullptr, source ? ds.as<double>() : nullptr, rho, dt,
+                                    prm.max_iters, prm.tol, false, 1.0, 0.0, dp.as<double>(), nullptr);
+    HIPCHK(hipMemcpyAsync(u, du.p, nu * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(v, dv.p, nv * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, st));
+    if (pressure_out) HIPCHK(hipMemcpyAsync(pressure_out, dp.p, nc * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (report) *report = r;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,246 @@
+"""Synthetic cavities of the benchmark and test workloads.
+
+These are build-defined inputs (the reference builds its cavities from meshes or from
+`SinusParams::implicit`, src/sinus.rs:89-119): an analytic signed-distance field on a
+node grid (`Sdf::from_fn`, src/sdf.rs:90-116), a needle, an outlet box at the socket
+opening (as src/scene.rs:299-305 derives it) and an initial jittered fill.  The same
+arrays feed the device library and the CPU oracle, so both start from identical bits.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .solver import Aabb, FluidParams, Needle, Sdf, SolveParams
+
+MM = 1e-3
+ML_PER_S = 1e-6
+
+
+def smooth_min(a, b, k):
+    """Polynomial smooth minimum (src/sinus.rs:66-74)."""
+    h = np.clip(0.5 + 0.5 * (b - a) / k, 0.0, 1.0)
+    return (b * (1.0 - h) + a * h) - k * h * (1.0 - h)
+
+
+def value_noise(x, y, z, scale, seed, octaves=1):
+    """Seeded smooth value noise in [-1, 1] (trilinear over a hashed lattice, smoothstep
+    weights).  The build's stand-in for the wall noise of the larger configs."""
+    out = np.zeros_like(x)
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        s = scale / (2 ** o)
+        gx, gy, gz = x / s, y / s, z / s
+        ix, iy, iz = np.floor(gx), np.floor(gy), np.floor(gz)
+        fx, fy, fz = gx - ix, gy - iy, gz - iz
+        ux, uy, uz = (f * f * (3 - 2 * f) for f in (fx, fy, fz))
+
+        def lat(a, b, c):
+            h = (a.astype(np.int64) * 73856093) ^ (b.astype(np.int64) * 19349663) ^ \
+                (c.astype(np.int64) * 83492791) ^ (seed * 2654435761 + o * 97)
+            h = (h * 0x5DEECE66D + 11) & 0xFFFFFFFF
+            return (h / 4294967295.0) * 2.0 - 1.0
+
+        v = 0.0
+        for dk in (0, 1):
+            for dj in (0, 1):
+                for di in (0, 1):
+                    w = (ux if di else 1 - ux) * (uy if dj else 1 - uy) * (uz if dk else 1 - uz)
+                    v = v + w * lat(ix + di, iy + dj, iz + dk)
+        out = out + amp * v
+        tot += amp
+        amp *= 0.5
+    return out / tot
+
+
+@dataclass
+class Scene:
+    name: str
+    sdf: Sdf
+    sim_dx: float
+    fluid: FluidParams
+    needle: Needle
+    outlet: Aabb | None
+    solve: SolveParams
+    dims: tuple
+    fill_lo: float = -1e9         # seed cavity cells with fill_lo <= centre y <= fill_y
+    fill_y: float = 1e9
+    fill_ppc: int = 8
+    fill_seed: int = 1
+    dt: float = 5e-5               # fixed substep of the workload
+
+
+def _cavity_fn(semi, socket_r, blend, ybot_open=True, noise=None, power=2.0):
+    ax, ay, az = semi
+
+    def f(x, y, z):
+        if power == 2.0:
+            body = np.sqrt((x / ax) ** 2 + (y / ay) ** 2 + (z / az) ** 2) - 1.0
+        else:
+            body = (np.abs(x / ax) ** power + np.abs(y / ay) ** power +
+                    np.abs(z / az) ** power) ** (1.0 / power) - 1.0
+        body = body * min(semi)
+        radial = np.sqrt(x * x + z * z) - socket_r
+        axial = y - (-0.5 * ay)
+        socket = np.maximum(radial, axial)
+        phi = smooth_min(body, socket, blend)
+        if noise is not None:
+            amp, scale, seed, octaves = noise
+            phi = phi + amp * value_noise(x, y, z, scale, seed, octaves)
+        return phi
+
+    return f
+
+
+def _socket_outlet(ybot, r, sim_dx):
+    rr = r + 1.5 * sim_dx
+    return Aabb((-rr, ybot - 2.0 * sim_dx, -rr), (rr, ybot + 1.5 * sim_dx, rr))
+
+
+def cavity_scene(level: str) -> Scene:
+    """The benchmark cavities.
+
+    c1: 15x18x14 mm ellipsoid + 2 mm socket, 1 mm grid 36x42x34, 2 ml/s 0.5 mm jet.
+    c2: the same cavity at 0.5 mm (72x84x68), 0.3 mm seeded wall noise, 4 ml/s.
+    c3: 256^3 grid, a box-filling rounded cavity (superellipsoid, power 12) + socket,
+        filled to ~130M particles at 8 per cell.
+    c3a: 256^3 grid, the literal 60x72x56-cell ellipsoid (~8M particles).
+    """
+    fluid = FluidParams()
+    solve = SolveParams(200, 1e-6)
+    if level in ("c1", "c2"):
+        dx = 1.0 * MM if level == "c1" else 0.5 * MM
+        semi = (15 * MM, 18 * MM, 14 * MM)
+        noise = None if level == "c1" else (0.3 * MM, 3.0 * MM, 7, 2)
+        origin = (-18 * MM, -21 * MM, -17 * MM)
+        dims = (37, 43, 35) if level == "c1" else (73, 85, 69)
+        sdf = Sdf.from_fn(origin, dx, dims, _cavity_fn(semi, 2 * MM, 1.5 * MM, noise=noise))
+        flow = (2.0 if level == "c1" else 4.0) * ML_PER_S
+        needle = Needle(tip=(0.0, -4 * MM, -10 * MM), axis=(0.0, 0.0, 1.0), radius=0.25 * MM,
+                        flow_rate=flow, inject_until=math.inf if level == "c1" else 0.25)
+        outlet = _socket_outlet(origin[1], 2 * MM, dx)
+        grid = (36, 42, 34) if level == "c1" else (72, 84, 68)
+        fill_y = 1.0 * MM if level == "c1" else 1e9
+        return Scene(level, sdf, dx, fluid, needle, outlet, solve, grid, fill_lo=-17 * MM,
+                     fill_y=fill_y, dt=5e-5 if level == "c1" else 2.4e-5)
+    if level in ("c3", "c3a"):
+        n = 256
+        dx = 0.25 * MM
+        half = n * dx / 2.0
+        origin = (-half, -half, -half)
+        if level == "c3":
+            semi = (0.495 * n * dx, 0.495 * n * dx, 0.495 * n * dx)
+            fn = _cavity_fn(semi, 8 * dx, 4 * dx, power=12.0)
+            fill_y = 0.9 * half
+        else:
+            semi = (60 * dx, 72 * dx, 56 * dx)
+            fn = _cavity_fn(semi, 8 * dx, 4 * dx)
+            fill_y = 1e9
+        sdf = Sdf.from_fn(origin, dx, (n + 1, n + 1, n + 1), fn)
+        needle = Needle(tip=(0.0, -0.25 * half, -0.5 * half), axis=(0.0, 0.0, 1.0),
+                        radius=2 * dx, flow_rate=4.0 * ML_PER_S)
+        outlet = _socket_outlet(origin[1], 8 * dx, dx)
+        return Scene(level, sdf, dx, fluid, needle, outlet, solve, (n, n, n),
+                     fill_lo=origin[1] + 4 * dx, fill_y=fill_y, dt=2e-5)
+    raise ValueError(level)
+
+
+def box_scene() -> Scene:
+    """`box_solver` of the reference tests (src/solver.rs:1170-1189): a 24 mm box cavity
+    in a 16^3-node SDF at 2 mm, needle pointing down, 8 particles per cell."""
+    dx = 0.002
+    a = 0.012
+    sdf = Sdf.from_fn((-0.016, -0.016, -0.016), dx, (16, 16, 16),
+                      lambda x, y, z: np.maximum(np.maximum(np.abs(x) - a, np.abs(y) - a),
+                                                 np.abs(z) - a))
+    needle = Needle(tip=(0.0, 0.010, 0.0), axis=(0.0, -1.0, 0.0), radius=0.0006, flow_rate=5.0e-6)
+    return Scene("box", sdf, dx, FluidParams(particles_per_cell=8), needle, None, SolveParams(),
+                 (0, 0, 0))
+
+
+def sinus_scene(sim_dx: float = 0.0025) -> Scene:
+    """A coarse parametric-sinus scene in the spirit of `small_solver`
+    (src/solver.rs:1016-1043): the default `SinusParams` cavity (src/sinus.rs:47-60)
+    sampled analytically (`Sdf::from_fn`) instead of through its mesh, 16 particles per
+    cell, needle at the socket, outlet at the socket bottom."""
+    semi = np.array([0.017, 0.016, 0.013])
+    taper, bump, sx, sz, sr, sdepth = 0.6, 0.12, 0.006, 0.0, 0.0025, 0.006
+
+    def implicit(x, y, z):  # src/sinus.rs:89-119
+        qx, qy, qz = x, y, z
+        h = ((qy / semi[1]) + 1.0) * 0.5
+        scale = taper + (1.0 - taper) * np.clip(h, 0.0, 1.5)
+        b = bump * (0.6 * np.sin(4.0 * qx / semi[0]) * np.cos(3.0 * qz / semi[2]) +
+                    0.4 * np.sin(5.0 * qy / semi[1] + 1.3))
+        body = np.sqrt((qx / (semi[0] * scale)) ** 2 + (qy / semi[1]) ** 2 +
+                       (qz / (semi[2] * scale)) ** 2) - (1.0 + b)
+        body = body * semi.min()
+        radial = np.sqrt((x - sx) ** 2 + (z - sz) ** 2) - sr
+        ytop = -0.5 * semi[1]
+        ybot = -semi[1] - sdepth
+        axial = np.maximum(y - ytop, ybot - y)
+        return smooth_min(body, np.maximum(radial, axial), 0.0015)
+
+    pad = 0.006
+    lo = np.array([-semi[0] * 1.12 - pad, -semi[1] - sdepth - pad, -semi[2] * 1.12 - pad])
+    hi = np.array([semi[0] * 1.12 + pad, semi[1] * 1.12 + pad, semi[2] * 1.12 + pad])
+    dims = tuple(int(math.ceil((hi[a] - lo[a]) / sim_dx)) + 1 for a in range(3))
+    sdf = Sdf.from_fn(tuple(lo), sim_dx, dims, implicit)
+    tip = (sx, -0.5 * semi[1] + semi[1] * 0.15, sz)  # src/sinus.rs:154-160
+    needle = Needle(tip=tip, axis=(0.0, 1.0, 0.0), radius=0.0006, flow_rate=5.0e-6)
+    yb = -semi[1] - sdepth
+    outlet = Aabb((sx - 0.004, yb - 0.002, sz - 0.004), (sx + 0.004, yb + 0.002, sz + 0.004))
+    return Scene("sinus", sdf, sim_dx, FluidParams(particles_per_cell=16), needle, outlet,
+                 SolveParams(), (0, 0, 0))
+
+
+def make_solver(scene: Scene, lib=None, fill: bool = True, **kw):
+    """Build a `Solver` for a scene on `lib` (device library by default), apply its outlet
+    and solve params, and seed the initial fill (device fill kernel when available)."""
+    from .solver import Solver
+
+    s = Solver(scene.sdf, scene.sim_dx, scene.fluid, scene.needle, lib=lib, **kw)
+    if scene.dims != (0, 0, 0):
+        assert (s.nx, s.ny, s.nz) == scene.dims, ((s.nx, s.ny, s.nz), scene.dims)
+    s.outlet = scene.outlet
+    s.solve = scene.solve
+    return s
+
+
+def fill_particles(solver, scene: Scene, mask: np.ndarray) -> np.ndarray:
+    """Host-side initial fill (identical to the device `seed_fill` kernel): `fill_ppc`
+    particles per cavity cell whose centre is at or below `fill_y`, at centre +
+    splitmix64 jitter of (seed, index).  Returns (N,3) positions."""
+    nz, ny, nx = mask.shape
+    kk, jj, ii = np.nonzero(mask)
+    order = np.lexsort((ii, jj, kk))
+    ii, jj, kk = ii[order], jj[order], kk[order]
+    o, dx = solver.origin, solver.dx
+    cy = o[1] + (jj + 0.5) * dx
+    sel = (cy >= scene.fill_lo) & (cy <= scene.fill_y)
+    ii, jj, kk = ii[sel], jj[sel], kk[sel]
+    ppc = scene.fill_ppc
+    t = np.arange(ii.size * ppc, dtype=np.uint64)
+    key = np.uint64(scene.fill_seed) * np.uint64(1 << 32) + t
+
+    def mix(k):
+        with np.errstate(over="ignore"):
+            z = k + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return ((z >> np.uint64(11)).astype(np.float64) / float(1 << 53) - 0.5) * 0.6
+
+    with np.errstate(over="ignore"):
+        k3 = key * np.uint64(3)
+    j = np.stack([mix(k3) / 0.6 * 0.9999, mix(k3 + np.uint64(1)) / 0.6 * 0.9999,
+                  mix(k3 + np.uint64(2)) / 0.6 * 0.9999], axis=1)
+    ci = np.repeat(ii, ppc).astype(np.float64)
+    cj = np.repeat(jj, ppc).astype(np.float64)
+    ck = np.repeat(kk, ppc).astype(np.float64)
+    cx = o[0] + (ci + 0.5) * dx
+    cyy = o[1] + (cj + 0.5) * dx
+    cz = o[2] + (ck + 0.5) * dx
+    return np.stack([cx + j[:, 0] * dx, cyy + j[:, 1] * dx, cz + j[:, 2] * dx], axis=1)
