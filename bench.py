@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark of the FLIP/PIC substep on one MI355X.
+
+A "step" is one `Solver::substep(dt)` (emit, classify, P2G, forces, MIC(0)-PCG
+projection, extrapolation, G2P, advection, culling, redistribution) over the 256^3
+cavity workload: a box-filling rounded cavity with a socket channel, ~130M particles
+seeded 8 per cell, fixed dt.  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line with particle-substeps/s (whole job), the pressure-solve time,
+the HBM-roofline fraction of the dominant kernel (HIP events on the solver's stream)
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-substeps/s + pressure-solve ms at 256³ vs FP64 GPU; p99 scenario latency"
+
+
+def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> float | None:
+    """Algorithmic HBM bytes of one call of a timed region (the traffic the mathematics
+    needs, independent of the implementation): s = scalar size, nc = cells, np_ = particles."""
+    flags = 1
+    table = {
+        "mic_fwd": nc * (3 * s + flags),        # read r, precon, flags; write q
+        "mic_bwd": nc * (3 * s + flags),        # read q, precon, flags; write z
+        "mic_build": nc * (s + flags),          # read flags; write precon
+        "spmv_dot": nc * (4 * s + flags),       # spmv: read s, flags, write As; dot: read s, As
+        "axpy_norm": nc * 6 * s,                # p += a s; r -= a As; max|r|
+        "dot_update": nc * 5 * s,               # dot(z, r); s = z + b s
+        "p2g": np_ * 6 * s + 3 * nc * s,        # read x, v; write u, v, w
+        "g2p": np_ * 9 * s + 6 * nc * s,        # read x, v; write v; read u,v,w and saved
+        "advect": np_ * 6 * s + 3 * nc * s + nsdf * s,  # read/write x; read u,v,w; SDF
+    }
+    return table.get(name)
+
+
+def cpu_baseline(scene, sdf, gpu_iters: float, budget_s: float, threads: int):
+    """The CPU oracle (f64, reference operand order and threading) on a bounded sample
+    of the same workload: every stage on the full 256^3 grid, the particle stages on a
+    strided 1/`stride` subset of the particles, the PCG at 2 and 4 iterations.  The
+    per-substep time is composed as grid stages + PCG per-iteration x the GPU's
+    iteration count + particle stages x stride."""
+    import oracle
+    from sdr_amd import scenes
+
+    oracle.set_threads(threads)
+    lib = oracle.load()
+    s = scenes.make_solver(scene, lib=lib)
+    mask = oracle.cavity_mask(s)
+    kk, jj, ii = np.nonzero(mask)
+    cy = s.origin[1] + (jj + 0.5) * s.dx
+    n_full = int(((cy >= scene.fill_lo) & (cy <= scene.fill_y)).sum()) * scene.fill_ppc
+    stride = max(1, n_full // 1_000_000)
+    sub = np.ascontiguousarray(scenes.fill_particles(s, scene, mask, stride=stride))
+    s.set_particles(sub)
+    dt = scene.dt
+    times = {}
+    stages = ["emit", "classify", "p2g", "save", "forces"]
+    for st in stages:
+        t = time.perf_counter()
+        s.run_stage(st, dt)
+        times[st] = time.perf_counter() - t
+        print(f"[cpu] {st}: {times[st]:.3f}s", file=sys.stderr, flush=True)
+    # PCG cost: two capped solves on the same post-P2G state.
+    from sdr_amd.solver import SolveParams
+    u, v, w = s.grid()
+    proj = {}
+    for m in (2, 4):  # capped solves: the per-iteration slope and the fixed part
+        s.set_grid(u, v, w)
+        s.solve = SolveParams(m, 0.0)
+        t = time.perf_counter()
+        s.run_stage("project", dt)
+        proj[m] = time.perf_counter() - t
+        print(f"[cpu] project ({m} iterations): {proj[m]:.3f}s", file=sys.stderr, flush=True)
+    per_iter = max((proj[4] - proj[2]) / 2.0, 0.0)
+    fixed_proj = max(proj[2] - 2 * per_iter, 0.0)
+    for st in ["extrapolate", "g2p", "advect", "bounds", "redistribute"]:
+        t = time.perf_counter()
+        s.run_stage(st, dt)
+        times[st] = time.perf_counter() - t
+        print(f"[cpu] {st}: {times[st]:.3f}s", file=sys.stderr, flush=True)
+    particle_stages = ["p2g", "g2p", "advect", "bounds", "redistribute", "classify"]
+    t_particles = sum(times[k] for k in particle_stages) * stride
+    t_grid = sum(v for k, v in times.items() if k not in particle_stages)
+    t_sub = t_grid + t_particles + fixed_proj + per_iter * gpu_iters
+    sample_s = sum(times.values()) + sum(proj.values())
+    s.close()
+    return {
+        "value": n_full / t_sub,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"C++ f64 oracle, 1 substep of the same 256^3 workload: all grid stages on "
+                   f"the full grid, particle stages on every {stride}th particle "
+                   f"({sub.shape[0]} of {n_full}) scaled x{stride}, PCG timed at 2 and 4 "
+                   f"iterations and scaled to the GPU's {gpu_iters:.1f} iterations; "
+                   f"{sample_s:.1f} s of CPU work, composed CPU substep {t_sub:.1f} s"),
+        "ms_per_substep": t_sub * 1e3,
+        "pcg_ms_per_iter": per_iter * 1e3,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c3a"])
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--dot", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the substep runs "
+              "on one device; see DESIGN.md for how it would shard). Use --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import sdr_amd
+    from sdr_amd import _abi as A
+    from sdr_amd import scenes
+
+    ndev, arch = sdr_amd.device_info()
+    if ndev < 1:
+        print("no HIP device", file=sys.stderr)
+        return 3
+    t0 = time.time()
+    scene = scenes.cavity_scene(args.config)
+    prec = A.PRECISION_F32 if args.precision == "f32" else A.PRECISION_F64
+    dot_mode = A.DOT_FAST if args.dot == "fast" else A.DOT_EXACT
+    s = scenes.make_solver(scene, precision=prec, dot_mode=dot_mode)
+    n0 = s.seed_fill(scene.fill_ppc, scene.fill_lo, scene.fill_y, scene.fill_seed)
+    print(f"[bench] {args.config} grid {s.nx}x{s.ny}x{s.nz} particles {n0} on {arch}; "
+          f"setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dt = scene.dt
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        s.substep(dt)
+        s.synchronize()
+        print(f"[bench] warmup {k}: {time.perf_counter() - t:.3f}s iters {s.last_pcg_iters}",
+              file=sys.stderr, flush=True)
+    s.enable_timing(True)
+    s.synchronize()
+    particle_steps = 0
+    iters = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        particle_steps += s.particle_count()
+        t = time.perf_counter()
+        s.substep(dt)
+        iters.append(s.last_pcg_iters)
+        print(f"[bench] step {k}: {time.perf_counter() - t:.3f}s pcg iters {iters[-1]}",
+              file=sys.stderr, flush=True)
+    s.synchronize()
+    elapsed = time.perf_counter() - t_start
+    timing = s.timing()
+    s.enable_timing(False)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = particle_steps / elapsed
+    sz = 4 if prec == A.PRECISION_F32 else 8
+    nc = s.nx * s.ny * s.nz
+    nsdf = scene.sdf.nx * scene.sdf.ny * scene.sdf.nz
+    npart = particle_steps / max(args.steps, 1)
+    solve_regions = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_bwd", "spmv_dot",
+                     "axpy_norm", "dot_update", "gradient"]
+    pressure_ms = sum(timing.get(r, (0.0, 0))[0] for r in solve_regions) / args.steps
+    # Dominant kernel region by total time; roofline from its average call duration.
+    best, best_t = None, -1.0
+    for name, (ms, calls) in timing.items():
+        if region_bytes(name, sz, nc, 0, 0, 0) is None:
+            continue
+        if ms > best_t:
+            best, best_t = name, ms
+    roof = None
+    if best is not None:
+        ms, calls = timing[best]
+        per_call_s = ms / calls / 1e3
+        by = region_bytes(best, sz, nc, nc, npart, nsdf)
+        ach = by / per_call_s / 1e9
+        roof = {"bound": "hbm", "kernel": best, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_call": int(by), "avg_call_ms": round(per_call_s * 1e3, 4)}
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(scene, scene.sdf, float(np.mean(iters)), 20.0, threads)
+        except Exception as e:  # the GPU number stands on its own
+            cpu = {"error": repr(e)}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if prec == A.PRECISION_F32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: 256^3 MAC grid rounded cavity + socket, "
+                               f"~{npart / 1e6:.0f}M particles 8/cell, fixed dt {dt:g} s"
+                   if args.config == "c3" else args.config,
+                   "grid": [s.nx, s.ny, s.nz], "particles": int(npart), "dt": dt,
+                   "precision": args.precision, "dot": args.dot, "pcg_tol": scene.solve.tol,
+                   "pcg_max_iters": scene.solve.max_iters},
+        "pressure_solve_ms": round(pressure_ms, 3),
+        "pcg_iters": [int(i) for i in iters],
+        "regions_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(timing.items())},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu and "value" in cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out))
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -122,6 +122,8 @@ typedef struct sdr_state {
   uint64_t emitted;
   uint64_t culled;
   double last_residual;
+  uint64_t redist_active; /* particles visited by the last substep's redistribution passes */
+  uint64_t redist_moved;  /* particles those passes moved */
 } sdr_state;
 
 typedef struct sdr_dims {
@@ -195,6 +197,9 @@ int sdr_solver_set_cells(sdr_solver* s, const uint8_t* cells);
 int sdr_solver_get_pressure(sdr_solver* s, double* p);
 int sdr_solver_get_state(const sdr_solver* s, sdr_state* st);
 int sdr_solver_set_state(sdr_solver* s, const sdr_state* st);
+
+/* Block until every operation queued on the solver's stream has finished. */
+int sdr_solver_synchronize(sdr_solver* s);
 
 /* Per-kernel timing of the last substep, filled when enabled (HIP events on the
  * solver's stream).  names/ms arrays of length cap; *count receives the number written. */

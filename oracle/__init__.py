@@ -43,8 +43,8 @@ def load() -> _abi.Lib:
         d.orc_hash_jitter.argtypes = [C.c_uint64, _abi.c_double_p]
         d.orc_rng_units.argtypes = [C.c_uint64, C.c_uint64, _abi.c_double_p, _abi.c_u64_p]
         d.orc_laplacian.argtypes = [C.c_uint64] * 3 + [_abi.c_u8_p] + [_abi.c_double_p] * 4
-        d.orc_mic0.argtypes = [C.c_uint64] * 3 + [_abi.c_u8_p, _abi.c_double_p, _abi.c_double_p,
-                                                  _abi.c_double_p]
+        d.orc_mic0.argtypes = [C.c_uint64] * 3 + [_abi.c_u8_p, _abi.c_double_p, C.c_double,
+                                                  _abi.c_double_p, _abi.c_double_p]
         d.orc_apply_a.argtypes = [C.c_uint64] * 3 + [_abi.c_u8_p, _abi.c_double_p, _abi.c_double_p]
         d.orc_dot.argtypes = [_abi.c_double_p, _abi.c_double_p, C.c_uint64, _abi.c_double_p]
         d.orc_extrapolate.argtypes = [_abi.c_double_p, _abi.c_u8_p, C.c_uint64, C.c_uint64,
@@ -106,13 +106,14 @@ def laplacian(cells: np.ndarray):
     return d, px, py, pz
 
 
-def mic0(cells: np.ndarray, r: np.ndarray | None = None):
+def mic0(cells: np.ndarray, r: np.ndarray | None = None, tau: float = -1.0):
     nz, ny, nx = cells.shape
     c = np.ascontiguousarray(cells, np.uint8)
     pc = np.zeros(cells.shape)
     z = np.zeros(cells.shape) if r is not None else None
     rr = None if r is None else np.ascontiguousarray(r, np.float64)
-    load().dll.orc_mic0(nx, ny, nz, _abi.u8ptr(c), _abi.dptr(rr), _abi.dptr(pc), _abi.dptr(z))
+    load().dll.orc_mic0(nx, ny, nz, _abi.u8ptr(c), _abi.dptr(rr), tau, _abi.dptr(pc),
+                        _abi.dptr(z))
     return pc, z
 
 

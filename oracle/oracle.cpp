@@ -275,8 +275,9 @@ void apply_a(const Laplacian& a, const uint8_t* cells, const double* x, double* 
   }
 }
 
-std::vector<double> build_mic0(const Laplacian& a, const uint8_t* cells) {  // pressure.rs:442-513
-  const double TAU = 0.97, SIGMA = 0.25;
+std::vector<double> build_mic0(const Laplacian& a, const uint8_t* cells,
+                               double TAU = 0.97) {  // pressure.rs:442-513
+  const double SIGMA = 0.25;
   const uint64_t nx = a.nx, ny = a.ny, nz = a.nz;
   std::vector<double> precon(nx * ny * nz, 0.0);
   for (uint64_t k = 0; k < nz; ++k)
@@ -529,6 +530,7 @@ struct Solver {
   std::vector<double> source;
   Rng rng{0x5deece66dull | 1};
   double emit_accum = 0;
+  uint64_t redist_active = 0, redist_moved = 0;  // diagnostics of the last redistribute
 
   void create(Sdf s, double sim_dx, Fluid f, Needle nd) {  // solver.rs:184-222
     solid = std::move(s);
@@ -699,11 +701,16 @@ struct Solver {
     uint32_t cap = (uint32_t)as_usize(std::ceil(rmax(fluid.redist_cap_factor, 1.0) * (double)target));
     V3 gdir = normalize_or_zero(fluid.gravity);
     std::vector<uint32_t> count(ncells, 0);
+    redist_active = redist_moved = 0;
     for (int sweep = 0; sweep < 4; ++sweep) {
       std::fill(count.begin(), count.end(), 0u);
       for (const V3& p : pos) {
         uint64_t i, j, k;
         if (cell_of(p, i, j, k)) count[(k * ny + j) * nx + i] += 1;
+      }
+      for (const V3& p : pos) {  // diagnostics: particles in over-cap cells
+        uint64_t i, j, k;
+        if (cell_of(p, i, j, k) && count[(k * ny + j) * nx + i] > cap) redist_active++;
       }
       for (uint64_t idx = 0; idx < pos.size(); ++idx) {
         uint64_t i, j, k;
@@ -742,6 +749,7 @@ struct Solver {
           }
         }
         if (have) {
+          redist_moved++;
           count[c] -= 1;
           count[best] += 1;
           uint64_t ci = best % nx, cj = (best / nx) % ny, ck = best / (nx * ny);
@@ -1067,6 +1075,7 @@ int orc_solver_get_state(const void* h, sdr_state* st) {
   st->last_substeps = s->last_substeps, st->drained = s->drained, st->rng = s->rng.s;
   st->emit_accum = s->emit_accum, st->emitted = s->emitted, st->culled = s->culled;
   st->last_residual = s->last_residual;
+  st->redist_active = s->redist_active, st->redist_moved = s->redist_moved;
   return 0;
 }
 int orc_solver_set_state(void* h, const sdr_state* st) {
@@ -1143,12 +1152,14 @@ int orc_laplacian(uint64_t nx, uint64_t ny, uint64_t nz, const uint8_t* cells, d
       memcpy(pz, a.plus_z.data(), n);
   return 0;
 }
+// tau < 0 selects the reference constant 0.97; other values exist only to check the
+// IC(0) (tau = 0) and modified (tau = 1) identities of the factor in tests.
 int orc_mic0(uint64_t nx, uint64_t ny, uint64_t nz, const uint8_t* cells, const double* r,
-             double* precon_out, double* z_out) {
+             double tau, double* precon_out, double* z_out) {
   MacGrid g;
   g.init(nx, ny, nz, 1.0, v3(0, 0, 0));
   Laplacian a = assemble(g, cells);
-  std::vector<double> pc = build_mic0(a, cells);
+  std::vector<double> pc = build_mic0(a, cells, tau < 0 ? 0.97 : tau);
   if (precon_out) memcpy(precon_out, pc.data(), pc.size() * 8);
   if (r && z_out) apply_mic0(a, pc.data(), cells, r, z_out);
   return 0;

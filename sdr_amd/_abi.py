@@ -76,7 +76,8 @@ class State(C.Structure):
     _fields_ = [("time", C.c_double), ("steps", C.c_uint64), ("last_pcg_iters", C.c_uint64),
                 ("last_substeps", C.c_uint32), ("_pad", C.c_uint32), ("drained", C.c_uint64),
                 ("rng", C.c_uint64), ("emit_accum", C.c_double), ("emitted", C.c_uint64),
-                ("culled", C.c_uint64), ("last_residual", C.c_double)]
+                ("culled", C.c_uint64), ("last_residual", C.c_double),
+                ("redist_active", C.c_uint64), ("redist_moved", C.c_uint64)]
 
 
 class Dims(C.Structure):
@@ -118,6 +119,7 @@ SIGNATURES = {
 # Entry points only the device library has.
 DEVICE_ONLY = {
     "device_info": [C.POINTER(C.c_int32), C.c_char_p, C.c_size_t],
+    "solver_synchronize": [P],
     "solver_enable_timing": [P, C.c_int32],
     "solver_get_timing": [P, C.c_char_p, C.c_size_t, c_double_p, c_u64_p, C.c_uint64, c_u64_p],
     "solver_seed_fill": [P, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Vec3),

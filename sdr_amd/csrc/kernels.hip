@@ -769,8 +769,10 @@ __global__ void k_active_compact(const uint32_t* __restrict__ flag, const uint32
 __global__ void k_redist_decide(const uint32_t* __restrict__ act, int64_t nact,
                                 const int32_t* __restrict__ cellkey, uint32_t* count,
                                 const uint8_t* __restrict__ mask, int nx, int ny, int nz,
-                                uint32_t cap, double gx, double gy, double gz, int32_t* dest) {
+                                uint32_t cap, double gx, double gy, double gz, int32_t* dest,
+                                unsigned long long* moved) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned long long nmoved = 0;
   // dir . gdir for -x,+x,-y,+y,-z,+z (math.rs:42-44 with the unit dirs of solver.rs:492-509)
   double dn[6];
   dn[0] = -1.0 * gx + 0.0 * gy + 0.0 * gz;
@@ -815,8 +817,16 @@ __global__ void k_redist_decide(const uint32_t* __restrict__ act, int64_t nact,
       count[c] -= 1;
       count[best] += 1;
       dest[a] = (int32_t)best;
+      ++nmoved;
     }
   }
+  *moved = nmoved;
+}
+
+__global__ void k_gather_cells(const uint32_t* __restrict__ act, const int32_t* __restrict__ cellkey,
+                               int64_t nact, int32_t* act_cell) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < nact) act_cell[a] = cellkey[act[a]];
 }
 
 template <class T>
@@ -1120,10 +1130,16 @@ void launch_active_compact(const uint32_t* flag, const uint32_t* pos, int64_t n,
 }
 void launch_redist_decide(const uint32_t* act, int64_t nact, const int32_t* cellkey,
                           uint32_t* count, const uint8_t* mask, int nx, int ny, int nz, uint32_t cap,
-                          double gx, double gy, double gz, int32_t* dest, hipStream_t st) {
+                          double gx, double gy, double gz, int32_t* dest, unsigned long long* moved,
+                          hipStream_t st) {
   if (nact <= 0) return;
   k_redist_decide<<<1, 1, 0, st>>>(act, nact, cellkey, count, mask, nx, ny, nz, cap, gx, gy, gz,
-                                   dest);
+                                   dest, moved);
+}
+void launch_gather_cells(const uint32_t* act, const int32_t* cellkey, int64_t nact, int32_t* act_cell,
+                         hipStream_t st) {
+  if (nact <= 0) return;
+  k_gather_cells<<<grid_for(nact), kBlock, 0, st>>>(act, cellkey, nact, act_cell);
 }
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act, const int32_t* dest, int64_t nact,

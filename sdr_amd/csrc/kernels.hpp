@@ -126,7 +126,10 @@ void launch_active_compact(const uint32_t* flag, const uint32_t* pos, int64_t n,
                            hipStream_t st);
 void launch_redist_decide(const uint32_t* act_slot_sorted, int64_t nact, const int32_t* cellkey,
                           uint32_t* count, const uint8_t* mask, int nx, int ny, int nz, uint32_t cap,
-                          double gx, double gy, double gz, int32_t* dest, hipStream_t st);
+                          double gx, double gy, double gz, int32_t* dest, unsigned long long* moved,
+                          hipStream_t st);
+void launch_gather_cells(const uint32_t* act, const int32_t* cellkey, int64_t nact, int32_t* act_cell,
+                         hipStream_t st);
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act_slot_sorted, const int32_t* dest,
                          int64_t nact, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st);

@@ -1,0 +1,44 @@
+// mic_pipe.hpp — launch interface of the pipelined MIC(0) sweeps (mic_pipe.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace sdr {
+
+// Cross-tile state of one sweep.  Halo words carry `tag` in their high 32 bits; a
+// consumer waits until the words it needs carry the current sweep's tag, so tags must
+// differ between consecutive sweeps that share the buffers (and the buffers start
+// zeroed, tags start at 1).  `error` is set to 1 if any bounded wait expired.
+template <class T>
+struct MicPipeArgs {
+  uint64_t* halo_x;  // [(ntile_i-1) x nz x NJ x V x words(T)]
+  uint64_t* halo_z;  // [(ntile_k-1) x NJ x (ntile_i*TI*64) x V x words(T)]
+  T* sink;           // [1024] per-thread dummy words for out-of-grid steps
+  int* error;
+  long spin_limit;
+  int ntile_i, ntile_k;
+  uint32_t tag;
+};
+
+struct MicPipeShape {
+  int ti, tk, ntile_i, ntile_k;
+};
+
+MicPipeShape mic_pipe_shape(Shear sh, int variant);
+
+// Sizes (64-bit words) of the halo buffers for a variant, with 2 values per cell
+// (enough for every op) of up to 2 words each.
+inline void mic_pipe_halo_sizes(Shear sh, MicPipeShape s, size_t* hx, size_t* hz) {
+  *hx = (size_t)(s.ntile_i > 1 ? s.ntile_i - 1 : 1) * sh.nz * sh.NJ * 4;
+  *hz = (size_t)(s.ntile_k > 1 ? s.ntile_k - 1 : 1) * sh.NJ * (size_t)(s.ntile_i * s.ti * 64) * 4;
+}
+
+template <class T>
+void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* precon, const T* in,
+                     T* out, const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st);
+
+}  // namespace sdr

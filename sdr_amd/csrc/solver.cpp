@@ -25,6 +25,7 @@
 
 #include "../../include/sdr_amd.h"
 #include "kernels.hpp"
+#include "mic_pipe.hpp"
 #include "sort.hpp"
 
 namespace sdr {
@@ -191,7 +192,10 @@ struct Projector {
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
   Buf b, z, s, p, as, q, precon, flags, partials, scal;
+  Buf halo_x, halo_z, pipe_err, sink;
+  uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
+  int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
   void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
@@ -203,15 +207,55 @@ struct Projector {
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
     partials.alloc((size_t)nb * sizeof(double));
     scal.alloc(sizeof(PcgScalars));
-    host_scal.ensure(sizeof(PcgScalars));
+    host_scal.ensure(sizeof(PcgScalars) + 16);
+    // f64 runs 512-thread tiles (1x8 planes), f32 1024-thread tiles (1x16 planes)
+    mic_variant = sizeof(T) == 8 ? 1 : 0;
+    if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
+    size_t hx = 0, hz = 0;
+    for (int v = 0; v < 6; ++v) {
+      MicPipeShape ps = mic_pipe_shape(sh, v);
+      size_t a, c;
+      mic_pipe_halo_sizes(sh, ps, &a, &c);
+      hx = std::max(hx, a), hz = std::max(hz, c);
+    }
+    halo_x.alloc(hx * sizeof(uint64_t));
+    halo_z.alloc(hz * sizeof(uint64_t));
+    pipe_err.alloc(sizeof(int));
+    sink.alloc(1024 * sizeof(T));
+    HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
+    HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
+    HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
   }
 
   PcgScalars* sc() const { return scal.as<PcgScalars>(); }
 
   void mic_sweep(int op, const T* in, T* out) {
-    const int planes = sh.NJ + sh.nz - 1;
-    for (int sgm = 0; sgm < planes; ++sgm)
-      launch_mic_plane<T>(op, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sgm, sc(), st);
+    if (mic_variant < 0) {
+      const int planes = sh.NJ + sh.nz - 1;
+      for (int sgm = 0; sgm < planes; ++sgm)
+        launch_mic_plane<T>(op, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sgm, sc(), st);
+      return;
+    }
+    MicPipeShape ps = mic_pipe_shape(sh, mic_variant);
+    if (++sweep_tag == 0) {  // tags wrapped: clear the halo words so no stale tag matches
+      HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
+      HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
+      sweep_tag = 1;
+    }
+    MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
+                      1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag};
+    launch_mic_pipe<T>(op, mic_variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
+  }
+
+  // A bounded wait of a pipelined sweep expired: the solve result is not valid.
+  void check_pipe() {
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, pipe_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (e) {
+      HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
+      throw Error(SDR_E_NUMERIC, "pipelined MIC(0) sweep: a tile's bounded wait expired");
+    }
   }
 
   void dot(const T* a, const T* bb, int op, double tol) {
@@ -245,8 +289,11 @@ struct Projector {
     }
     T* r = b.as<T>();
     {
-      Scope sc_(*timer, "mic_apply");
+      Scope sc_(*timer, "mic_fwd");
       mic_sweep(kMicFwd, r, q.as<T>());
+    }
+    {
+      Scope sc_(*timer, "mic_bwd");
       mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
     }
     HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -266,8 +313,11 @@ struct Projector {
         launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
       }
       {
-        Scope sc_(*timer, "mic_apply");
+        Scope sc_(*timer, "mic_fwd");
         mic_sweep(kMicFwd, r, q.as<T>());
+      }
+      {
+        Scope sc_(*timer, "mic_bwd");
         mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
       }
       {
@@ -297,6 +347,7 @@ struct Projector {
                           flags.as<uint16_t>(), st);
     }
     PcgScalars r = pcg(max_iters, tol);
+    if (mic_variant >= 0) check_pipe();
     {
       Scope sc_(*timer, "gradient");
       double coef = dt / (rho * g.dx);
@@ -334,6 +385,12 @@ struct Solver : SolverBase {
   uint64_t drained = 0, emitted = 0, culled = 0;
   uint64_t rng = 0x5deece66dull | 1;
   double emit_accum = 0;
+  uint64_t redist_active = 0, redist_moved = 0;
+  // Active sets above this size take the host decision pass (device: one thread).
+  int64_t kHostDecideMin = 2048;
+  std::vector<uint32_t> h_count;
+  std::vector<int32_t> h_cell, h_dest;
+  std::vector<uint8_t> h_mask;
   int64_t n = 0, cap = 0;
   bool sorted_valid = false, have_saved = false;
   int cur = 0;
@@ -425,6 +482,7 @@ struct Solver : SolverBase {
     HIPCHK(hipMemsetAsync(cells.p, kAir, (size_t)nc, st));
     launch_cavity_mask<T>(grid(), sdf(), mask.as<uint8_t>(), st);
     launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
+    if (const char* e = getenv("SDR_REDIST_HOST_MIN")) kHostDecideMin = atoll(e);
     proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
     proj.init(nx, ny, nz, dx, st, &timer);
     reserve(1024);
@@ -559,6 +617,7 @@ struct Solver : SolverBase {
     launch_p2g<T>(2, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fw.as<T>(), exact(), st);
   }
   void save_grid() {  // solver.rs:587-591
+    Scope sc_(timer, "save_forces");
     HIPCHK(hipMemcpyAsync(su.p, fu.p, (size_t)nu() * sizeof(T), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sv.p, fv.p, (size_t)nv() * sizeof(T), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sw.p, fw.p, (size_t)nw() * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -637,8 +696,71 @@ struct Solver : SolverBase {
     n = kept;
     sorted_valid = false;
   }
+  // The order-dependent decision pass of redistribute (solver.rs:476-555) on the host:
+  // visits the active particles (cells over the cap at the sweep start) in reference
+  // order, mutating the counts as the reference loop does.  Sequential by definition,
+  // so it runs where a sequential pass is fastest; the device only ships the active
+  // particles' cells and the histogram, and receives the chosen destinations.
+  uint64_t decide_on_host(int64_t nact, uint32_t capc, H3 gdir) {
+    const int64_t nc = ncells();
+    h_count.resize((size_t)nc);
+    h_cell.resize((size_t)nact);
+    h_dest.resize((size_t)nact);
+    launch_gather_cells(act_slot_sorted.as<uint32_t>(), cellkey.as<int32_t>(), nact,
+                        act_rank.as<int32_t>(), st);
+    HIPCHK(hipMemcpyAsync(h_cell.data(), act_rank.p, (size_t)nact * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(h_count.data(), count.p, (size_t)nc * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double dn[6] = {-1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z,
+                          1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z,
+                          0.0 * gdir.x + -1.0 * gdir.y + 0.0 * gdir.z,
+                          0.0 * gdir.x + 1.0 * gdir.y + 0.0 * gdir.z,
+                          0.0 * gdir.x + 0.0 * gdir.y + -1.0 * gdir.z,
+                          0.0 * gdir.x + 0.0 * gdir.y + 1.0 * gdir.z};
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    uint32_t* cnt = h_count.data();
+    uint64_t moved = 0;
+    for (int64_t a = 0; a < nact; ++a) {
+      const int64_t c = h_cell[(size_t)a];
+      h_dest[(size_t)a] = -1;
+      if (cnt[c] <= capc) continue;
+      const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+      int64_t nb[6];
+      double dd[6];
+      int nn = 0;
+      if (i > 0) nb[nn] = c - 1, dd[nn++] = dn[0];
+      if (i + 1 < nx) nb[nn] = c + 1, dd[nn++] = dn[1];
+      if (j > 0) nb[nn] = c - sy, dd[nn++] = dn[2];
+      if (j + 1 < ny) nb[nn] = c + sy, dd[nn++] = dn[3];
+      if (k > 0) nb[nn] = c - sz, dd[nn++] = dn[4];
+      if (k + 1 < nz) nb[nn] = c + sz, dd[nn++] = dn[5];
+      bool have = false;
+      int64_t best = 0;
+      double best_down = -INFINITY;
+      uint32_t best_room = 0;
+      for (int q = 0; q < nn; ++q) {
+        const int64_t ncell = nb[q];
+        if (!h_mask[(size_t)ncell] || cnt[ncell] >= capc) continue;
+        const uint32_t room = capc - cnt[ncell];
+        if (!have || dd[q] > best_down || (dd[q] == best_down && room > best_room)) {
+          have = true, best = ncell, best_down = dd[q], best_room = room;
+        }
+      }
+      if (have) {
+        cnt[c] -= 1;
+        cnt[best] += 1;
+        h_dest[(size_t)a] = (int32_t)best;
+        ++moved;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(dest.p, h_dest.data(), (size_t)nact * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // h_dest is reused by the next sweep
+    return moved;
+  }
+
   // redistribute (solver.rs:454-557)
   void redistribute() {
+    redist_active = redist_moved = 0;
     if (n == 0) return;
     Scope sc_(timer, "redistribute");
     uint64_t target = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
@@ -646,6 +768,11 @@ struct Solver : SolverBase {
     H3 gdir = hnorm(h3(fluid.gravity));
     pin.ensure(64);
     uint32_t* hn = (uint32_t*)pin.p;
+    if (h_mask.empty()) {
+      h_mask.resize((size_t)ncells());
+      HIPCHK(hipMemcpyAsync(h_mask.data(), mask.p, h_mask.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
     for (int sweep = 0; sweep < 4; ++sweep) {
       HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
       launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
@@ -660,9 +787,18 @@ struct Solver : SolverBase {
                             act_rank.as<uint32_t>(), act_slot.as<uint32_t>(), st);
       HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(),
                             act_slot.as<uint32_t>(), act_slot_sorted.as<uint32_t>(), nact, 32, st));
-      launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(),
-                           count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x, gdir.y,
-                           gdir.z, dest.as<int32_t>(), st);
+      redist_active += (uint64_t)nact;
+      if (nact > kHostDecideMin) {
+        redist_moved += decide_on_host(nact, capc, gdir);
+      } else {
+        unsigned long long* mv = counters.as<unsigned long long>() + 3;
+        launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(),
+                             count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x,
+                             gdir.y, gdir.z, dest.as<int32_t>(), mv, st);
+        HIPCHK(hipMemcpyAsync((unsigned long long*)pin.p + 4, mv, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        redist_moved += ((unsigned long long*)pin.p)[4];
+      }
       launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact,
                              grid(), cfaces(), sdf(), st);
     }
@@ -951,7 +1087,8 @@ int sdr_solver_get_state(const sdr_solver* s, sdr_state* st) {
               st->steps = S.steps; st->last_pcg_iters = S.last_pcg_iters;
               st->last_substeps = S.last_substeps; st->drained = S.drained; st->rng = S.rng;
               st->emit_accum = S.emit_accum; st->emitted = S.emitted; st->culled = S.culled;
-              st->last_residual = S.last_residual);
+              st->last_residual = S.last_residual; st->redist_active = S.redist_active;
+              st->redist_moved = S.redist_moved);
 }
 int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
   return WITH(s, S.time = st->time; S.steps = st->steps; S.last_pcg_iters = st->last_pcg_iters;
@@ -959,6 +1096,7 @@ int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
               S.emit_accum = st->emit_accum; S.emitted = st->emitted; S.culled = st->culled;
               S.last_residual = st->last_residual);
 }
+int sdr_solver_synchronize(sdr_solver* s) { return WITH(s, HIPCHK(hipStreamSynchronize(S.st))); }
 int sdr_solver_enable_timing(sdr_solver* s, int32_t on) {
   return WITH(s, S.timer.collect(); S.timer.on = on != 0; if (on) S.timer.acc.clear());
 }

@@ -209,10 +209,11 @@ def make_solver(scene: Scene, lib=None, fill: bool = True, **kw):
     return s
 
 
-def fill_particles(solver, scene: Scene, mask: np.ndarray) -> np.ndarray:
+def fill_particles(solver, scene: Scene, mask: np.ndarray, stride: int = 1) -> np.ndarray:
     """Host-side initial fill (identical to the device `seed_fill` kernel): `fill_ppc`
-    particles per cavity cell whose centre is at or below `fill_y`, at centre +
-    splitmix64 jitter of (seed, index).  Returns (N,3) positions."""
+    particles per cavity cell whose centre y is in [fill_lo, fill_y], at centre +
+    splitmix64 jitter of (seed, index).  Returns (N,3) positions; `stride` > 1 returns
+    only every stride-th particle of that sequence."""
     nz, ny, nx = mask.shape
     kk, jj, ii = np.nonzero(mask)
     order = np.lexsort((ii, jj, kk))
@@ -222,7 +223,7 @@ def fill_particles(solver, scene: Scene, mask: np.ndarray) -> np.ndarray:
     sel = (cy >= scene.fill_lo) & (cy <= scene.fill_y)
     ii, jj, kk = ii[sel], jj[sel], kk[sel]
     ppc = scene.fill_ppc
-    t = np.arange(ii.size * ppc, dtype=np.uint64)
+    t = np.arange(0, ii.size * ppc, stride, dtype=np.uint64)
     key = np.uint64(scene.fill_seed) * np.uint64(1 << 32) + t
 
     def mix(k):
@@ -237,9 +238,10 @@ def fill_particles(solver, scene: Scene, mask: np.ndarray) -> np.ndarray:
         k3 = key * np.uint64(3)
     j = np.stack([mix(k3) / 0.6 * 0.9999, mix(k3 + np.uint64(1)) / 0.6 * 0.9999,
                   mix(k3 + np.uint64(2)) / 0.6 * 0.9999], axis=1)
-    ci = np.repeat(ii, ppc).astype(np.float64)
-    cj = np.repeat(jj, ppc).astype(np.float64)
-    ck = np.repeat(kk, ppc).astype(np.float64)
+    q = (t // np.uint64(ppc)).astype(np.int64)
+    ci = ii[q].astype(np.float64)
+    cj = jj[q].astype(np.float64)
+    ck = kk[q].astype(np.float64)
     cx = o[0] + (ci + 0.5) * dx
     cyy = o[1] + (cj + 0.5) * dx
     cz = o[2] + (ck + 0.5) * dx

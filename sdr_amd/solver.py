@@ -284,6 +284,9 @@ class Solver:
         return (o[0] + (i + 0.5) * d, o[1] + (j + 0.5) * d, o[2] + (k + 0.5) * d)
 
     # ---- device-only extras
+    def synchronize(self):
+        self.lib.solver_synchronize(self._h)
+
     def enable_timing(self, on: bool = True):
         self.lib.solver_enable_timing(self._h, 1 if on else 0)
 

@@ -1,0 +1,292 @@
+"""Device parity: the gfx950 path against the CPU oracle on identical seeded inputs.
+
+Bar (the parity contract of this build):
+  * bit-exact: cell classification, particle order and counts (sort / compaction /
+    drained / culled), and — in exact mode (reference-order P2G merge, 4096-chunk dot
+    tree) — every float of every stage of the f64 substep;
+  * fast mode (cell-ordered P2G sums, tree dot): fields within 1e-5 relative to the
+    field's max magnitude after one substep from the same state, integers exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sdr_amd
+from sdr_amd import _abi as A
+from sdr_amd import scenes
+from sdr_amd.solver import FLUID, Needle, SolveParams, project_capturing_with_source
+
+from helpers import (bits_equal, cell_of, classify_box, counts, fill_divergent,
+                     interior_divergence, pack_one_cell, paired, radial_field, rel_close)
+
+pytestmark = pytest.mark.gpu
+EXACT = {"dot_mode": A.DOT_EXACT}
+FAST = {"dot_mode": A.DOT_FAST}
+FIELD_TOL = 1e-5
+
+
+def test_device_is_gfx950(dev):
+    n, arch = sdr_amd.device_info()
+    assert n >= 1 and arch.startswith("gfx950")
+
+
+def _compare_all(o, d, exact=True):
+    po, vo = o.particles()
+    pd, vd = d.particles()
+    assert po.shape == pd.shape
+    assert np.array_equal(o.cells(), d.cells())
+    so, sd = o.state(), d.state()
+    assert (so.drained, so.culled, so.emitted) == (sd.drained, sd.culled, sd.emitted)
+    if exact:
+        assert bits_equal(po, pd) and bits_equal(vo, vd)
+        for a, b in zip(o.grid(), d.grid()):
+            assert bits_equal(a, b)
+        assert bits_equal(o.pressure(), d.pressure())
+        assert so.last_pcg_iters == sd.last_pcg_iters
+
+
+@pytest.mark.parametrize("level", ["c1", "c2"])
+def test_substep_stages_bit_exact(dev, orc, level):
+    """Every stage of two substeps, stage by stage, bit for bit (exact mode)."""
+    sc, o, d = paired(level, orc, EXACT, stride=1 if level == "c1" else 2)
+    for sub in range(2):
+        for st in A.STAGES:
+            o.run_stage(st, sc.dt)
+            d.run_stage(st, sc.dt)
+            po, vo = o.particles()
+            pd, vd = d.particles()
+            assert bits_equal(po, pd), (sub, st, "positions")
+            assert bits_equal(vo, vd), (sub, st, "velocities")
+            if st in ("classify", "p2g", "forces", "project", "extrapolate"):
+                assert np.array_equal(o.cells(), d.cells()), (sub, st)
+                for a, b in zip(o.grid(), d.grid()):
+                    assert bits_equal(a, b), (sub, st, "grid")
+            if st == "project":
+                assert bits_equal(o.pressure(), d.pressure()), (sub, st)
+                assert o.last_pcg_iters == d.last_pcg_iters
+        so, sd = o.state(), d.state()
+        assert (so.drained, so.culled, so.emitted) == (sd.drained, sd.culled, sd.emitted)
+
+
+def test_multi_substep_run_bit_exact(dev, orc):
+    sc, o, d = paired("c1", orc, EXACT)
+    for _ in range(10):
+        o.substep(sc.dt)
+        d.substep(sc.dt)
+    _compare_all(o, d, exact=True)
+
+
+def test_cfl_step_bit_exact(dev, orc):
+    """Solver::step's CFL sub-stepping (solver.rs:237-281) picks the same substeps."""
+    sc, o, d = paired("c1", orc, EXACT, stride=2)
+    o.step(0.0004)
+    d.step(0.0004)
+    assert o.last_substeps == d.last_substeps
+    assert o.time == d.time
+    _compare_all(o, d, exact=True)
+
+
+def test_fast_mode_within_tolerance(dev, orc):
+    """Fast mode from the same state: integers exact, floats within 1e-5 relative."""
+    sc, o, d = paired("c1", orc, FAST)
+    o.substep(sc.dt)
+    d.substep(sc.dt)
+    _compare_all(o, d, exact=False)
+    po, vo = o.particles()
+    pd, vd = d.particles()
+    assert rel_close(pd, po, FIELD_TOL)[0]
+    assert rel_close(vd, vo, FIELD_TOL)[0], rel_close(vd, vo, FIELD_TOL)
+    assert rel_close(d.pressure(), o.pressure(), FIELD_TOL)[0]
+    for a, b in zip(d.grid(), o.grid()):
+        ok, err = rel_close(a, b, FIELD_TOL)
+        assert ok, err
+
+
+def test_deterministic_run_to_run(dev):
+    """No float atomics anywhere: two device runs agree bit for bit, fast mode included."""
+    out = []
+    for _ in range(2):
+        sc = scenes.cavity_scene("c1")
+        d = scenes.make_solver(sc, **FAST)
+        d.seed_fill(sc.fill_ppc, sc.fill_lo, sc.fill_y, sc.fill_seed)
+        for _ in range(3):
+            d.substep(sc.dt)
+        out.append((d.particles(), d.pressure()))
+    (pa, va), qa = out[0]
+    (pb, vb), qb = out[1]
+    assert bits_equal(pa, pb) and bits_equal(va, vb) and bits_equal(qa, qb)
+
+
+def test_device_seed_fill_matches_host_fill(dev, orc):
+    sc = scenes.cavity_scene("c1")
+    o = scenes.make_solver(sc, lib=orc)
+    want = scenes.fill_particles(o, sc, oracle.cavity_mask(o))
+    d = scenes.make_solver(sc, **EXACT)
+    n = d.seed_fill(sc.fill_ppc, sc.fill_lo, sc.fill_y, sc.fill_seed)
+    assert n == want.shape[0]
+    assert bits_equal(d.particles()[0], want)
+
+
+# ---------------------------------------------------------------- projection surface
+def _proj(lib, n, fill, tol, iters=400, dot=A.DOT_EXACT):
+    u, v, w = fill(n, n, n)
+    cells = classify_box(n, n, n)
+    rep, p = project_capturing_with_source(n, n, n, 0.01, (0.0, 0.0, 0.0), u, v, w, cells, 1000.0,
+                                           1e-3, SolveParams(iters, tol), lib=lib, dot_mode=dot)
+    return rep, p, (u, v, w)
+
+
+def test_projection_bit_exact_vs_oracle(dev, orc):
+    ro, po, fo = _proj(orc, 24, fill_divergent, 1e-12)
+    rd, pd, fd = _proj(dev, 24, fill_divergent, 1e-12)
+    assert ro.iters == rd.iters and ro.residual == rd.residual
+    assert bits_equal(po, pd)
+    for a, b in zip(fo, fd):
+        assert bits_equal(a, b)
+
+
+def test_projection_reference_known_answers_on_device(dev):
+    # pressure.rs:619-642 idempotence
+    _, _, (u, v, w) = _proj(dev, 12, fill_divergent, 1e-12)
+    u1, v1, w1 = u.copy(), v.copy(), w.copy()
+    project_capturing_with_source(12, 12, 12, 0.01, (0, 0, 0), u, v, w, classify_box(12, 12, 12),
+                                  1000.0, 1e-3, SolveParams(400, 1e-12), lib=dev)
+    assert max(np.abs(u - u1).max(), np.abs(v - v1).max(), np.abs(w - w1).max()) < 1e-7
+    # pressure.rs:675-723 divergence removal, both dot modes
+    for dot in (A.DOT_EXACT, A.DOT_FAST):
+        _, _, (u, v, w) = _proj(dev, 12, radial_field, 1e-9, dot=dot)
+        assert interior_divergence(u, v, w) < 1e-5
+    # pressure.rs:731-752 bit reproducibility
+    _, pa, _ = _proj(dev, 24, fill_divergent, 1e-12, dot=A.DOT_FAST)
+    _, pb, _ = _proj(dev, 24, fill_divergent, 1e-12, dot=A.DOT_FAST)
+    assert bits_equal(pa, pb)
+
+
+def test_projection_edge_cases(dev, orc):
+    n = 8
+    cells = classify_box(n, n, n)
+    zero = lambda: (np.zeros((n, n, n + 1)), np.zeros((n, n + 1, n)), np.zeros((n + 1, n, n)))
+    u, v, w = zero()
+    rep, p = project_capturing_with_source(n, n, n, 0.01, (0, 0, 0), u, v, w, cells, 1000.0, 1e-3,
+                                           lib=dev)
+    assert rep.iters == 0 and not p.any()  # zero RHS: pressure.rs:345-352
+    # max_iters exhaustion (tol 0) and the all-air breakdown guard match the oracle
+    for cl in (cells, np.full((n, n, n), 2, np.uint8)):
+        res = []
+        for lib in (orc, dev):
+            u, v, w = fill_divergent(n, n, n)
+            r, p = project_capturing_with_source(n, n, n, 0.01, (0, 0, 0), u, v, w, cl, 1000.0,
+                                                 1e-3, SolveParams(5, 0.0), lib=lib)
+            res.append((r.iters, r.residual, p, u))
+        assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+        assert bits_equal(res[0][2], res[1][2]) and bits_equal(res[0][3], res[1][3])
+
+
+# ---------------------------------------------------------------- reference solver tests on device
+def test_redistribution_known_answers_on_device(dev, orc):
+    sc = scenes.box_scene()
+    for n, here, below in ((12, 8, 4), (100, None, None)):
+        d = scenes.make_solver(sc)
+        o = scenes.make_solver(sc, lib=orc)
+        pack_one_cell(d, (8, 8, 8), n, oracle.hash_jitter)
+        pack_one_cell(o, (8, 8, 8), n, oracle.hash_jitter)
+        d.redistribute()
+        o.redistribute()
+        pd, vd = d.particles()
+        assert pd.shape[0] == n
+        cnt = counts(d, pd)
+        if here is not None:  # solver.rs:1212-1243
+            assert cnt.get((8, 8, 8), 0) == here and cnt.get((8, 7, 8), 0) == below
+            assert cnt.get((8, 9, 8), 0) == 0
+        else:  # solver.rs:1251-1335
+            assert max(cnt.values()) < n
+            assert sum(1 for c in cnt.values() if c > 8) <= 1
+        po, vo = o.particles()
+        assert bits_equal(pd, po) and bits_equal(vd, vo)
+
+
+def test_injection_and_walls_on_device(dev, orc):
+    # solver.rs:1075-1099
+    sc = scenes.box_scene()
+    d = scenes.make_solver(sc)
+    nd = sc.needle
+    d.needle = Needle(nd.tip, nd.axis, nd.radius, nd.flow_rate, inject_until=0.02)
+    for _ in range(10):
+        d.step(0.005)
+    after = d.particle_count()
+    assert after > 0 and d.time > 0.02
+    for _ in range(10):
+        d.step(0.005)
+    assert d.particle_count() == after
+    # solver.rs:1102-1118 on the parametric sinus, and bit parity with the oracle
+    sc = scenes.sinus_scene()
+    d = scenes.make_solver(sc, **EXACT)
+    o = scenes.make_solver(sc, lib=orc)
+    for _ in range(12):
+        d.step(0.005)
+        o.step(0.005)
+    pd, _ = d.particles()
+    assert pd.shape[0] > 0
+    assert max(oracle.sdf_sample(sc.sdf, p)[0] for p in pd) < sc.sdf.dx
+    _compare_all(o, d, exact=True)
+
+
+def test_mass_invariant_on_device(dev):
+    sc = scenes.cavity_scene("c1")
+    d = scenes.make_solver(sc, **FAST)
+    n0 = d.seed_fill(sc.fill_ppc, sc.fill_lo, sc.fill_y, sc.fill_seed)
+    for _ in range(5):
+        d.substep(sc.dt)
+    st = d.state()
+    assert st.emitted == d.particle_count() + st.drained + st.culled
+    assert st.emitted >= n0
+
+
+@pytest.mark.parametrize("variant", ["-1", "0", "1", "2", "3", "4", "5"])
+def test_mic_sweep_variants_bit_exact(dev, orc, variant, monkeypatch):
+    """Every tile shape of the pipelined MIC(0) sweeps (and the plane-by-plane kernels,
+    variant -1) reproduces the oracle's projection bit for bit on a grid spanning
+    several i-tiles and k-tiles, with solid, air and fluid cells mixed."""
+    monkeypatch.setenv("SDR_MIC_VARIANT", variant)
+    nx, ny, nz = 150, 37, 41
+    g = np.random.default_rng(7)
+    r = g.random((nz, ny, nx))
+    cells = np.where(r < 0.8, 1, np.where(r < 0.9, 0, 2)).astype(np.uint8)
+    u0 = g.standard_normal((nz, ny, nx + 1))
+    v0 = g.standard_normal((nz, ny + 1, nx))
+    w0 = g.standard_normal((nz + 1, ny, nx))
+    out = []
+    for lib in (orc, dev):
+        u, v, w = u0.copy(), v0.copy(), w0.copy()
+        rep, p = project_capturing_with_source(nx, ny, nz, 0.01, (0, 0, 0), u, v, w, cells, 1000.0,
+                                               1e-3, SolveParams(25, 1e-9), lib=lib)
+        out.append((rep, p, u, v, w))
+    (ro, po, *fo), (rd, pd, *fd) = out
+    assert ro.iters == rd.iters and ro.residual == rd.residual
+    assert bits_equal(po, pd)
+    for a, b in zip(fo, fd):
+        assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("host_min", ["0", "100000000"])
+def test_redistribute_decision_paths_bit_exact(dev, orc, host_min, monkeypatch):
+    """The order-dependent redistribution pass gives the oracle's bits whether it runs
+    on the host (large active sets) or on one device thread (small ones)."""
+    monkeypatch.setenv("SDR_REDIST_HOST_MIN", host_min)
+    for n in (12, 100):
+        sc = scenes.box_scene()
+        d = scenes.make_solver(sc)
+        o = scenes.make_solver(sc, lib=orc)
+        pack_one_cell(d, (8, 8, 8), n, oracle.hash_jitter)
+        pack_one_cell(o, (8, 8, 8), n, oracle.hash_jitter)
+        d.redistribute()
+        o.redistribute()
+        assert bits_equal(d.particles()[0], o.particles()[0])
+        assert d.state().redist_moved == o.state().redist_moved
+        assert d.state().redist_active == o.state().redist_active
+    sc, o, d = paired("c1", orc, EXACT, stride=1)
+    for _ in range(4):
+        o.substep(sc.dt)
+        d.substep(sc.dt)
+    _compare_all(o, d, exact=True)
+    assert d.state().redist_moved == o.state().redist_moved
