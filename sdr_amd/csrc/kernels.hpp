@@ -128,6 +128,21 @@ void launch_redist_decide(const uint32_t* act_slot_sorted, int64_t nact, const i
                           uint32_t* count, const uint8_t* mask, int nx, int ny, int nz, uint32_t cap,
                           double gx, double gy, double gz, int32_t* dest, unsigned long long* moved,
                           hipStream_t st);
+// dir . gdir of the -x,+x,-y,+y,-z,+z neighbours (solver.rs:492-509)
+struct RedistDirs {
+  double d[6];
+};
+void launch_rd_sources(const uint32_t* scell, int64_t nact, const uint32_t* count, uint32_t cap,
+                       uint32_t* src_cell, uint32_t* src_head, uint32_t* src_lim, uint32_t* nsrc,
+                       hipStream_t st);
+// One reservation round over up to max_src sources (the device count is *nsrc); after
+// it, *pending holds the number of sources with candidates left.
+void launch_rd_round(int64_t max_src, const uint32_t* nsrc, const uint32_t* src_cell, uint32_t* src_head,
+                     const uint32_t* src_lim, uint8_t* src_rmask, const uint32_t* spos, uint32_t* count,
+                     const uint8_t* mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
+                     unsigned long long* resv, uint32_t epoch, int32_t* dest, uint32_t* pending,
+                     unsigned long long* moved, hipStream_t st);
+void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t st);
 void launch_gather_cells(const uint32_t* act, const int32_t* cellkey, int64_t nact, int32_t* act_cell,
                          hipStream_t st);
 template <class T>

@@ -22,6 +22,7 @@ struct MicPipeArgs {
   long spin_limit;
   int ntile_i, ntile_k;
   uint32_t tag;
+  uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
 };
 
 struct MicPipeShape {

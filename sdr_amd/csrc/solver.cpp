@@ -211,6 +211,9 @@ struct Projector {
     // f64 runs 512-thread tiles (1x8 planes), f32 1024-thread tiles (1x16 planes)
     mic_variant = sizeof(T) == 8 ? 1 : 0;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
+    // The pipelined sweeps address with 32-bit byte offsets.
+    const size_t lim = (size_t)1 << 31;
+    if (n * sizeof(T) >= lim) mic_variant = -1;
     size_t hx = 0, hz = 0;
     for (int v = 0; v < 6; ++v) {
       MicPipeShape ps = mic_pipe_shape(sh, v);
@@ -218,6 +221,8 @@ struct Projector {
       mic_pipe_halo_sizes(sh, ps, &a, &c);
       hx = std::max(hx, a), hz = std::max(hz, c);
     }
+    if (hx * sizeof(uint64_t) >= lim || hz * sizeof(uint64_t) >= lim) mic_variant = -1;
+    if (mic_variant < 0) hx = hz = 1;
     halo_x.alloc(hx * sizeof(uint64_t));
     halo_z.alloc(hz * sizeof(uint64_t));
     pipe_err.alloc(sizeof(int));
@@ -243,7 +248,8 @@ struct Projector {
       sweep_tag = 1;
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
-                      1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag};
+                      1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
+                      (uint32_t)halo_z.bytes};
     launch_mic_pipe<T>(op, mic_variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
@@ -386,11 +392,10 @@ struct Solver : SolverBase {
   uint64_t rng = 0x5deece66dull | 1;
   double emit_accum = 0;
   uint64_t redist_active = 0, redist_moved = 0;
-  // Active sets above this size take the host decision pass (device: one thread).
-  int64_t kHostDecideMin = 2048;
-  std::vector<uint32_t> h_count;
-  std::vector<int32_t> h_cell, h_dest;
-  std::vector<uint8_t> h_mask;
+  // Decision pass: reservation rounds (default) or the single-thread reference loop.
+  bool redist_sequential = false;
+  uint64_t redist_rounds = 0;
+  uint32_t rd_epoch = 0;
   int64_t n = 0, cap = 0;
   bool sorted_valid = false, have_saved = false;
   int cur = 0;
@@ -404,7 +409,7 @@ struct Solver : SolverBase {
   Buf temp;
   Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
   Buf cellkey, act_rank, act_slot, act_rank_sorted, act_slot_sorted, dest;
-  Buf counters, aos_pos, aos_vel, fill_cells;
+  Buf counters, aos_pos, aos_vel, fill_cells, rd_resv, rd_ctr;
   PinnedBuf pin;
   Projector<T> proj;
 
@@ -482,7 +487,7 @@ struct Solver : SolverBase {
     HIPCHK(hipMemsetAsync(cells.p, kAir, (size_t)nc, st));
     launch_cavity_mask<T>(grid(), sdf(), mask.as<uint8_t>(), st);
     launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
-    if (const char* e = getenv("SDR_REDIST_HOST_MIN")) kHostDecideMin = atoll(e);
+    if (const char* e = getenv("SDR_REDIST_SEQUENTIAL")) redist_sequential = atoi(e) != 0;
     proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
     proj.init(nx, ny, nz, dx, st, &timer);
     reserve(1024);
@@ -696,66 +701,65 @@ struct Solver : SolverBase {
     n = kept;
     sorted_valid = false;
   }
-  // The order-dependent decision pass of redistribute (solver.rs:476-555) on the host:
-  // visits the active particles (cells over the cap at the sweep start) in reference
-  // order, mutating the counts as the reference loop does.  Sequential by definition,
-  // so it runs where a sequential pass is fastest; the device only ships the active
-  // particles' cells and the histogram, and receives the chosen destinations.
-  uint64_t decide_on_host(int64_t nact, uint32_t capc, H3 gdir) {
+  // The order-dependent decision pass of redistribute (solver.rs:476-555) by
+  // deterministic reservation rounds (kernels.hip, k_rd_*): the active particles
+  // (reference order a = 0..nact-1) are regrouped by cell keeping that order, every
+  // source cell gets its candidate range, and rounds run until no source has candidates
+  // left.  Chosen destinations land in dest[a]; returns the number of moves.
+  uint64_t decide(int64_t nact, uint32_t capc, const RedistDirs& dn) {
     const int64_t nc = ncells();
-    h_count.resize((size_t)nc);
-    h_cell.resize((size_t)nact);
-    h_dest.resize((size_t)nact);
+    if (rd_resv.bytes < (size_t)nc * 8) {
+      rd_resv.alloc((size_t)nc * 8);
+      HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
+      rd_epoch = 0;
+    }
+    if (!rd_ctr.p) rd_ctr.alloc(64);
+    uint32_t* ctr = rd_ctr.as<uint32_t>();  // [0] sources, [1] pending
+    unsigned long long* moved = (unsigned long long*)(ctr + 4);
+    // cells of the active particles in reference order, then grouped by cell (stable)
     launch_gather_cells(act_slot_sorted.as<uint32_t>(), cellkey.as<int32_t>(), nact,
                         act_rank.as<int32_t>(), st);
-    HIPCHK(hipMemcpyAsync(h_cell.data(), act_rank.p, (size_t)nact * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(h_count.data(), count.p, (size_t)nc * 4, hipMemcpyDeviceToHost, st));
+    launch_iota_u32(act_slot.as<uint32_t>(), nact, st);
+    HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), keys.as<uint32_t>(),
+                          act_slot.as<uint32_t>(), vals.as<uint32_t>(), nact, bits_for((uint64_t)nc), st));
+    HIPCHK(hipMemsetAsync(rd_ctr.p, 0, rd_ctr.bytes, st));
+    HIPCHK(hipMemsetAsync(dest.p, 0xff, (size_t)nact * 4, st));
+    uint32_t* src_cell = keys_alt.as<uint32_t>();
+    uint32_t* src_head = vals_alt.as<uint32_t>();
+    uint32_t* src_lim = cullflag.as<uint32_t>();
+    uint8_t* src_rmask = culled_ranks.as<uint8_t>();
+    launch_rd_sources(keys.as<uint32_t>(), nact, count.as<uint32_t>(), capc, src_cell, src_head, src_lim,
+                      ctr, st);
+    uint32_t* hc = (uint32_t*)pin.p;
+    HIPCHK(hipMemcpyAsync(hc, ctr, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const double dn[6] = {-1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z,
-                          1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z,
-                          0.0 * gdir.x + -1.0 * gdir.y + 0.0 * gdir.z,
-                          0.0 * gdir.x + 1.0 * gdir.y + 0.0 * gdir.z,
-                          0.0 * gdir.x + 0.0 * gdir.y + -1.0 * gdir.z,
-                          0.0 * gdir.x + 0.0 * gdir.y + 1.0 * gdir.z};
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    uint32_t* cnt = h_count.data();
-    uint64_t moved = 0;
-    for (int64_t a = 0; a < nact; ++a) {
-      const int64_t c = h_cell[(size_t)a];
-      h_dest[(size_t)a] = -1;
-      if (cnt[c] <= capc) continue;
-      const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
-      int64_t nb[6];
-      double dd[6];
-      int nn = 0;
-      if (i > 0) nb[nn] = c - 1, dd[nn++] = dn[0];
-      if (i + 1 < nx) nb[nn] = c + 1, dd[nn++] = dn[1];
-      if (j > 0) nb[nn] = c - sy, dd[nn++] = dn[2];
-      if (j + 1 < ny) nb[nn] = c + sy, dd[nn++] = dn[3];
-      if (k > 0) nb[nn] = c - sz, dd[nn++] = dn[4];
-      if (k + 1 < nz) nb[nn] = c + sz, dd[nn++] = dn[5];
-      bool have = false;
-      int64_t best = 0;
-      double best_down = -INFINITY;
-      uint32_t best_room = 0;
-      for (int q = 0; q < nn; ++q) {
-        const int64_t ncell = nb[q];
-        if (!h_mask[(size_t)ncell] || cnt[ncell] >= capc) continue;
-        const uint32_t room = capc - cnt[ncell];
-        if (!have || dd[q] > best_down || (dd[q] == best_down && room > best_room)) {
-          have = true, best = ncell, best_down = dd[q], best_room = room;
-        }
+    const int64_t nsrc = hc[0];
+    // Each round commits at least the lowest pending candidate, so the candidates (at
+    // most nact) bound the rounds; the pending count is read every few rounds.
+    const int check_every = 4;
+    int64_t rounds = 0;
+    while (nsrc > 0) {
+      if (++rd_epoch == 0) {  // epochs wrapped: no stale reservation may outrank a new one
+        HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
+        rd_epoch = 1;
       }
-      if (have) {
-        cnt[c] -= 1;
-        cnt[best] += 1;
-        h_dest[(size_t)a] = (int32_t)best;
-        ++moved;
+      launch_rd_round(nsrc, ctr, src_cell, src_head, src_lim, src_rmask, vals.as<uint32_t>(),
+                      count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, dn,
+                      rd_resv.as<unsigned long long>(), rd_epoch, dest.as<int32_t>(), ctr + 1, moved, st);
+      ++rounds;
+      if (rounds % check_every == 0) {
+        HIPCHK(hipMemcpyAsync(hc, ctr + 1, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hc[0] == 0) break;
+        if (rounds > nact + check_every) throw Error(SDR_E_STATE, "redistribute: rounds did not converge");
       }
     }
-    HIPCHK(hipMemcpyAsync(dest.p, h_dest.data(), (size_t)nact * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));  // h_dest is reused by the next sweep
-    return moved;
+    redist_rounds += (uint64_t)rounds;
+    HIPCHK(hipMemcpyAsync(hc + 2, moved, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint64_t m;
+    memcpy(&m, hc + 2, 8);
+    return m;
   }
 
   // redistribute (solver.rs:454-557)
@@ -766,13 +770,12 @@ struct Solver : SolverBase {
     uint64_t target = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
     uint32_t capc = (uint32_t)std::ceil(std::fmax(fluid.redist_cap_factor, 1.0) * (double)target);
     H3 gdir = hnorm(h3(fluid.gravity));
+    // dir . gdir of the unit directions -x,+x,-y,+y,-z,+z (math.rs dot, left to right)
+    RedistDirs dn{{-1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z, 1.0 * gdir.x + 0.0 * gdir.y + 0.0 * gdir.z,
+                   0.0 * gdir.x + -1.0 * gdir.y + 0.0 * gdir.z, 0.0 * gdir.x + 1.0 * gdir.y + 0.0 * gdir.z,
+                   0.0 * gdir.x + 0.0 * gdir.y + -1.0 * gdir.z, 0.0 * gdir.x + 0.0 * gdir.y + 1.0 * gdir.z}};
     pin.ensure(64);
     uint32_t* hn = (uint32_t*)pin.p;
-    if (h_mask.empty()) {
-      h_mask.resize((size_t)ncells());
-      HIPCHK(hipMemcpyAsync(h_mask.data(), mask.p, h_mask.size(), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-    }
     for (int sweep = 0; sweep < 4; ++sweep) {
       HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
       launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
@@ -788,9 +791,7 @@ struct Solver : SolverBase {
       HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(),
                             act_slot.as<uint32_t>(), act_slot_sorted.as<uint32_t>(), nact, 32, st));
       redist_active += (uint64_t)nact;
-      if (nact > kHostDecideMin) {
-        redist_moved += decide_on_host(nact, capc, gdir);
-      } else {
+      if (redist_sequential) {
         unsigned long long* mv = counters.as<unsigned long long>() + 3;
         launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(),
                              count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x,
@@ -798,6 +799,8 @@ struct Solver : SolverBase {
         HIPCHK(hipMemcpyAsync((unsigned long long*)pin.p + 4, mv, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         redist_moved += ((unsigned long long*)pin.p)[4];
+      } else {
+        redist_moved += decide(nact, capc, dn);
       }
       launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact,
                              grid(), cfaces(), sdf(), st);

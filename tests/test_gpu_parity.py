@@ -268,11 +268,26 @@ def test_mic_sweep_variants_bit_exact(dev, orc, variant, monkeypatch):
         assert bits_equal(a, b)
 
 
-@pytest.mark.parametrize("host_min", ["0", "100000000"])
-def test_redistribute_decision_paths_bit_exact(dev, orc, host_min, monkeypatch):
-    """The order-dependent redistribution pass gives the oracle's bits whether it runs
-    on the host (large active sets) or on one device thread (small ones)."""
-    monkeypatch.setenv("SDR_REDIST_HOST_MIN", host_min)
+def _random_clumps(solver, seed, lo=0, hi=16):
+    """Particles with a random count in [lo, hi] per cavity cell, at random offsets inside
+    the cell and in shuffled reference order, so sources and receivers interleave and
+    many sources share receivers (the order-dependent case of redistribute)."""
+    g = np.random.default_rng(seed)
+    mask = (solver.cells().reshape(solver.nz, solver.ny, solver.nx) != 0)
+    kk, jj, ii = np.nonzero(mask)
+    cnt = g.integers(lo, hi + 1, size=kk.size)
+    ci = np.repeat(np.stack([ii, jj, kk], 1), cnt, axis=0).astype(np.float64)
+    pos = np.asarray(solver.origin) + (ci + 0.5 + g.uniform(-0.45, 0.45, ci.shape)) * solver.dx
+    pos = pos[g.permutation(pos.shape[0])]
+    return pos
+
+
+@pytest.mark.parametrize("sequential", ["0", "1"])
+def test_redistribute_decision_paths_bit_exact(dev, orc, sequential, monkeypatch):
+    """The order-dependent redistribution pass gives the oracle's bits both by the
+    parallel reservation rounds and by the single-thread reference loop: one packed cell,
+    random clumps on the box and on the 36x42x34 cavity, and four full substeps."""
+    monkeypatch.setenv("SDR_REDIST_SEQUENTIAL", sequential)
     for n in (12, 100):
         sc = scenes.box_scene()
         d = scenes.make_solver(sc)
@@ -284,6 +299,22 @@ def test_redistribute_decision_paths_bit_exact(dev, orc, host_min, monkeypatch):
         assert bits_equal(d.particles()[0], o.particles()[0])
         assert d.state().redist_moved == o.state().redist_moved
         assert d.state().redist_active == o.state().redist_active
+    for name, seed in (("box", 1), ("c1", 2), ("c1", 3)):
+        sc = scenes.box_scene() if name == "box" else scenes.cavity_scene(name)
+        d = scenes.make_solver(sc)
+        o = scenes.make_solver(sc, lib=orc)
+        d.run_stage("classify")
+        pos = _random_clumps(d, seed)
+        vel = np.random.default_rng(seed).normal(size=pos.shape)
+        d.set_particles(pos, vel)
+        o.set_particles(pos, vel)
+        d.redistribute()
+        o.redistribute()
+        assert d.state().redist_moved == o.state().redist_moved > 0
+        assert d.state().redist_active == o.state().redist_active
+        pd, vd = d.particles()
+        po, vo = o.particles()
+        assert bits_equal(pd, po) and bits_equal(vd, vo)
     sc, o, d = paired("c1", orc, EXACT, stride=1)
     for _ in range(4):
         o.substep(sc.dt)
