@@ -108,14 +108,127 @@ def cpu_baseline(scene, sdf, gpu_iters: float, budget_s: float, threads: int):
     }
 
 
+def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, label):
+    """Seed the workload, run warmup + timed substeps; returns (solver, elapsed, iters,
+    particle-substeps, timing regions)."""
+    from sdr_amd import _abi as A
+    from sdr_amd import scenes
+    from sdr_amd.solver import SolveParams
+
+    t0 = time.time()
+    s = scenes.make_solver(scene, precision=prec, dot_mode=dot_mode, solve_mode=solve_mode,
+                           refine_tol=tol)
+    if solve_mode == A.SOLVE_RELATIVE:
+        s.solve = SolveParams(max_iters, scene.solve.tol)
+    n0 = s.seed_fill(scene.fill_ppc, scene.fill_lo, scene.fill_y, scene.fill_seed)
+    print(f"[bench] {label}: grid {s.nx}x{s.ny}x{s.nz} particles {n0}; setup "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dt = scene.dt
+    for k in range(warmup):
+        t = time.perf_counter()
+        s.substep(dt)
+        s.synchronize()
+        print(f"[bench] {label} warmup {k}: {time.perf_counter() - t:.3f}s iters {s.last_pcg_iters}",
+              file=sys.stderr, flush=True)
+    s.enable_timing(True)
+    s.synchronize()
+    particle_steps = 0
+    iters = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        particle_steps += s.particle_count()
+        t = time.perf_counter()
+        s.substep(dt)
+        iters.append(s.last_pcg_iters)
+        stt = s.state()
+        print(f"[bench] {label} step {k}: {time.perf_counter() - t:.3f}s pcg iters {iters[-1]} "
+              f"redistribution: active {stt.redist_active} moved {stt.redist_moved} rounds "
+              f"{stt.redist_rounds}", file=sys.stderr, flush=True)
+    s.synchronize()
+    elapsed = time.perf_counter() - t_start
+    timing = s.timing()
+    s.enable_timing(False)
+    return s, elapsed, iters, particle_steps, timing
+
+
+SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_bwd", "spmv_dot", "axpy_norm",
+                 "dot_update", "gradient", "refine"]
+
+
+def solve_error(s, scene, rel_ref):
+    """Pressure and divergence error of the solver's projection of one substep against an
+    f64 solve of the same inputs (the f64 device path, which reproduces the CPU oracle
+    bit for bit in reference mode) run to a relative residual of `rel_ref`."""
+    from sdr_amd import _abi as A
+    from sdr_amd.solver import SolveParams, project_capturing_with_source
+
+    dt = scene.dt
+    for st in ("emit", "classify", "p2g", "save", "forces"):
+        s.run_stage(st, dt)
+    u, v, w = s.grid()
+    cells = s.cells()
+    # The solver's projection applies the volume-control source of its particle counts
+    # (solver.rs:392-423); the reference solve gets the same source explicitly.
+    pos, _ = s.particles()
+    g = (pos - np.asarray(s.origin)) / s.dx
+    ok = np.all(g >= 0, axis=1)
+    ijk = g.astype(np.int64)
+    ok &= (ijk[:, 0] < s.nx) & (ijk[:, 1] < s.ny) & (ijk[:, 2] < s.nz)
+    cnt = np.bincount(((ijk[ok, 2] * s.ny + ijk[ok, 1]) * s.nx + ijk[ok, 0]),
+                      minlength=s.nx * s.ny * s.nz).reshape(s.nz, s.ny, s.nx)
+    del pos, g, ijk, ok
+    fp = scene.fluid
+    rest = float(max(fp.particles_per_cell, 1))
+    unit = s.dx / dt
+    over = cnt / rest - 1.0
+    src = np.where((cells == 1) & (over > 0.0),
+                   np.minimum(fp.volume_stiffness * over * unit, 0.25 * unit), 0.0)
+    if fp.volume_stiffness <= 0.0:
+        src = None
+    t = time.perf_counter()
+    uo, vo, wo = u.copy(), v.copy(), w.copy()
+    rep, p_ref = project_capturing_with_source(
+        s.nx, s.ny, s.nz, s.dx, s.origin, uo, vo, wo, cells, scene.fluid.density, dt,
+        SolveParams(20000, 0.0), src, dot_mode=A.DOT_FAST, solve_mode=A.SOLVE_RELATIVE,
+        refine_tol=rel_ref)
+    t_ref = time.perf_counter() - t
+    s.run_stage("project", dt)
+    p = s.pressure()
+    ud, vd, wd = s.grid()
+    fl = cells == 1
+
+    def div(a, b, c):
+        d = (a[:, :, 1:] - a[:, :, :-1]) + (b[:, 1:, :] - b[:, :-1, :]) + (c[1:] - c[:-1])
+        return np.abs(d[fl]).max()
+
+    d_in = div(u, v, w)
+    return {
+        "reference": f"f64 device solve of the same substep to relative residual {rel_ref:g} "
+                     f"({rep.iters} iterations, {t_ref:.1f} s)",
+        "pressure_rel_err": float(np.abs(p - p_ref).max() / max(np.abs(p_ref).max(), 1e-300)),
+        "divergence_in": float(d_in),
+        "divergence_out": float(div(ud, vd, wd)),
+        "divergence_out_reference": float(div(uo, vo, wo)),
+        "face_rel_err": float(max(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+                                  for a, b in ((ud, uo), (vd, vo), (wd, wo)))),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c3a"])
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--dot", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--precision", default="f32", choices=["f64", "f32"])
+    ap.add_argument("--dot", default="fast", choices=["exact", "fast"])
+    ap.add_argument("--solve", default="relative", choices=["relative", "reference"],
+                    help="relative: stop at --tol * max|b| (f32: by f64 refinement); "
+                         "reference: the reference's absolute tol / 200 iterations")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iters", type=int, default=2000)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain f64 comparison run")
+    ap.add_argument("--no-error", action="store_true", help="skip the error-vs-f64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -133,46 +246,20 @@ def main() -> int:
     if ndev < 1:
         print("no HIP device", file=sys.stderr)
         return 3
-    t0 = time.time()
     scene = scenes.cavity_scene(args.config)
     prec = A.PRECISION_F32 if args.precision == "f32" else A.PRECISION_F64
     dot_mode = A.DOT_FAST if args.dot == "fast" else A.DOT_EXACT
-    s = scenes.make_solver(scene, precision=prec, dot_mode=dot_mode)
-    n0 = s.seed_fill(scene.fill_ppc, scene.fill_lo, scene.fill_y, scene.fill_seed)
-    print(f"[bench] {args.config} grid {s.nx}x{s.ny}x{s.nz} particles {n0} on {arch}; "
-          f"setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    dt = scene.dt
-    for k in range(args.warmup):
-        t = time.perf_counter()
-        s.substep(dt)
-        s.synchronize()
-        print(f"[bench] warmup {k}: {time.perf_counter() - t:.3f}s iters {s.last_pcg_iters}",
-              file=sys.stderr, flush=True)
-    s.enable_timing(True)
-    s.synchronize()
-    particle_steps = 0
-    iters = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        particle_steps += s.particle_count()
-        t = time.perf_counter()
-        s.substep(dt)
-        iters.append(s.last_pcg_iters)
-        print(f"[bench] step {k}: {time.perf_counter() - t:.3f}s pcg iters {iters[-1]}",
-              file=sys.stderr, flush=True)
-    s.synchronize()
-    elapsed = time.perf_counter() - t_start
-    timing = s.timing()
-    s.enable_timing(False)
+    smode = A.SOLVE_RELATIVE if args.solve == "relative" else A.SOLVE_REFERENCE
+    s, elapsed, iters, particle_steps, timing = run_path(
+        scene, prec, dot_mode, smode, args.tol, args.max_iters, args.steps, args.warmup,
+        args.precision)
     ms_per_step = elapsed * 1e3 / args.steps
     value = particle_steps / elapsed
     sz = 4 if prec == A.PRECISION_F32 else 8
     nc = s.nx * s.ny * s.nz
     nsdf = scene.sdf.nx * scene.sdf.ny * scene.sdf.nz
     npart = particle_steps / max(args.steps, 1)
-    solve_regions = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_bwd", "spmv_dot",
-                     "axpy_norm", "dot_update", "gradient"]
-    pressure_ms = sum(timing.get(r, (0.0, 0))[0] for r in solve_regions) / args.steps
+    pressure_ms = sum(timing.get(r, (0.0, 0))[0] for r in SOLVE_REGIONS) / args.steps
     # Dominant kernel region by total time; roofline from its average call duration.
     best, best_t = None, -1.0
     for name, (ms, calls) in timing.items():
@@ -189,6 +276,24 @@ def main() -> int:
         roof = {"bound": "hbm", "kernel": best, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_call": int(by), "avg_call_ms": round(per_call_s * 1e3, 4)}
+    err = None
+    if not args.no_error and smode == A.SOLVE_RELATIVE:
+        try:
+            err = solve_error(s, scene, 1e-12)
+        except Exception as e:  # the throughput number stands on its own
+            err = {"error": repr(e)}
+    s.close()
+    fp64 = None
+    if not args.no_fp64 and prec == A.PRECISION_F32:
+        s64, el64, it64, ps64, tm64 = run_path(scene, A.PRECISION_F64, dot_mode, smode, args.tol,
+                                               args.max_iters, args.steps, args.warmup, "f64")
+        s64.close()
+        fp64 = {"ms_per_step": round(el64 * 1e3 / args.steps, 3),
+                "value": round(ps64 / el64, 1),
+                "pressure_solve_ms": round(sum(tm64.get(r, (0.0, 0))[0] for r in SOLVE_REGIONS)
+                                           / args.steps, 3),
+                "pcg_iters": [int(i) for i in it64],
+                "speedup_of_f32": round((ps64 / el64) and value / (ps64 / el64), 3)}
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -196,6 +301,7 @@ def main() -> int:
             cpu = cpu_baseline(scene, scene.sdf, float(np.mean(iters)), 20.0, threads)
         except Exception as e:  # the GPU number stands on its own
             cpu = {"error": repr(e)}
+    prec_txt = ("f32 storage + f64 dots/norms/refinement" if prec == A.PRECISION_F32 else "f64")
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -210,13 +316,17 @@ def main() -> int:
         "dtype": "f32+f64" if prec == A.PRECISION_F32 else "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: 256^3 MAC grid rounded cavity + socket, "
-                               f"~{npart / 1e6:.0f}M particles 8/cell, fixed dt {dt:g} s"
+                               f"~{npart / 1e6:.0f}M particles 8/cell, fixed dt {scene.dt:g} s"
                    if args.config == "c3" else args.config,
-                   "grid": [s.nx, s.ny, s.nz], "particles": int(npart), "dt": dt,
-                   "precision": args.precision, "dot": args.dot, "pcg_tol": scene.solve.tol,
-                   "pcg_max_iters": scene.solve.max_iters},
+                   "grid": [s.nx, s.ny, s.nz], "particles": int(npart), "dt": scene.dt,
+                   "precision": prec_txt, "dot": args.dot,
+                   "solve": (f"relative {args.tol:g} (max {args.max_iters} iterations)"
+                             if smode == A.SOLVE_RELATIVE else
+                             f"reference tol {scene.solve.tol:g} / {scene.solve.max_iters}")},
         "pressure_solve_ms": round(pressure_ms, 3),
         "pcg_iters": [int(i) for i in iters],
+        "fp64_gpu": fp64,
+        "error_vs_fp64": err,
         "regions_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(timing.items())},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -224,7 +334,6 @@ def main() -> int:
     if cpu and "value" in cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out))
-    s.close()
     return 0
 
 

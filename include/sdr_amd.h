@@ -49,6 +49,15 @@ extern "C" {
 #define SDR_DOT_EXACT 0
 #define SDR_DOT_FAST 1
 
+/* Stopping rule of the pressure solve.
+ *   SDR_SOLVE_REFERENCE: the reference's `inf_norm(r) <= tol` with `max_iters`
+ *                        (`pressure.rs:341-397`).
+ *   SDR_SOLVE_RELATIVE:  `inf_norm(r) <= refine_tol * inf_norm(b)` within `max_iters`
+ *                        PCG iterations; with SDR_PRECISION_F32 the residual is formed
+ *                        in f64 and the f32 solve is refined until it holds. */
+#define SDR_SOLVE_REFERENCE 0
+#define SDR_SOLVE_RELATIVE 1
+
 typedef struct sdr_vec3 {
   double x, y, z;
 } sdr_vec3;
@@ -103,7 +112,9 @@ typedef struct sdr_options {
   int32_t precision; /* SDR_PRECISION_* */
   int32_t dot_mode;  /* SDR_DOT_* (f64 path only) */
   int32_t device;    /* HIP device ordinal */
-  double refine_tol; /* f32 path: relative residual target of the f64 refinement (1e-8) */
+  double refine_tol; /* SDR_SOLVE_RELATIVE: relative residual target (1e-8) */
+  int32_t solve_mode; /* SDR_SOLVE_* */
+  int32_t _pad;
 } sdr_options;
 
 /* Scalar solver state: the reference's pub fields `time, steps, last_pcg_iters,
@@ -115,7 +126,7 @@ typedef struct sdr_state {
   uint64_t steps;
   uint64_t last_pcg_iters;
   uint32_t last_substeps;
-  uint32_t _pad;
+  uint32_t redist_rounds; /* reservation rounds of the last substep's redistribution */
   uint64_t drained;
   uint64_t rng;
   double emit_accum;

@@ -27,6 +27,8 @@ PRECISION_F64 = 0
 PRECISION_F32 = 1
 DOT_EXACT = 0
 DOT_FAST = 1
+SOLVE_REFERENCE = 0
+SOLVE_RELATIVE = 1
 
 STAGES = [
     "emit", "classify", "p2g", "save", "forces", "project", "extrapolate", "g2p", "advect",
@@ -69,12 +71,12 @@ class SolveReportC(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("dot_mode", C.c_int32), ("device", C.c_int32),
-                ("refine_tol", C.c_double)]
+                ("refine_tol", C.c_double), ("solve_mode", C.c_int32), ("_pad", C.c_int32)]
 
 
 class State(C.Structure):
     _fields_ = [("time", C.c_double), ("steps", C.c_uint64), ("last_pcg_iters", C.c_uint64),
-                ("last_substeps", C.c_uint32), ("_pad", C.c_uint32), ("drained", C.c_uint64),
+                ("last_substeps", C.c_uint32), ("redist_rounds", C.c_uint32), ("drained", C.c_uint64),
                 ("rng", C.c_uint64), ("emit_accum", C.c_double), ("emitted", C.c_uint64),
                 ("culled", C.c_uint64), ("last_residual", C.c_double),
                 ("redist_active", C.c_uint64), ("redist_moved", C.c_uint64)]

@@ -235,7 +235,7 @@ __global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* _
                             const uint32_t* __restrict__ count, const double* __restrict__ srcarr,
                             double inv_scale, int use_source, double rest, double k_vol,
                             double unit, double cap, T* __restrict__ b,
-                            uint16_t* __restrict__ flags) {
+                            uint16_t* __restrict__ flags, double* __restrict__ b64) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= sh.size()) return;
   int i = (int)(t % sh.nx);
@@ -245,6 +245,7 @@ __global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* _
   if (j < 0 || j >= sh.ny) {
     b[t] = T(0);
     flags[t] = 0;
+    if (b64) b64[t] = 0.0;
     return;
   }
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -256,6 +257,7 @@ __global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* _
   if (cells[c] != kFluid) {
     b[t] = T(0);
     flags[t] = 0;
+    if (b64) b64[t] = 0.0;
     return;
   }
   uint16_t fl = kFlFluid;
@@ -284,6 +286,21 @@ __global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* _
   }
   b[t] = T(-inv_scale) * (div - s);
   flags[t] = fl;
+  if (b64) {
+    // the same right-hand side formed in f64 from the stored faces (refinement path)
+    const double du64 = (double)f.u[((int64_t)k * ny + j) * ux + i + 1] - (double)f.u[((int64_t)k * ny + j) * ux + i];
+    const double dv64 = (double)f.v[((int64_t)k * (ny + 1) + j + 1) * nx + i] -
+                        (double)f.v[((int64_t)k * (ny + 1) + j) * nx + i];
+    const double dw64 = (double)f.w[((int64_t)(k + 1) * ny + j) * nx + i] - (double)f.w[((int64_t)k * ny + j) * nx + i];
+    double s64 = 0.0;
+    if (srcarr) {
+      s64 = srcarr[c];
+    } else if (use_source) {
+      const double over = (double)count[c] / rest - 1.0;
+      if (over > 0.0) s64 = fmin(k_vol * over * unit, cap);
+    }
+    b64[t] = -inv_scale * (((du64 + dv64) + dw64) - s64);
+  }
 }
 
 // ------------------------------------------------------------------ apply_a (pressure.rs:400-439)
@@ -389,7 +406,7 @@ __global__ void __launch_bounds__(256) k_dot_fast(Shear sh, const T* __restrict_
 
 // Combine partials and apply the scalar step of the PCG loop (pressure.rs:345-389).
 __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblocks, int exact, int op,
-                                                    PcgScalars* sc, double tol) {
+                                                    PcgScalars* sc, double tol, int rel) {
   if (sc->done) return;
   __shared__ double red[4];
   double v;
@@ -433,12 +450,13 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
     case kOpResidual:
       sc->residual = v;
       if (v != v) sc->nan_seen = 1;
-      if (v <= tol) sc->done = 1;
+      if (v <= sc->tol_eff) sc->done = 1;
       break;
     case kOpRhsNorm:
       sc->residual = v;
       sc->rhs_norm = v;
-      if (v <= tol) sc->done = 1;
+      sc->tol_eff = rel ? tol * v : tol;
+      if (v <= sc->tol_eff) sc->done = 1;
       break;
   }
 }
@@ -586,12 +604,92 @@ __global__ void k_gradient(GridD<T> g, Shear sh, T* __restrict__ a, const uint8_
   if (l == kFluid || r == kFluid) a[f] -= coef * (p[sh.at(i, j, k)] - p[sh.at(li, lj, lk)]);
 }
 
-template <class T>
-__global__ void k_unshear(Shear sh, const T* __restrict__ p, T* __restrict__ out) {
+// The refinement path's gradient: the f64 pressure difference applied in f64 to the
+// stored face, rounded once.
+template <class T, int COMP>
+__global__ void k_gradient64(GridD<T> g, Shear sh, T* __restrict__ a, const uint8_t* __restrict__ cells,
+                             const double* __restrict__ p, double coef) {
+  const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (int64_t)fnx * fny * fnz) return;
+  int i = (int)(f % fnx), j = (int)((f / fnx) % fny), k = (int)(f / ((int64_t)fnx * fny));
+  int li = i - (COMP == 0), lj = j - (COMP == 1), lk = k - (COMP == 2);
+  if (li < 0 || lj < 0 || lk < 0) return;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  uint8_t l = cells[((int64_t)lk * g.ny + lj) * g.nx + li];
+  uint8_t r = cells[((int64_t)k * g.ny + j) * g.nx + i];
+  if (l == kSolid || r == kSolid) return;
+  if (l == kFluid || r == kFluid)
+    a[f] = (T)((double)a[f] - coef * (p[sh.at(i, j, k)] - p[sh.at(li, lj, lk)]));
+}
+
+template <class P, class T>
+__global__ void k_unshear(Shear sh, const P* __restrict__ p, T* __restrict__ out) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int64_t)sh.nx * sh.ny * sh.nz) return;
   int i = (int)(c % sh.nx), j = (int)((c / sh.nx) % sh.ny), k = (int)(c / ((int64_t)sh.nx * sh.ny));
-  out[c] = p[sh.at(i, j, k)];
+  out[c] = (T)p[sh.at(i, j, k)];
+}
+
+// ---- iterative refinement in f64 (f32 storage path)
+// r = b - A p in f64 from the flags (the reference apply_a operand order), with max|r|
+// partials per block.
+__global__ void __launch_bounds__(256) k_residual64(Shear sh, const uint16_t* __restrict__ flags,
+                                                    const double* __restrict__ b,
+                                                    const double* __restrict__ x, double* __restrict__ r,
+                                                    double* partials) {
+  __shared__ double red[4];
+  const int64_t n = sh.size();
+  const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const uint16_t f = flags[t];
+    double v = 0.0;
+    if (f & kFlFluid) {
+      const int i = (int)(t % sh.nx);
+      const int J = (int)((t / sh.nx) % sh.NJ);
+      const int k = (int)(t / plane);
+      const int j = J - i;
+      double ax = (double)fl_diag(f) * x[t];
+      if (i + 1 < sh.nx) ax += ((f & kFlPx) ? -1.0 : 0.0) * x[t + row + 1];
+      if (i >= 1) ax += ((f & kFlMx) ? -1.0 : 0.0) * x[t - row - 1];
+      if (j + 1 < sh.ny) ax += ((f & kFlPy) ? -1.0 : 0.0) * x[t + row];
+      if (j >= 1) ax += ((f & kFlMy) ? -1.0 : 0.0) * x[t - row];
+      if (k + 1 < sh.nz) ax += ((f & kFlPz) ? -1.0 : 0.0) * x[t + plane];
+      if (k >= 1) ax += ((f & kFlMz) ? -1.0 : 0.0) * x[t - plane];
+      v = b[t] - ax;
+    }
+    r[t] = v;
+    m = fmax(m, fabs(v));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+}
+
+__global__ void __launch_bounds__(256) k_max_finish(const double* partials, int nblocks, double* out) {
+  __shared__ double red[4];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < nblocks; q += 256) m = fmax(m, partials[q]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+}
+
+// rhs of a correction solve: b = r * inv (rounded to the storage type)
+template <class T>
+__global__ void k_scale_cast(int64_t n, const double* __restrict__ r, double inv, T* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (T)(r[t] * inv);
+}
+
+// p += scale * e in f64
+template <class T>
+__global__ void k_accum64(int64_t n, double* __restrict__ p, const T* __restrict__ e, double scale) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] += scale * (double)e[t];
 }
 
 // ------------------------------------------------------------------ extrapolate (solver.rs:608-647, 960-1009)
@@ -883,12 +981,18 @@ __global__ void k_rd_sources(const uint32_t* __restrict__ scell, int64_t nact,
   src_lim[q] = (uint32_t)a + (count[c] - cap);
 }
 
+// A head reserves every open receiver it could still ever write (rooms only shrink, so
+// later particles of its cell touch a subset), but its own decision reads only the open
+// receivers of the best gravity class: a receiver of a higher class is closed for good,
+// and one of a lower class loses to any open member of the best class.  It commits when
+// it holds that class (src_rmask), even if an earlier head holds one of its other
+// receivers, because that head's particles never touch the class it holds.
 __global__ void k_rd_reserve(const uint32_t* __restrict__ nsrc, const uint32_t* __restrict__ src_cell,
                              uint32_t* src_head, const uint32_t* __restrict__ src_lim,
                              uint8_t* src_rmask, const uint32_t* __restrict__ spos,
                              const uint32_t* __restrict__ count, const uint8_t* __restrict__ mask,
-                             int nx, int ny, int nz, uint32_t cap, unsigned long long* resv,
-                             uint32_t epoch, uint32_t* pending) {
+                             int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
+                             unsigned long long* resv, uint32_t epoch, uint32_t* pending) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s == 0) *pending = 0;
   if (s >= *nsrc) return;
@@ -901,12 +1005,17 @@ __global__ void k_rd_reserve(const uint32_t* __restrict__ nsrc, const uint32_t* 
   const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
   const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
   uint8_t rm = 0;
+  double best = -INFINITY;
   for (int q = 0; q < 6; ++q) {
     if (!in[q] || !mask[nb[q]] || count[nb[q]] >= cap) continue;
     rm |= (uint8_t)(1u << q);
+    best = fmax(best, dn.d[q]);
     atomicMax(resv + nb[q], val);
   }
-  src_rmask[s] = rm;
+  uint8_t fm = 0;
+  for (int q = 0; q < 6; ++q)
+    if (((rm >> q) & 1u) && dn.d[q] == best) fm |= (uint8_t)(1u << q);
+  src_rmask[s] = fm;
   if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
 }
 
@@ -1130,10 +1239,10 @@ template <class T>
 void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
                       const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
                       double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
-                      hipStream_t st) {
+                      double* b64, hipStream_t st) {
   k_rhs_flags<T><<<grid_for(sh.size()), kBlock, 0, st>>>(g, sh, f, cells, count, srcarr, inv_scale,
                                                           use_source, rest, k_vol, unit, cap, b,
-                                                          flags);
+                                                          flags, b64);
 }
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
@@ -1159,8 +1268,8 @@ void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int
     k_dot_fast<T><<<nblocks, 256, 0, st>>>(sh, a, b, partials, sc);
 }
 void launch_dot_finish(double* partials, int nblocks, bool exact, int op, PcgScalars* sc,
-                       double tol, hipStream_t st) {
-  k_dot_finish<<<1, 256, 0, st>>>(partials, nblocks, exact ? 1 : 0, op, sc, tol);
+                       double tol, hipStream_t st, int rel) {
+  k_dot_finish<<<1, 256, 0, st>>>(partials, nblocks, exact ? 1 : 0, op, sc, tol, rel);
 }
 template <class T>
 void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
@@ -1195,7 +1304,33 @@ void launch_gradient(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, co
 }
 template <class T>
 void launch_unshear(Shear sh, const T* p, T* out, hipStream_t st) {
-  k_unshear<T><<<grid_for((int64_t)sh.nx * sh.ny * sh.nz), kBlock, 0, st>>>(sh, p, out);
+  k_unshear<T, T><<<grid_for((int64_t)sh.nx * sh.ny * sh.nz), kBlock, 0, st>>>(sh, p, out);
+}
+template <class T>
+void launch_gradient64(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, const double* p,
+                       double coef, hipStream_t st) {
+  int64_t nu = (int64_t)(g.nx + 1) * g.ny * g.nz, nv = (int64_t)g.nx * (g.ny + 1) * g.nz,
+          nw = (int64_t)g.nx * g.ny * (g.nz + 1);
+  k_gradient64<T, 0><<<grid_for(nu), kBlock, 0, st>>>(g, sh, f.u, cells, p, coef);
+  k_gradient64<T, 1><<<grid_for(nv), kBlock, 0, st>>>(g, sh, f.v, cells, p, coef);
+  k_gradient64<T, 2><<<grid_for(nw), kBlock, 0, st>>>(g, sh, f.w, cells, p, coef);
+}
+template <class T>
+void launch_unshear64(Shear sh, const double* p, T* out, hipStream_t st) {
+  k_unshear<double, T><<<grid_for((int64_t)sh.nx * sh.ny * sh.nz), kBlock, 0, st>>>(sh, p, out);
+}
+void launch_residual64(Shear sh, const uint16_t* flags, const double* b, const double* x, double* r,
+                       double* partials, int nblocks, double* out, hipStream_t st) {
+  k_residual64<<<nblocks, 256, 0, st>>>(sh, flags, b, x, r, partials);
+  k_max_finish<<<1, 256, 0, st>>>(partials, nblocks, out);
+}
+template <class T>
+void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t st) {
+  k_scale_cast<T><<<grid_for(n), kBlock, 0, st>>>(n, r, inv, b);
+}
+template <class T>
+void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st) {
+  k_accum64<T><<<grid_for(n), kBlock, 0, st>>>(n, p, e, scale);
 }
 template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st) {
@@ -1277,7 +1412,8 @@ void launch_rd_round(int64_t max_src, const uint32_t* nsrc, const uint32_t* src_
                      unsigned long long* moved, hipStream_t st) {
   if (max_src <= 0) return;
   k_rd_reserve<<<grid_for(max_src), kBlock, 0, st>>>(nsrc, src_cell, src_head, src_lim, src_rmask, spos,
-                                                     count, mask, nx, ny, nz, cap, resv, epoch, pending);
+                                                     count, mask, nx, ny, nz, cap, dn, resv, epoch,
+                                                     pending);
   k_rd_commit<<<grid_for(max_src), kBlock, 0, st>>>(nsrc, src_cell, src_head, src_lim, src_rmask, spos,
                                                     count, mask, nx, ny, nz, cap, dn, resv, epoch, dest,
                                                     pending, moved);
@@ -1347,7 +1483,7 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
   template void launch_enforce_solid<T>(GridD<T>, FacesD<T>, const uint8_t*, hipStream_t);         \
   template void launch_rhs_flags<T>(GridD<T>, Shear, CFacesD<T>, const uint8_t*, const uint32_t*,  \
                                     const double*, double, int, double, double, double, double,    \
-                                    T*, uint16_t*, hipStream_t);                                   \
+                                    T*, uint16_t*, double*, hipStream_t);                          \
   template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
                                hipStream_t);                                                       \
   template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
@@ -1362,6 +1498,11 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
   template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \
                                    hipStream_t);                                                   \
   template void launch_unshear<T>(Shear, const T*, T*, hipStream_t);                               \
+  template void launch_gradient64<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const double*,     \
+                                     double, hipStream_t);                                         \
+  template void launch_unshear64<T>(Shear, const double*, T*, hipStream_t);                        \
+  template void launch_scale_cast<T>(int64_t, const double*, double, T*, hipStream_t);             \
+  template void launch_accum64<T>(int64_t, double*, const T*, double, hipStream_t);                \
   template void launch_extrap_init<T>(GridD<T>, int, const uint8_t*, uint8_t*, hipStream_t);       \
   template void launch_extrap_iter<T>(int, int, int, const T*, T*, const uint8_t*, uint8_t*,       \
                                       hipStream_t);                                                \

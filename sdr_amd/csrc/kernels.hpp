@@ -29,6 +29,7 @@ struct PcgScalars {
   int32_t done;
   int32_t nan_seen;
   int32_t pad;
+  double tol_eff;  // the residual bound in use: tol, or tol * max|b| in relative mode
 };
 
 enum DotOp : int32_t {
@@ -70,7 +71,7 @@ template <class T>
 void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
                       const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
                       double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
-                      hipStream_t st);
+                      double* b64, hipStream_t st);
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
                  int it, hipStream_t st);
@@ -78,8 +79,9 @@ template <class T>
 void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,
                          const PcgScalars* sc, hipStream_t st);
 int dot_blocks(Shear sh, bool exact);
+// rel = 1: kOpRhsNorm sets the residual bound to tol * max|b| (relative stop)
 void launch_dot_finish(double* partials, int nblocks, bool exact, int op, PcgScalars* sc,
-                       double tol, hipStream_t st);
+                       double tol, hipStream_t st, int rel = 0);
 template <class T>
 void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
                             const PcgScalars* sc, hipStream_t st);
@@ -97,6 +99,19 @@ void launch_gradient(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, co
                      hipStream_t st);
 template <class T>
 void launch_unshear(Shear sh, const T* p, T* out, hipStream_t st);
+// f64 refinement of the f32 solve
+template <class T>
+void launch_gradient64(GridD<T> g, Shear sh, FacesD<T> f, const uint8_t* cells, const double* p,
+                       double coef, hipStream_t st);
+template <class T>
+void launch_unshear64(Shear sh, const double* p, T* out, hipStream_t st);
+// r = b - A x and *out = max|r| (nblocks partials)
+void launch_residual64(Shear sh, const uint16_t* flags, const double* b, const double* x, double* r,
+                       double* partials, int nblocks, double* out, hipStream_t st);
+template <class T>
+void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t st);
+template <class T>
+void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st);
 template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st);
 template <class T>

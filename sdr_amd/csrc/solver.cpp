@@ -193,6 +193,7 @@ struct Projector {
   Timer* timer = nullptr;
   Buf b, z, s, p, as, q, precon, flags, partials, scal;
   Buf halo_x, halo_z, pipe_err, sink;
+  Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
@@ -278,18 +279,20 @@ struct Projector {
     return r;
   }
 
-  // pcg (pressure.rs:341-397) on b (becomes r); the solution is left in p.
-  PcgScalars pcg(uint64_t max_iters, double tol) {
+  // pcg (pressure.rs:341-397) on b (becomes r); the solution is left in p.  rel: stop at
+  // tol * max|b| instead of tol.  build: factor the preconditioner (else reuse precon).
+  PcgScalars pcg(uint64_t max_iters, double tol, bool rel = false, bool build = true) {
     const size_t n = (size_t)sh.size();
     {
       Scope sc_(*timer, "pcg_setup");
-      for (Buf* bb : {&z, &s, &p, &as, &q, &precon}) HIPCHK(hipMemsetAsync(bb->p, 0, n * sizeof(T), st));
+      for (Buf* bb : {&z, &s, &p, &as, &q}) HIPCHK(hipMemsetAsync(bb->p, 0, n * sizeof(T), st));
+      if (build) HIPCHK(hipMemsetAsync(precon.p, 0, n * sizeof(T), st));
       HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(PcgScalars), st));
       int mb = maxabs_blocks(sh);
       launch_maxabs_partials<T>(sh, b.as<T>(), partials.as<double>(), mb, sc(), st);
-      launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), tol, st);
+      launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), tol, st, rel ? 1 : 0);
     }
-    {
+    if (build) {
       Scope sc_(*timer, "mic_build");
       mic_sweep(kMicBuild, nullptr, nullptr);
     }
@@ -338,11 +341,73 @@ struct Projector {
     return read_scalars();
   }
 
+  // Mixed-precision iterative refinement (f32 storage): r = b - A p in f64, solve
+  // A e = r / |r| by the f32 MIC(0)-PCG (its dots and norms accumulate in f64), p += |r| e,
+  // until max|r| <= rel_tol * max|b|.  Each correction solve is asked for kInner of the
+  // scaled residual (or less in the last pass), which f32 arithmetic still delivers.
+  // Returns the total PCG iterations; the f64 pressure is left in p64.
+  PcgScalars refine(uint64_t max_iters, double rel_tol, double* final_res) {
+    const size_t n = (size_t)sh.size();
+    const double kInner = 1e-4;
+    const int mb = maxabs_blocks(sh);
+    double* hn = (double*)host_scal.p;
+    // with p = 0 the residual kernel gives r = b and max|b|
+    HIPCHK(hipMemsetAsync(p64.p, 0, n * 8, st));
+    launch_residual64(sh, flags.as<uint16_t>(), b64.as<double>(), p64.as<double>(), r64.as<double>(),
+                      partials.as<double>(), mb, norms.as<double>(), st);
+    HIPCHK(hipMemcpyAsync(hn, norms.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double bnorm = hn[0];
+    double rnorm = bnorm;
+    PcgScalars tot{};
+    uint64_t used = 0;
+    bool first = true;
+    for (int outer = 0; outer < 16 && rnorm > rel_tol * bnorm && used < max_iters; ++outer) {
+      {
+        Scope sc_(*timer, "refine");
+        launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / rnorm, b.as<T>(), st);
+      }
+      const double inner = std::max(kInner, 0.5 * rel_tol * bnorm / rnorm);
+      PcgScalars r = pcg(max_iters - used, inner, false, first);
+      if (mic_variant >= 0) check_pipe();
+      first = false;
+      used += (uint64_t)r.iters;
+      tot.nan_seen |= r.nan_seen;
+      {
+        Scope sc_(*timer, "refine");
+        launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), rnorm, st);
+        launch_residual64(sh, flags.as<uint16_t>(), b64.as<double>(), p64.as<double>(),
+                          r64.as<double>(), partials.as<double>(), mb, norms.as<double>(), st);
+        HIPCHK(hipMemcpyAsync(hn, norms.p, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+      }
+      const double prev = rnorm;
+      rnorm = hn[0];
+      if (!(rnorm == rnorm)) {
+        tot.nan_seen = 1;
+        break;
+      }
+      if (r.iters == 0 || rnorm >= prev) break;  // no progress left in f32
+    }
+    tot.iters = (int32_t)used;
+    tot.residual = rnorm;
+    tot.rhs_norm = bnorm;
+    if (final_res) *final_res = bnorm > 0 ? rnorm / bnorm : 0.0;
+    return tot;
+  }
+
   // Steps 1-5 of project_capturing_with_source on the device faces (MAC layout).
+  // relative: stop at rel_tol * max|b| (f32 storage: by f64 iterative refinement).
   sdr_solve_report project(GridD<T> g, FacesD<T> f, const uint8_t* cells, const uint32_t* count,
                            const double* srcarr, double rho, double dt, uint64_t max_iters,
                            double tol, bool use_count_source, double rest, double k_vol,
-                           T* pressure_out, int* nan_flag) {
+                           T* pressure_out, int* nan_flag, bool relative = false,
+                           double rel_tol = 0.0) {
+    const bool ref = relative && sizeof(T) == 4;
+    if (ref && !p64.p) {
+      const size_t n = (size_t)sh.size();
+      b64.alloc(n * 8), p64.alloc(n * 8), r64.alloc(n * 8), norms.alloc(64);
+    }
     {
       Scope sc_(*timer, "rhs");
       launch_enforce_solid<T>(g, f, cells, st);
@@ -350,15 +415,25 @@ struct Projector {
       double unit_ = g.dx / dt;
       launch_rhs_flags<T>(g, sh, CFacesD<T>{f.u, f.v, f.w}, cells, count, srcarr, inv_scale,
                           use_count_source ? 1 : 0, rest, k_vol, unit_, 0.25 * unit_, b.as<T>(),
-                          flags.as<uint16_t>(), st);
+                          flags.as<uint16_t>(), ref ? b64.as<double>() : nullptr, st);
     }
-    PcgScalars r = pcg(max_iters, tol);
-    if (mic_variant >= 0) check_pipe();
+    PcgScalars r;
+    if (ref) {
+      r = refine(max_iters, rel_tol, nullptr);
+    } else {
+      r = pcg(max_iters, relative ? rel_tol : tol, relative);
+      if (mic_variant >= 0) check_pipe();
+    }
     {
       Scope sc_(*timer, "gradient");
       double coef = dt / (rho * g.dx);
-      launch_gradient<T>(g, sh, f, cells, p.as<T>(), (T)coef, st);
-      if (pressure_out) launch_unshear<T>(sh, p.as<T>(), pressure_out, st);
+      if (ref) {
+        launch_gradient64<T>(g, sh, f, cells, p64.as<double>(), coef, st);
+        if (pressure_out) launch_unshear64<T>(sh, p64.as<double>(), pressure_out, st);
+      } else {
+        launch_gradient<T>(g, sh, f, cells, p.as<T>(), (T)coef, st);
+        if (pressure_out) launch_unshear<T>(sh, p.as<T>(), pressure_out, st);
+      }
     }
     if (nan_flag) *nan_flag = r.nan_seen;
     return sdr_solve_report{(uint64_t)r.iters, r.residual};
@@ -443,7 +518,7 @@ struct Solver : SolverBase {
   // Solver::new (solver.rs:184-222).
   void create(const sdr_sdf* s, double sim_dx, const sdr_fluid_params* f, const sdr_needle* nd,
               const sdr_options* o) {
-    opts = o ? *o : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8};
+    opts = o ? *o : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8, SDR_SOLVE_REFERENCE, 0};
     HIPCHK(hipSetDevice(opts.device));
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     timer.st = st;
@@ -641,7 +716,8 @@ struct Solver : SolverBase {
     int nanf = 0;
     sdr_solve_report r = proj.project(grid(), faces(), cells.as<uint8_t>(), count.as<uint32_t>(),
                                       nullptr, fluid.density, dt, solve.max_iters, solve.tol, use_src,
-                                      (double)ppc, fluid.volume_stiffness, pressure.as<T>(), &nanf);
+                                      (double)ppc, fluid.volume_stiffness, pressure.as<T>(), &nanf,
+                                      opts.solve_mode == SDR_SOLVE_RELATIVE, opts.refine_tol);
     last_pcg_iters = r.iters;
     last_residual = r.residual;
   }
@@ -764,7 +840,7 @@ struct Solver : SolverBase {
 
   // redistribute (solver.rs:454-557)
   void redistribute() {
-    redist_active = redist_moved = 0;
+    redist_active = redist_moved = redist_rounds = 0;
     if (n == 0) return;
     Scope sc_(timer, "redistribute");
     uint64_t target = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
@@ -1091,6 +1167,7 @@ int sdr_solver_get_state(const sdr_solver* s, sdr_state* st) {
               st->last_substeps = S.last_substeps; st->drained = S.drained; st->rng = S.rng;
               st->emit_accum = S.emit_accum; st->emitted = S.emitted; st->culled = S.culled;
               st->last_residual = S.last_residual; st->redist_active = S.redist_active;
+              st->redist_rounds = (uint32_t)S.redist_rounds;
               st->redist_moved = S.redist_moved);
 }
 int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
@@ -1126,7 +1203,7 @@ int sdr_project_capturing_with_source(const sdr_dims* gd, double* u, double* v, 
   return guard([&] {
     if (!gd || !u || !v || !w || !cells) throw Error(SDR_E_INVALID, "null argument");
     if (gd->nx < 2 || gd->ny < 2 || gd->nz < 2) throw Error(SDR_E_INVALID, "grid needs >= 2 cells per axis");
-    sdr_options o = opts ? *opts : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8};
+    sdr_options o = opts ? *opts : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8, SDR_SOLVE_REFERENCE, 0};
     if (o.precision != SDR_PRECISION_F64) throw Error(SDR_E_INVALID, "standalone projection is f64");
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
@@ -1159,7 +1236,8 @@ int sdr_project_capturing_with_source(const sdr_dims* gd, double* u, double* v, 
     sdr_solve_params prm = params ? *params : sdr_solve_params{200, 1e-6};
     sdr_solve_report r = pj.project(g, sdr::FacesD<double>{du.as<double>(), dv.as<double>(), dw.as<double>()},
                                     dc.as<uint8_t>(), nullptr, source ? ds.as<double>() : nullptr, rho, dt,
-                                    prm.max_iters, prm.tol, false, 1.0, 0.0, dp.as<double>(), nullptr);
+                                    prm.max_iters, prm.tol, false, 1.0, 0.0, dp.as<double>(), nullptr,
+                                    o.solve_mode == SDR_SOLVE_RELATIVE, o.refine_tol);
     HIPCHK(hipMemcpyAsync(u, du.p, nu * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(v, dv.p, nv * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, st));

@@ -136,13 +136,14 @@ class Solver:
 
     def __init__(self, solid: Sdf, sim_dx: float, fluid: FluidParams, needle: Needle, *,
                  precision: int = A.PRECISION_F64, dot_mode: int = A.DOT_EXACT, device: int = 0,
-                 refine_tol: float = 1e-8, lib: A.Lib | None = None):
+                 refine_tol: float = 1e-8, solve_mode: int = A.SOLVE_REFERENCE,
+                 lib: A.Lib | None = None):
         self.lib = lib if lib is not None else _lib_default()
         self.fluid = fluid
         self._needle = needle
         self.solid = solid
         self._h = C.c_void_p()
-        opts = A.Options(precision, dot_mode, device, refine_tol)
+        opts = A.Options(precision, dot_mode, device, refine_tol, solve_mode, 0)
         sd = solid.c()
         self.lib.solver_new(C.byref(sd), sim_dx, C.byref(fluid.c()), C.byref(needle.c()),
                             C.byref(opts), C.byref(self._h))
@@ -314,7 +315,8 @@ class Solver:
 
 def project_capturing_with_source(nx, ny, nz, dx, origin, u, v, w, cells, rho, dt,
                                   params: SolveParams = SolveParams(), source=None, *,
-                                  lib: A.Lib | None = None, dot_mode: int = A.DOT_EXACT):
+                                  lib: A.Lib | None = None, dot_mode: int = A.DOT_EXACT,
+                                  solve_mode: int = A.SOLVE_REFERENCE, refine_tol: float = 1e-8):
     """`pressure::project_capturing_with_source` (src/pressure.rs:123-239).
 
     u, v, w are updated in place (f64 arrays of the staggered dims); returns
@@ -334,7 +336,7 @@ def project_capturing_with_source(nx, ny, nz, dx, origin, u, v, w, cells, rho, d
     d = A.Dims(nx, ny, nz, dx, A.vec3(origin))
     p = np.zeros((nz, ny, nx), np.float64)
     rep = A.SolveReportC()
-    opts = A.Options(A.PRECISION_F64, dot_mode, 0, 1e-8)
+    opts = A.Options(A.PRECISION_F64, dot_mode, 0, refine_tol, solve_mode, 0)
     lib.project_capturing_with_source(C.byref(d), A.dptr(u), A.dptr(v), A.dptr(w), A.u8ptr(cells),
                                       rho, dt, C.byref(A.SolveParamsC(int(params.max_iters), params.tol)),
                                       A.dptr(source), A.dptr(p), C.byref(rep), C.byref(opts))

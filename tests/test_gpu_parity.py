@@ -321,3 +321,68 @@ def test_redistribute_decision_paths_bit_exact(dev, orc, sequential, monkeypatch
         d.substep(sc.dt)
     _compare_all(o, d, exact=True)
     assert d.state().redist_moved == o.state().redist_moved
+
+
+def _precision_problem(seed=5):
+    """A cavity-shaped projection problem on the 36x42x34 grid: fluid below a plane in
+    the cavity mask, air above, solid outside; seeded random faces rounded to f32 (the
+    storage of the precision path), so the f64 oracle sees the same inputs."""
+    sc = scenes.cavity_scene("c1")
+    s = scenes.make_solver(sc)
+    s.run_stage("classify")
+    mask = s.cells() != 0
+    s.close()
+    nz, ny, nx = mask.shape
+    cells = np.where(mask, np.uint8(2), np.uint8(0))
+    jj = np.arange(ny)[None, :, None] * np.ones((nz, 1, nx), int)
+    cells[mask & (jj < ny * 2 // 3)] = FLUID
+    g = np.random.default_rng(seed)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    u = f32(g.normal(size=(nz, ny, nx + 1)))
+    v = f32(g.normal(size=(nz, ny + 1, nx)))
+    w = f32(g.normal(size=(nz + 1, ny, nx)))
+    return sc, cells, u, v, w
+
+
+def _oracle_converged(orc, sc, cells, u, v, w, dt, rel):
+    nz, ny, nx = cells.shape
+    s = scenes.make_solver(sc, lib=orc)
+    args = (nx, ny, nz, s.dx, s.origin)
+    s.close()
+    rho = sc.fluid.density
+    rep0, _ = project_capturing_with_source(*args, u.copy(), v.copy(), w.copy(), cells, rho, dt,
+                                            SolveParams(0, 0.0), lib=orc)
+    bmax = rep0.residual  # zero iterations report max|b|
+    uo, vo, wo = u.copy(), v.copy(), w.copy()
+    rep, p = project_capturing_with_source(*args, uo, vo, wo, cells, rho, dt,
+                                           SolveParams(20000, rel * bmax), lib=orc)
+    assert rep.residual <= rel * bmax
+    return bmax, p, (uo, vo, wo)
+
+
+@pytest.mark.parametrize("precision", [A.PRECISION_F32, A.PRECISION_F64])
+def test_relative_solve_matches_converged_oracle(dev, orc, precision):
+    """SDR_SOLVE_RELATIVE: the f32 path (f64 residual, iterative refinement) and the f64
+    path stop at max|r| <= 1e-8 max|b|; their pressure is within 1e-6 (relative, inf-norm)
+    of the f64 oracle's solve of the same inputs run to 1e-12, and the projected faces
+    within 1e-5."""
+    sc, cells, u, v, w = _precision_problem()
+    dt = sc.dt
+    bmax, p_ref, faces_ref = _oracle_converged(orc, sc, cells, u, v, w, dt, 1e-12)
+    fluid = sc.fluid
+    fluid.volume_stiffness = 0.0
+    sc.solve = SolveParams(5000, 1e-6)
+    d = scenes.make_solver(sc, precision=precision, dot_mode=A.DOT_FAST,
+                           solve_mode=A.SOLVE_RELATIVE, refine_tol=1e-8)
+    d.set_cells(cells)
+    d.set_grid(u, v, w)
+    d.run_stage("project", dt)
+    st = d.state()
+    assert 0 < st.last_pcg_iters < 5000
+    assert st.last_residual <= 1e-8 * bmax * (1 + 1e-12)
+    p = d.pressure()
+    err = np.abs(p - p_ref).max() / np.abs(p_ref).max()
+    assert err <= 1e-6, err
+    for a, b in zip(d.grid(), faces_ref):
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    d.close()
