@@ -155,7 +155,7 @@ SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_bwd", "spmv_do
                  "dot_update", "gradient", "refine"]
 
 
-def solve_error(s, scene, rel_ref):
+def solve_error(s, scene, rel_ref, rel_tol, f32):
     """Pressure and divergence error of the solver's projection of one substep against an
     f64 solve of the same inputs (the f64 device path, which reproduces the CPU oracle
     bit for bit in reference mode) run to a relative residual of `rel_ref`."""
@@ -169,8 +169,11 @@ def solve_error(s, scene, rel_ref):
     cells = s.cells()
     # The solver's projection applies the volume-control source of its particle counts
     # (solver.rs:392-423); the reference solve gets the same source explicitly.
+    # cell_of (solver.rs:897-908) in the solver's own arithmetic: with f32 storage a
+    # particle within rounding distance of a cell face counts where the device puts it
     pos, _ = s.particles()
-    g = (pos - np.asarray(s.origin)) / s.dx
+    ft = np.float32 if f32 else np.float64
+    g = (pos.astype(ft) - np.asarray(s.origin).astype(ft)) / ft(s.dx)
     ok = np.all(g >= 0, axis=1)
     ijk = g.astype(np.int64)
     ok &= (ijk[:, 0] < s.nx) & (ijk[:, 1] < s.ny) & (ijk[:, 2] < s.nz)
@@ -192,6 +195,12 @@ def solve_error(s, scene, rel_ref):
         SolveParams(20000, 0.0), src, dot_mode=A.DOT_FAST, solve_mode=A.SOLVE_RELATIVE,
         refine_tol=rel_ref)
     t_ref = time.perf_counter() - t
+    # the plain f64 path stopped by the same rule as the solver under test
+    u8, v8, w8 = u.copy(), v.copy(), w.copy()
+    rep8, p8 = project_capturing_with_source(
+        s.nx, s.ny, s.nz, s.dx, s.origin, u8, v8, w8, cells, scene.fluid.density, dt,
+        SolveParams(20000, 0.0), src, dot_mode=A.DOT_FAST, solve_mode=A.SOLVE_RELATIVE,
+        refine_tol=rel_tol)
     s.run_stage("project", dt)
     p = s.pressure()
     ud, vd, wd = s.grid()
@@ -211,6 +220,9 @@ def solve_error(s, scene, rel_ref):
         "divergence_out_reference": float(div(uo, vo, wo)),
         "face_rel_err": float(max(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
                                   for a, b in ((ud, uo), (vd, vo), (wd, wo)))),
+        "fp64_same_rule_pressure_rel_err": float(np.abs(p8 - p_ref).max() /
+                                                 max(np.abs(p_ref).max(), 1e-300)),
+        "fp64_same_rule_iters": int(rep8.iters),
     }
 
 
@@ -279,7 +291,7 @@ def main() -> int:
     err = None
     if not args.no_error and smode == A.SOLVE_RELATIVE:
         try:
-            err = solve_error(s, scene, 1e-12)
+            err = solve_error(s, scene, 1e-12, args.tol, prec == A.PRECISION_F32)
         except Exception as e:  # the throughput number stands on its own
             err = {"error": repr(e)}
     s.close()

@@ -303,35 +303,75 @@ __global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* _
   }
 }
 
+// Calls f(t, i, J, k) for every in-grid slot t of the sheared layout.  The in-grid slots
+// of a row (k, J) are the contiguous run i in [max(0, J-ny+1), min(nx-1, J)], so a
+// wavefront walks one row's run and the slots outside the grid (half of the layout) are
+// never touched; rows are strided over the grid's wavefronts.
+template <class F>
+__device__ inline void for_cells(Shear sh, F&& f) {
+  const int lane = threadIdx.x & 63;
+  const int nrows = sh.nz * sh.NJ;
+  const int wstride = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
+  int row = __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  for (; row < nrows; row += wstride) {
+    const int J = row % sh.NJ;
+    const int k = row / sh.NJ;
+    const int imin = J - sh.ny + 1 > 0 ? J - sh.ny + 1 : 0;
+    const int imax = J < sh.nx - 1 ? J : sh.nx - 1;
+    const int64_t base = (int64_t)row * sh.nx;
+    for (int i = imin + lane; i <= imax; i += 64) f(base + i, i, J, k);
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int OPMAX>
+__device__ inline void block_partial(double v, double* partials) {
+  __shared__ double red[4];
+  v = OPMAX ? wave_max(v) : wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = OPMAX ? fmax(fmax(fmax(red[0], red[1]), red[2]), red[3])
+                                 : ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // ------------------------------------------------------------------ apply_a (pressure.rs:400-439)
 // Matrix-free: diag and the -1 couplings come from the flags.  In-bounds neighbours
 // without coupling contribute (+0 * x) exactly as the reference's stored 0.0 does.
 template <class T>
-__global__ void k_spmv(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ x,
-                       T* __restrict__ out, const PcgScalars* sc, int it) {
+__global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restrict__ flags,
+                                              const T* __restrict__ x, T* __restrict__ out,
+                                              const PcgScalars* sc, int it, double* partials) {
   if (sc->done) return;
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
-  if (t >= sh.size()) return;
-  uint16_t f = flags[t];
-  if (!(f & kFlFluid)) {
-    out[t] = T(0);
-    return;
-  }
-  int i = (int)(t % sh.nx);
-  int J = (int)((t / sh.nx) % sh.NJ);
-  int k = (int)(t / ((int64_t)sh.nx * sh.NJ));
-  int j = J - i;
+  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
   const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
-  T v = T(fl_diag(f)) * x[t];
-  // (i+1, j) -> (i+1, J+1); (i, j+1) -> (i, J+1); (i, j, k+1) -> next plane.
-  if (i + 1 < sh.nx) v += ((f & kFlPx) ? T(-1) : T(0)) * x[t + row + 1];
-  if (i >= 1) v += ((f & kFlMx) ? T(-1) : T(0)) * x[t - row - 1];
-  if (j + 1 < sh.ny) v += ((f & kFlPy) ? T(-1) : T(0)) * x[t + row];
-  if (j >= 1) v += ((f & kFlMy) ? T(-1) : T(0)) * x[t - row];
-  if (k + 1 < sh.nz) v += ((f & kFlPz) ? T(-1) : T(0)) * x[t + plane];
-  if (k >= 1) v += ((f & kFlMz) ? T(-1) : T(0)) * x[t - plane];
-  out[t] = v;
+  double acc = 0.0;  // dot(x, A x) of the stored values (fast reduction mode)
+  for_cells(sh, [&](int64_t t, int i, int J, int k) {
+    const uint16_t f = flags[t];
+    T v = T(0);
+    if (f & kFlFluid) {
+      const int j = J - i;
+      v = T(fl_diag(f)) * x[t];
+      // (i+1, j) -> (i+1, J+1); (i, j+1) -> (i, J+1); (i, j, k+1) -> next plane.
+      if (i + 1 < sh.nx) v += ((f & kFlPx) ? T(-1) : T(0)) * x[t + row + 1];
+      if (i >= 1) v += ((f & kFlMx) ? T(-1) : T(0)) * x[t - row - 1];
+      if (j + 1 < sh.ny) v += ((f & kFlPy) ? T(-1) : T(0)) * x[t + row];
+      if (j >= 1) v += ((f & kFlMy) ? T(-1) : T(0)) * x[t - row];
+      if (k + 1 < sh.nz) v += ((f & kFlPz) ? T(-1) : T(0)) * x[t + plane];
+      if (k >= 1) v += ((f & kFlMz) ? T(-1) : T(0)) * x[t - plane];
+      acc += (double)x[t] * (double)v;
+    }
+    out[t] = v;
+  });
+  if (partials) block_partial<0>(acc, partials);
 }
 
 // ------------------------------------------------------------------ dot (pressure.rs:588-603)
@@ -379,29 +419,15 @@ __global__ void __launch_bounds__(64) k_dot_exact(Shear sh, const T* __restrict_
 
 // Fast mode: deterministic fixed tree in f64 (per-thread strided sum, then a wavefront
 // butterfly, then the waves of the block in order).
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
 
 template <class T>
 __global__ void __launch_bounds__(256) k_dot_fast(Shear sh, const T* __restrict__ a,
                                                   const T* __restrict__ b, double* partials,
                                                   const PcgScalars* sc) {
   if (sc->done) return;
-  __shared__ double red[4];
-  const int64_t n = sh.size();
   double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
-    acc += (double)a[t] * (double)b[t];
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+  for_cells(sh, [&](int64_t t, int, int, int) { acc += (double)a[t] * (double)b[t]; });
+  block_partial<0>(acc, partials);
 }
 
 // Combine partials and apply the scalar step of the PCG loop (pressure.rs:345-389).
@@ -466,15 +492,9 @@ template <class T>
 __global__ void __launch_bounds__(256) k_maxabs(Shear sh, const T* __restrict__ a, double* partials,
                                                 const PcgScalars* sc) {
   if (sc->done) return;
-  __shared__ double red[4];
-  const int64_t n = sh.size();
   double m = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
-    m = fmax(m, fabs((double)a[t]));
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+  for_cells(sh, [&](int64_t t, int, int, int) { m = fmax(m, fabs((double)a[t])); });
+  block_partial<1>(m, partials);
 }
 
 // p += alpha s ; r -= alpha As (pressure.rs:371-375), fused with max|r| partials.
@@ -483,31 +503,24 @@ __global__ void __launch_bounds__(256) k_axpy_pr(Shear sh, T* __restrict__ p, T*
                                                  const T* __restrict__ s, const T* __restrict__ as,
                                                  const PcgScalars* sc, double* partials) {
   if (sc->done) return;
-  __shared__ double red[4];
   const T alpha = (T)sc->alpha;
-  const int64_t n = sh.size();
   double m = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+  for_cells(sh, [&](int64_t t, int, int, int) {
     p[t] += alpha * s[t];
-    T rv = r[t] - alpha * as[t];
+    const T rv = r[t] - alpha * as[t];
     r[t] = rv;
     m = fmax(m, fabs((double)rv));
-  }
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+  });
+  block_partial<1>(m, partials);
 }
 
 // s = z + beta s (pressure.rs:386-388).
 template <class T>
-__global__ void k_update_s(Shear sh, T* __restrict__ s, const T* __restrict__ z,
-                           const PcgScalars* sc) {
+__global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, const T* __restrict__ z,
+                                                  const PcgScalars* sc) {
   if (sc->done) return;
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= sh.size()) return;
   const T beta = (T)sc->beta;
-  s[t] = z[t] + beta * s[t];
+  for_cells(sh, [&](int64_t t, int, int, int) { s[t] = z[t] + beta * s[t]; });
 }
 
 // ------------------------------------------------------------------ MIC(0) (pressure.rs:442-570)
@@ -835,11 +848,23 @@ __global__ void k_rank_fix(uint32_t* rank, int64_t n, const uint32_t* __restrict
 // ------------------------------------------------------------------ redistribute (solver.rs:454-557)
 template <class T>
 __global__ void k_hist(Parts<T> p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t c = g.cell_of(load_pos(p, t));
-  cellkey[t] = (int32_t)c;
-  if (c >= 0) atomicAdd(&count[c], 1u);
+  // Particles are still close to cell order (the last sort), so lanes of a wavefront
+  // mostly share cells: each run of equal cells adds its length with one atomic.
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int c = -1;
+  if (t < n) {
+    c = (int)g.cell_of(load_pos(p, t));
+    cellkey[t] = c;
+  }
+  const int prev = __shfl_up(c, 1);
+  const bool head = lane == 0 || prev != c;
+  const unsigned long long heads = __ballot(head);
+  if (head && c >= 0) {
+    const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    atomicAdd(&count[c], (uint32_t)(end - lane));
+  }
 }
 
 __global__ void k_active_flags(const int32_t* __restrict__ cellkey, const uint32_t* __restrict__ count,
@@ -965,6 +990,30 @@ __device__ inline int64_t redist_choose(int64_t c, const uint32_t* count, const 
   return best;
 }
 
+// One atomic per wavefront for a per-lane flag (a single global counter hit by every
+// lane serialises the whole grid on one address).
+__device__ inline void wave_count(uint32_t* ctr, bool flag) {
+  const unsigned long long m = __ballot(flag);
+  const int lane = threadIdx.x & 63;
+  if (m && lane == __ffsll((long long)m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+__device__ inline void wave_count64(unsigned long long* ctr, bool flag) {
+  const unsigned long long m = __ballot(flag);
+  const int lane = threadIdx.x & 63;
+  if (m && lane == __ffsll((long long)m) - 1) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+// Slot of this lane in a list appended to by every lane with flag set (order within
+// the wave by lane, between waves arbitrary).
+__device__ inline uint32_t wave_append(uint32_t* ctr, bool flag) {
+  const unsigned long long m = __ballot(flag);
+  const int lane = threadIdx.x & 63;
+  const int leader = m ? __ffsll((long long)m) - 1 : 0;
+  uint32_t base = 0;
+  if (m && lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // Sources from the active particles sorted by (cell, reference position): a segment of
 // count[c] entries starts where the cell changes; its first count[c] - cap entries are
 // the candidates.
@@ -972,10 +1021,10 @@ __global__ void k_rd_sources(const uint32_t* __restrict__ scell, int64_t nact,
                              const uint32_t* __restrict__ count, uint32_t cap, uint32_t* src_cell,
                              uint32_t* src_head, uint32_t* src_lim, uint32_t* nsrc) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= nact) return;
-  const uint32_t c = scell[a];
-  if (a > 0 && scell[a - 1] == c) return;
-  const uint32_t q = atomicAdd(nsrc, 1u);
+  const uint32_t c = a < nact ? scell[a] : 0u;
+  const bool start = a < nact && (a == 0 || scell[a - 1] != c);
+  const uint32_t q = wave_append(nsrc, start);
+  if (!start) return;
   src_cell[q] = c;
   src_head[q] = (uint32_t)a;
   src_lim[q] = (uint32_t)a + (count[c] - cap);
@@ -1027,31 +1076,38 @@ __global__ void k_rd_commit(const uint32_t* __restrict__ nsrc, const uint32_t* _
                             uint32_t epoch, int32_t* dest, uint32_t* pending,
                             unsigned long long* moved) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *nsrc) return;
-  const uint32_t head = src_head[s], lim = src_lim[s];
-  if (head >= lim) return;
-  const int64_t c = src_cell[s];
-  const uint32_t a = spos[head];
-  const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - a);
-  const int64_t sy = nx, sz = (int64_t)nx * ny;
-  const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-  const uint8_t rm = src_rmask[s];
-  for (int q = 0; q < 6; ++q)
-    if (((rm >> q) & 1u) && resv[nb[q]] != val) {
-      atomicAdd(pending, 1u);  // an earlier particle holds a shared receiver
-      return;
+  bool pend = false, mv = false;
+  if (s < *nsrc) {
+    const uint32_t head = src_head[s], lim = src_lim[s];
+    if (head < lim) {
+      const int64_t c = src_cell[s];
+      const uint32_t a = spos[head];
+      const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - a);
+      const int64_t sy = nx, sz = (int64_t)nx * ny;
+      const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+      const uint8_t rm = src_rmask[s];
+      bool held = true;
+      for (int q = 0; q < 6; ++q)
+        if (((rm >> q) & 1u) && resv[nb[q]] != val) held = false;
+      if (!held) {
+        pend = true;  // an earlier particle holds a receiver of the class
+      } else {
+        const int64_t best = redist_choose(c, count, mask, nx, ny, nz, cap, dn);
+        if (best < 0) {
+          src_head[s] = lim;
+        } else {
+          count[c] -= 1;
+          count[best] += 1;
+          dest[a] = (int32_t)best;
+          mv = true;
+          src_head[s] = head + 1;
+          pend = head + 1 < lim;
+        }
+      }
     }
-  const int64_t best = redist_choose(c, count, mask, nx, ny, nz, cap, dn);
-  if (best < 0) {
-    src_head[s] = lim;
-    return;
   }
-  count[c] -= 1;
-  count[best] += 1;
-  dest[a] = (int32_t)best;
-  atomicAdd(moved, 1ull);
-  src_head[s] = head + 1;
-  if (head + 1 < lim) atomicAdd(pending, 1u);
+  wave_count(pending, pend);
+  wave_count64(moved, mv);
 }
 
 __global__ void k_iota_u32(uint32_t* out, int64_t n) {
@@ -1244,19 +1300,23 @@ void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
                                                           use_source, rest, k_vol, unit, cap, b,
                                                           flags, b64);
 }
+int cell_blocks(Shear sh) {
+  int64_t waves = (int64_t)sh.nz * sh.NJ;  // one wavefront per row at most
+  int64_t b = (waves + 3) / 4;
+  if (b > 2048) b = 2048;
+  return (int)(b < 1 ? 1 : b);
+}
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
-                 hipStream_t st) {
-  k_spmv<T><<<grid_for(sh.size()), kBlock, 0, st>>>(sh, flags, x, out, sc, it);
+                 double* partials, hipStream_t st) {
+  k_spmv<T><<<cell_blocks(sh), 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
 }
 int dot_blocks(Shear sh, bool exact) {
   if (exact) {
     int64_t n = (int64_t)sh.nx * sh.ny * sh.nz;
     return (int)((n + 4095) / 4096);
   }
-  int64_t b = (sh.size() + 256 * 8 - 1) / (256 * 8);
-  if (b > 2048) b = 2048;
-  return (int)(b < 1 ? 1 : b);
+  return cell_blocks(sh);
 }
 int maxabs_blocks(Shear sh) { return dot_blocks(sh, false); }
 template <class T>
@@ -1283,7 +1343,7 @@ void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScal
 }
 template <class T>
 void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st) {
-  k_update_s<T><<<grid_for(sh.size()), kBlock, 0, st>>>(sh, s, z, sc);
+  k_update_s<T><<<cell_blocks(sh), 256, 0, st>>>(sh, s, z, sc);
 }
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
@@ -1485,7 +1545,7 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
                                     const double*, double, int, double, double, double, double,    \
                                     T*, uint16_t*, double*, hipStream_t);                          \
   template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
-                               hipStream_t);                                                       \
+                               double*, hipStream_t);                                              \
   template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
                                        const PcgScalars*, hipStream_t);                            \
   template void launch_maxabs_partials<T>(Shear, const T*, double*, int, const PcgScalars*,        \

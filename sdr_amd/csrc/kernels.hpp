@@ -72,9 +72,12 @@ void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
                       const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
                       double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
                       double* b64, hipStream_t st);
+// As = A x over the in-grid slots; with partials (cell_blocks(sh) of them) it also
+// leaves the per-block sums of x . As for the fast reduction mode.
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
-                 int it, hipStream_t st);
+                 int it, double* partials, hipStream_t st);
+int cell_blocks(Shear sh);
 template <class T>
 void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,
                          const PcgScalars* sc, hipStream_t st);

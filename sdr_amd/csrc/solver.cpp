@@ -312,8 +312,14 @@ struct Projector {
     for (uint64_t it = 0; it < max_iters; ++it) {
       {
         Scope sc_(*timer, "spmv_dot");
-        launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, st);
-        dot(s.as<T>(), as.as<T>(), kOpDenom, tol);
+        if (exact_dot) {
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, nullptr, st);
+          dot(s.as<T>(), as.as<T>(), kOpDenom, tol);
+        } else {  // dot(s, As) fused into the product
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it,
+                         partials.as<double>(), st);
+          launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
+        }
       }
       {
         Scope sc_(*timer, "axpy_norm");
@@ -383,6 +389,10 @@ struct Projector {
       }
       const double prev = rnorm;
       rnorm = hn[0];
+      if (getenv("SDR_TRACE_REFINE"))
+        fprintf(stderr, "[refine] pass %d: %d iterations (f32 recursive residual %.3e of %.3e), "
+                "f64 residual %.3e of max|b| %.3e -> %.3e\n", outer, r.iters, r.residual, inner,
+                rnorm, bnorm, rnorm / bnorm);
       if (!(rnorm == rnorm)) {
         tot.nan_seen = 1;
         break;
