@@ -246,3 +246,79 @@ def test_cavity_dims_match_the_workload():
     # box_solver: bounds (-0.016 + 15*0.002) - (-0.016) = 0.030, ceil(0.030/0.002) = 15
     s2 = scenes.make_solver(scenes.box_scene(), lib=oracle.load())
     assert (s2.nx, s2.ny, s2.nz) == (15, 15, 15)
+
+
+# ---------------------------------------------------------------- G2P / advect closed forms
+def _box_oracle(orc, flip_ratio):
+    sc = scenes.box_scene()
+    sc.fluid.flip_ratio = flip_ratio
+    return scenes.make_solver(sc, lib=orc)
+
+
+def _interior_points(s, n, seed):
+    """Points at least 2 cells inside the 12^3-cell cavity of the box scene."""
+    g = np.random.default_rng(seed)
+    lo = np.asarray(s.origin) + 4.0 * s.dx
+    return lo + g.random((n, 3)) * (7.0 * s.dx)
+
+
+def _linear_faces(s, a):
+    """Face velocities of the linear field u = a0 + a1 x + a2 y + a3 z (per component),
+    sampled at the face positions of the MAC grid (grid.rs:128-175)."""
+    o, dx = np.asarray(s.origin), s.dx
+    out = []
+    for comp, shape in enumerate(s.face_shapes()):
+        kk, jj, ii = np.meshgrid(*(np.arange(m, dtype=np.float64) for m in shape), indexing="ij")
+        pos = [ii, jj, kk]
+        x = [o[d] + (pos[d] + (0.0 if d == comp else 0.5)) * dx for d in range(3)]
+        c = a[comp]
+        out.append(c[0] + c[1] * x[0] + c[2] * x[1] + c[3] * x[2])
+    return out
+
+
+def test_g2p_pure_flip_on_an_unchanged_grid_keeps_velocities(orc):
+    """solver.rs:650-677 with flip = 1 and grid == saved grid: every interpolated change
+    is exactly 0, so flip_v = v and v*1 + pic*0 = v, bit for bit."""
+    s = _box_oracle(orc, 1.0)
+    g = np.random.default_rng(7)
+    pos = _interior_points(s, 500, 8)
+    vel = g.standard_normal(pos.shape)
+    s.set_particles(pos, vel)
+    s.set_grid(*[g.standard_normal(sh) for sh in s.face_shapes()])
+    s.run_stage("save")
+    s.run_stage("g2p")
+    assert bits_equal(s.particles()[1], vel)
+
+
+def test_g2p_pure_pic_interpolates_a_linear_field(orc):
+    """flip = 0: the particle takes the PIC sample of the grid, which for a linear face
+    field is the field at the particle (trilinear interpolation is exact on linears)."""
+    s = _box_oracle(orc, 0.0)
+    a = [(0.3, 2.0, -1.0, 0.5), (-0.2, 0.7, 1.5, -2.5), (1.1, -0.4, 0.9, 3.0)]
+    pos = _interior_points(s, 500, 9)
+    s.set_particles(pos, np.full(pos.shape, 9.0))
+    s.set_grid(*_linear_faces(s, a))
+    s.run_stage("save")
+    s.run_stage("g2p")
+    v = s.particles()[1]
+    for comp in range(3):
+        c = a[comp]
+        want = c[0] + c[1] * pos[:, 0] + c[2] * pos[:, 1] + c[3] * pos[:, 2]
+        assert np.allclose(v[:, comp], want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("speed_cells", [0.3, 3.0])
+def test_advect_uniform_flow_is_a_clamped_translation(orc, speed_cells):
+    """solver.rs:693-711: in a uniform field both RK2 samples equal the field, so the
+    step is U*dt, clamped to length dx when longer; interior points need no projection."""
+    s = _box_oracle(orc, 0.95)
+    dt = 1e-3
+    U = np.array([0.6, -0.8, 0.0]) * speed_cells * s.dx / dt
+    pos = _interior_points(s, 300, 10)
+    s.set_particles(pos, np.zeros_like(pos))
+    s.set_grid(*[np.full(sh, U[c]) for c, sh in enumerate(s.face_shapes())])
+    s.run_stage("advect", dt)
+    step = U * dt
+    if np.linalg.norm(step) > s.dx:
+        step = step * (s.dx / np.linalg.norm(step))
+    assert np.allclose(s.particles()[0], pos + step, rtol=0, atol=1e-12 * s.dx / 1e-3)
