@@ -946,50 +946,6 @@ __global__ void k_redist_decide(const uint32_t* __restrict__ act, int64_t nact,
   *moved = nmoved;
 }
 
-// ---- parallel decision pass, bit-identical to the sequential one above.
-// During a sweep a cell over the cap (a source) only loses particles and only down to
-// the cap, a cell under it (a receiver) only gains and only up to the cap, and a cell at
-// the cap never changes.  So a source's particle reads only its own count and its
-// receiver neighbours' counts, and only the first count - cap particles of a source (in
-// reference order) can move: once one finds no receiver with room, none of the later
-// ones will (rooms only shrink).  Sources therefore interact only through shared
-// receivers, and the reference order is kept by deterministic reservations: each round,
-// the head particle of every pending source reserves its open receiver neighbours with
-// (round, its reference position) by a 64-bit atomic max, and a head that won all of
-// its reservations has no earlier pending particle touching those counts, so it sees
-// exactly the counts the sequential loop would and commits.  The lowest pending head
-// always commits, so every round makes progress.
-__device__ inline int64_t redist_choose(int64_t c, const uint32_t* count, const uint8_t* mask, int nx,
-                                        int ny, int nz, uint32_t cap, const RedistDirs& dn) {
-  const int64_t sy = nx, sz = (int64_t)nx * ny;
-  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
-  int64_t nb[6];
-  double dd[6];
-  int nn = 0;
-  if (i > 0) { nb[nn] = c - 1; dd[nn++] = dn.d[0]; }
-  if (i + 1 < nx) { nb[nn] = c + 1; dd[nn++] = dn.d[1]; }
-  if (j > 0) { nb[nn] = c - sy; dd[nn++] = dn.d[2]; }
-  if (j + 1 < ny) { nb[nn] = c + sy; dd[nn++] = dn.d[3]; }
-  if (k > 0) { nb[nn] = c - sz; dd[nn++] = dn.d[4]; }
-  if (k + 1 < nz) { nb[nn] = c + sz; dd[nn++] = dn.d[5]; }
-  bool have = false;
-  int64_t best = -1;
-  double best_down = -INFINITY;
-  uint32_t best_room = 0;
-  for (int q = 0; q < nn; ++q) {
-    const int64_t ncl = nb[q];
-    if (!mask[ncl] || count[ncl] >= cap) continue;
-    const uint32_t room = cap - count[ncl];
-    if (!have || dd[q] > best_down || (dd[q] == best_down && room > best_room)) {
-      have = true;
-      best = ncl;
-      best_down = dd[q];
-      best_room = room;
-    }
-  }
-  return best;
-}
-
 // One atomic per wavefront for a per-lane flag (a single global counter hit by every
 // lane serialises the whole grid on one address).
 __device__ inline void wave_count(uint32_t* ctr, bool flag) {
@@ -1012,113 +968,6 @@ __device__ inline uint32_t wave_append(uint32_t* ctr, bool flag) {
   if (m && lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
   base = __shfl(base, leader);
   return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
-// Sources from the active particles sorted by (cell, reference position): a segment of
-// count[c] entries starts where the cell changes; its first count[c] - cap entries are
-// the candidates.
-__global__ void k_rd_sources(const uint32_t* __restrict__ scell, int64_t nact,
-                             const uint32_t* __restrict__ count, uint32_t cap, uint32_t* src_cell,
-                             uint32_t* src_head, uint32_t* src_lim, uint32_t* nsrc) {
-  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t c = a < nact ? scell[a] : 0u;
-  const bool start = a < nact && (a == 0 || scell[a - 1] != c);
-  const uint32_t q = wave_append(nsrc, start);
-  if (!start) return;
-  src_cell[q] = c;
-  src_head[q] = (uint32_t)a;
-  src_lim[q] = (uint32_t)a + (count[c] - cap);
-}
-
-// A head reserves every open receiver it could still ever write (rooms only shrink, so
-// later particles of its cell touch a subset), but its own decision reads only the open
-// receivers of the best gravity class: a receiver of a higher class is closed for good,
-// and one of a lower class loses to any open member of the best class.  It commits when
-// it holds that class (src_rmask), even if an earlier head holds one of its other
-// receivers, because that head's particles never touch the class it holds.
-__global__ void k_rd_reserve(const uint32_t* __restrict__ nsrc, const uint32_t* __restrict__ src_cell,
-                             uint32_t* src_head, const uint32_t* __restrict__ src_lim,
-                             uint8_t* src_rmask, const uint32_t* __restrict__ spos,
-                             const uint32_t* __restrict__ count, const uint8_t* __restrict__ mask,
-                             int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
-                             unsigned long long* resv, uint32_t epoch, uint32_t* pending) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) *pending = 0;
-  if (s >= *nsrc) return;
-  const uint32_t head = src_head[s];
-  if (head >= src_lim[s]) return;
-  const int64_t c = src_cell[s];
-  const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - spos[head]);
-  const int64_t sy = nx, sz = (int64_t)nx * ny;
-  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
-  const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-  const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
-  uint8_t rm = 0;
-  double best = -INFINITY;
-  for (int q = 0; q < 6; ++q) {
-    if (!in[q] || !mask[nb[q]] || count[nb[q]] >= cap) continue;
-    rm |= (uint8_t)(1u << q);
-    best = fmax(best, dn.d[q]);
-    atomicMax(resv + nb[q], val);
-  }
-  uint8_t fm = 0;
-  for (int q = 0; q < 6; ++q)
-    if (((rm >> q) & 1u) && dn.d[q] == best) fm |= (uint8_t)(1u << q);
-  src_rmask[s] = fm;
-  if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
-}
-
-__global__ void k_rd_commit(const uint32_t* __restrict__ nsrc, const uint32_t* __restrict__ src_cell,
-                            uint32_t* src_head, const uint32_t* __restrict__ src_lim,
-                            const uint8_t* __restrict__ src_rmask, const uint32_t* __restrict__ spos,
-                            uint32_t* count, const uint8_t* __restrict__ mask, int nx, int ny, int nz,
-                            uint32_t cap, RedistDirs dn, const unsigned long long* __restrict__ resv,
-                            uint32_t epoch, int32_t* dest, uint32_t* pending,
-                            unsigned long long* moved) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool pend = false, mv = false;
-  if (s < *nsrc) {
-    const uint32_t head = src_head[s], lim = src_lim[s];
-    if (head < lim) {
-      const int64_t c = src_cell[s];
-      const uint32_t a = spos[head];
-      const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - a);
-      const int64_t sy = nx, sz = (int64_t)nx * ny;
-      const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-      const uint8_t rm = src_rmask[s];
-      bool held = true;
-      for (int q = 0; q < 6; ++q)
-        if (((rm >> q) & 1u) && resv[nb[q]] != val) held = false;
-      if (!held) {
-        pend = true;  // an earlier particle holds a receiver of the class
-      } else {
-        const int64_t best = redist_choose(c, count, mask, nx, ny, nz, cap, dn);
-        if (best < 0) {
-          src_head[s] = lim;
-        } else {
-          count[c] -= 1;
-          count[best] += 1;
-          dest[a] = (int32_t)best;
-          mv = true;
-          src_head[s] = head + 1;
-          pend = head + 1 < lim;
-        }
-      }
-    }
-  }
-  wave_count(pending, pend);
-  wave_count64(moved, mv);
-}
-
-__global__ void k_iota_u32(uint32_t* out, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[t] = (uint32_t)t;
-}
-
-__global__ void k_gather_cells(const uint32_t* __restrict__ act, const int32_t* __restrict__ cellkey,
-                               int64_t nact, int32_t* act_cell) {
-  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < nact) act_cell[a] = cellkey[act[a]];
 }
 
 template <class T>
@@ -1457,35 +1306,6 @@ void launch_redist_decide(const uint32_t* act, int64_t nact, const int32_t* cell
   if (nact <= 0) return;
   k_redist_decide<<<1, 1, 0, st>>>(act, nact, cellkey, count, mask, nx, ny, nz, cap, gx, gy, gz,
                                    dest, moved);
-}
-void launch_rd_sources(const uint32_t* scell, int64_t nact, const uint32_t* count, uint32_t cap,
-                       uint32_t* src_cell, uint32_t* src_head, uint32_t* src_lim, uint32_t* nsrc,
-                       hipStream_t st) {
-  if (nact <= 0) return;
-  k_rd_sources<<<grid_for(nact), kBlock, 0, st>>>(scell, nact, count, cap, src_cell, src_head, src_lim,
-                                                  nsrc);
-}
-void launch_rd_round(int64_t max_src, const uint32_t* nsrc, const uint32_t* src_cell, uint32_t* src_head,
-                     const uint32_t* src_lim, uint8_t* src_rmask, const uint32_t* spos, uint32_t* count,
-                     const uint8_t* mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
-                     unsigned long long* resv, uint32_t epoch, int32_t* dest, uint32_t* pending,
-                     unsigned long long* moved, hipStream_t st) {
-  if (max_src <= 0) return;
-  k_rd_reserve<<<grid_for(max_src), kBlock, 0, st>>>(nsrc, src_cell, src_head, src_lim, src_rmask, spos,
-                                                     count, mask, nx, ny, nz, cap, dn, resv, epoch,
-                                                     pending);
-  k_rd_commit<<<grid_for(max_src), kBlock, 0, st>>>(nsrc, src_cell, src_head, src_lim, src_rmask, spos,
-                                                    count, mask, nx, ny, nz, cap, dn, resv, epoch, dest,
-                                                    pending, moved);
-}
-void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t st) {
-  if (n <= 0) return;
-  k_iota_u32<<<grid_for(n), kBlock, 0, st>>>(out, n);
-}
-void launch_gather_cells(const uint32_t* act, const int32_t* cellkey, int64_t nact, int32_t* act_cell,
-                         hipStream_t st) {
-  if (nact <= 0) return;
-  k_gather_cells<<<grid_for(nact), kBlock, 0, st>>>(act, cellkey, nact, act_cell);
 }
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act, const int32_t* dest, int64_t nact,

@@ -150,19 +150,23 @@ void launch_redist_decide(const uint32_t* act_slot_sorted, int64_t nact, const i
 struct RedistDirs {
   double d[6];
 };
-void launch_rd_sources(const uint32_t* scell, int64_t nact, const uint32_t* count, uint32_t cap,
-                       uint32_t* src_cell, uint32_t* src_head, uint32_t* src_lim, uint32_t* nsrc,
+// Bucket-based decision pass (redist.hip).  ctr (8 words): [0] sources, [1] bucket
+// entries, [2]/[3] pending-list lengths, [4] large buckets, [6..7] moves (u64).
+void launch_rd_prepare(const uint32_t* count, const uint32_t* sidx, const uint32_t* boff, int64_t nc,
+                       uint32_t cap, uint32_t* src_cell, uint32_t* src_off, uint32_t* src_head,
+                       uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend, uint32_t* ctr,
                        hipStream_t st);
-// One reservation round over up to max_src sources (the device count is *nsrc); after
-// it, *pending holds the number of sources with candidates left.
-void launch_rd_round(int64_t max_src, const uint32_t* nsrc, const uint32_t* src_cell, uint32_t* src_head,
-                     const uint32_t* src_lim, uint8_t* src_rmask, const uint32_t* spos, uint32_t* count,
-                     const uint8_t* mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
-                     unsigned long long* resv, uint32_t epoch, int32_t* dest, uint32_t* pending,
-                     unsigned long long* moved, hipStream_t st);
-void launch_iota_u32(uint32_t* out, int64_t n, hipStream_t st);
-void launch_gather_cells(const uint32_t* act, const int32_t* cellkey, int64_t nact, int32_t* act_cell,
-                         hipStream_t st);
+void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n, uint32_t cap,
+                       const uint32_t* sidx, int64_t nsrc, const uint32_t* src_cell,
+                       const uint32_t* src_off, uint32_t* src_fill, const uint32_t* rank,
+                       uint32_t* brank, uint32_t* bslot, uint32_t* trank, uint32_t* tslot,
+                       uint32_t* big, uint32_t* ctr, hipStream_t st);
+// One reservation round; parity selects which pending list is read (0: a, 1: b).
+void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
+                      const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
+                      uint8_t* src_rmask, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+                      int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
+                      uint32_t epoch, int32_t* dest, uint32_t* ctr, hipStream_t st);
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act_slot_sorted, const int32_t* dest,
                          int64_t nact, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st);

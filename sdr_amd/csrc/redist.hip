@@ -1,0 +1,361 @@
+// redist.hip — the order-dependent decision pass of redistribute (solver.rs:454-557)
+// as deterministic reservation rounds over cell buckets.
+//
+// During a sweep a cell over the cap (a source) only loses particles and only down to
+// the cap, a cell under it (a receiver) only gains and only up to the cap, and a cell at
+// the cap never changes.  So a source's particle reads only its own count and its
+// receiver neighbours' counts, and only the first count - cap particles of a source in
+// reference order can move (once one finds no receiver with room, none of the later
+// ones will: rooms only shrink).  Sources interact only through shared receivers.
+//
+// Layout of one sweep:
+//   * sources are numbered by an exclusive scan over the cells (count > cap), and each
+//     gets a bucket of count entries at an offset from a second scan (a counting sort:
+//     no radix sort of the particles);
+//   * every particle of a source cell lands in its bucket; the bucket's first
+//     count - cap entries are then put in reference (rank) order by a partial
+//     selection, one thread per source (one wave for large clumps);
+//   * rounds: the head particle of every pending source reserves its open receivers
+//     with (round, reference rank) by a 64-bit atomic max; a head that holds every
+//     receiver of its best gravity class sees exactly the counts the sequential loop
+//     would and commits.  The lowest pending head always commits, so every round makes
+//     progress.  Pending sources are appended to the next round's list with one atomic
+//     per block, so a round costs in proportion to what is still pending.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "kernels.hpp"
+
+namespace sdr {
+namespace {
+
+constexpr int kRdBlock = 256;
+constexpr uint32_t kBig = 64;  // buckets above this size are ordered by a whole wave
+
+inline unsigned rd_grid(int64_t n, int block = kRdBlock) {
+  int64_t b = (n + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// Slot of this thread in a list appended to by every thread with `flag`; one atomic per
+// block.  Every thread of the block must call it (it synchronises the block).
+__device__ inline uint32_t block_append(uint32_t* ctr, bool flag) {
+  __shared__ uint32_t wbase[kRdBlock / 64];
+  __shared__ uint32_t bbase;
+  const unsigned long long m = __ballot(flag);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wbase[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kRdBlock / 64; ++w) {
+      const uint32_t c = wbase[w];
+      wbase[w] = tot;
+      tot += c;
+    }
+    bbase = tot ? atomicAdd(ctr, tot) : 0u;
+  }
+  __syncthreads();
+  const uint32_t r = bbase + wbase[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  __syncthreads();
+  return r;
+}
+
+// Sum of a per-thread count over the block, added to *ctr with one atomic.
+__device__ inline void block_add64(unsigned long long* ctr, uint32_t v) {
+  __shared__ uint32_t part[kRdBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kRdBlock / 64; ++w) t += part[w];
+    if (t) atomicAdd(ctr, t);
+  }
+}
+
+__device__ inline int64_t rd_choose(int64_t c, const uint32_t* count, const uint8_t* mask, int nx,
+                                    int ny, int nz, uint32_t cap, const RedistDirs& dn) {
+  // solver.rs:486-544: visit -x,+x,-y,+y,-z,+z; most downhill, then most room, then first
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+  const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+  const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+  bool have = false;
+  int64_t best = -1;
+  double best_down = -INFINITY;
+  uint32_t best_room = 0;
+  for (int q = 0; q < 6; ++q) {
+    if (!in[q]) continue;
+    const int64_t ncl = nb[q];
+    if (!mask[ncl] || count[ncl] >= cap) continue;
+    const uint32_t room = cap - count[ncl];
+    if (!have || dn.d[q] > best_down || (dn.d[q] == best_down && room > best_room)) {
+      have = true;
+      best = ncl;
+      best_down = dn.d[q];
+      best_room = room;
+    }
+  }
+  return best;
+}
+
+// ctr: [0] sources, [1] bucket entries, [2] pending A, [3] pending B, [4] big buckets,
+// [6..7] moves (u64)
+__global__ void k_rd_totals(const uint32_t* __restrict__ count, const uint32_t* __restrict__ sidx,
+                            const uint32_t* __restrict__ boff, int64_t nc, uint32_t cap, uint32_t* ctr) {
+  const uint32_t last = count[nc - 1];
+  const uint32_t ns = sidx[nc - 1] + (last > cap ? 1u : 0u);
+  ctr[0] = ns;
+  ctr[1] = boff[nc - 1] + (last > cap ? last : 0u);
+  ctr[2] = ns;
+  ctr[3] = 0;
+  ctr[4] = 0;
+  ctr[5] = 0;
+  *(unsigned long long*)(ctr + 6) = 0ull;
+}
+
+__global__ void k_rd_src_init(const uint32_t* __restrict__ count, const uint32_t* __restrict__ sidx,
+                              const uint32_t* __restrict__ boff, int64_t nc, uint32_t cap,
+                              uint32_t* src_cell, uint32_t* src_off, uint32_t* src_head,
+                              uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t cnt = count[c];
+  if (cnt <= cap) return;
+  const uint32_t s = sidx[c], off = boff[c];
+  src_cell[s] = (uint32_t)c;
+  src_off[s] = off;
+  src_head[s] = off;
+  src_lim[s] = off + (cnt - cap);
+  src_fill[s] = 0;
+  pend[s] = s;
+}
+
+// Every particle of a source cell takes a slot of its cell's bucket.  Particles are
+// close to cell order, so each run of equal cells in a wave takes its slots with one
+// atomic; the order inside a bucket is fixed afterwards by reference rank.
+__global__ void k_rd_bucket(const int32_t* __restrict__ cellkey, const uint32_t* __restrict__ count,
+                            int64_t n, uint32_t cap, const uint32_t* __restrict__ sidx,
+                            const uint32_t* __restrict__ src_off, uint32_t* src_fill,
+                            const uint32_t* __restrict__ rank, uint32_t* brank, uint32_t* bslot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int key = -1;
+  if (t < n) {
+    const int c = cellkey[t];
+    if (c >= 0 && count[c] > cap) key = c;
+  }
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int start = 63 - __clzll((long long)upto);
+  uint32_t base = 0;
+  uint32_t s = 0;
+  if (key >= 0) s = sidx[key];
+  if (head && key >= 0) {
+    const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    base = atomicAdd(&src_fill[s], (uint32_t)(end - lane));
+  }
+  base = __shfl(base, start);
+  if (key < 0) return;
+  const uint32_t pos = src_off[s] + base + (uint32_t)(lane - start);
+  brank[pos] = rank[t];
+  bslot[pos] = (uint32_t)t;
+}
+
+// The first count - cap entries of each bucket in reference order (partial selection;
+// the order of the rest does not matter).  Buckets over kBig go to a wave each.
+__global__ void k_rd_select(int64_t nsrc, const uint32_t* __restrict__ src_cell,
+                            const uint32_t* __restrict__ src_off, const uint32_t* __restrict__ count,
+                            uint32_t cap, uint32_t* brank, uint32_t* bslot, uint32_t* big,
+                            uint32_t* nbig) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsrc) return;
+  const uint32_t cnt = count[src_cell[s]];
+  if (cnt > kBig) {
+    big[atomicAdd(nbig, 1u)] = (uint32_t)s;
+    return;
+  }
+  const uint32_t off = src_off[s], m = cnt - cap;
+  for (uint32_t p = 0; p < m; ++p) {
+    uint32_t bi = off + p, br = brank[bi];
+    for (uint32_t q = off + p + 1; q < off + cnt; ++q) {
+      const uint32_t r = brank[q];
+      if (r < br) br = r, bi = q;
+    }
+    if (bi != off + p) {
+      const uint32_t r0 = brank[off + p], s0 = bslot[off + p];
+      brank[off + p] = br;
+      bslot[off + p] = bslot[bi];
+      brank[bi] = r0;
+      bslot[bi] = s0;
+    }
+  }
+}
+
+// Large buckets: one wave sorts the whole bucket by rank (each entry's position is the
+// number of smaller ranks), through a scratch copy.
+__global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__ nbig,
+                                                    const uint32_t* __restrict__ big,
+                                                    const uint32_t* __restrict__ src_cell,
+                                                    const uint32_t* __restrict__ src_off,
+                                                    const uint32_t* __restrict__ count, uint32_t* brank,
+                                                    uint32_t* bslot, uint32_t* trank, uint32_t* tslot) {
+  const uint32_t nb = *nbig;
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const uint32_t s = big[b], off = src_off[s], cnt = count[src_cell[s]];
+    for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
+      const uint32_t r = brank[off + e];
+      uint32_t p = 0;
+      for (uint32_t f = 0; f < cnt; ++f) p += brank[off + f] < r ? 1u : 0u;
+      trank[off + p] = r;
+      tslot[off + p] = bslot[off + e];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
+      brank[off + e] = trank[off + e];
+      bslot[off + e] = tslot[off + e];
+    }
+    __syncthreads();
+  }
+}
+
+// Reservation: a head reserves every open receiver it could still ever write (rooms only
+// shrink, so later particles of its cell touch a subset), but its own decision reads only
+// the open receivers of the best gravity class: a receiver of a higher class is closed
+// for good, and one of a lower class loses to any open member of the best class.
+__global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* nnext,
+    const uint32_t* __restrict__ src_cell, uint32_t* src_head, const uint32_t* __restrict__ src_lim,
+    uint8_t* src_rmask, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
+    const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
+    unsigned long long* resv, uint32_t epoch) {
+  const uint32_t nl = *nlist;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nnext = 0;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nl;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = list[x];
+    const uint32_t head = src_head[s];
+    if (head >= src_lim[s]) continue;
+    const int64_t c = src_cell[s];
+    const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+    const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+    const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+    uint8_t rm = 0;
+    double best = -INFINITY;
+    for (int q = 0; q < 6; ++q) {
+      if (!in[q] || !mask[nb[q]] || count[nb[q]] >= cap) continue;
+      rm |= (uint8_t)(1u << q);
+      best = fmax(best, dn.d[q]);
+      atomicMax(resv + nb[q], val);
+    }
+    uint8_t fm = 0;
+    for (int q = 0; q < 6; ++q)
+      if (((rm >> q) & 1u) && dn.d[q] == best) fm |= (uint8_t)(1u << q);
+    src_rmask[s] = fm;
+    if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
+  }
+}
+
+// Commit: a head that holds its best class moves (even if an earlier head holds one of
+// its other receivers: that head's particles never touch the class it holds).  Sources
+// still pending go to the next round's list.
+__global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* next,
+    uint32_t* nnext, const uint32_t* __restrict__ src_cell, uint32_t* src_head,
+    const uint32_t* __restrict__ src_lim, const uint8_t* __restrict__ src_rmask,
+    const uint32_t* __restrict__ brank, uint32_t* count, const uint8_t* __restrict__ mask, int nx,
+    int ny, int nz, uint32_t cap, RedistDirs dn, const unsigned long long* __restrict__ resv,
+    uint32_t epoch, int32_t* dest, unsigned long long* moved) {
+  const uint32_t nl = *nlist;
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  uint32_t nmoved = 0;
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < nl; b0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = b0 + threadIdx.x;
+    bool pend = false;
+    uint32_t s = 0;
+    if (x < nl) {
+      s = list[x];
+      const uint32_t head = src_head[s], lim = src_lim[s];
+      if (head < lim) {
+        const int64_t c = src_cell[s];
+        const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
+        const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+        const uint8_t rm = src_rmask[s];
+        bool held = true;
+        for (int q = 0; q < 6; ++q)
+          if (((rm >> q) & 1u) && resv[nb[q]] != val) held = false;
+        if (!held) {
+          pend = true;  // an earlier particle holds a receiver of the class
+        } else {
+          const int64_t best = rd_choose(c, count, mask, nx, ny, nz, cap, dn);
+          if (best < 0) {
+            src_head[s] = lim;
+          } else {
+            count[c] -= 1;
+            count[best] += 1;
+            dest[head] = (int32_t)best;
+            ++nmoved;
+            src_head[s] = head + 1;
+            pend = head + 1 < lim;
+          }
+        }
+      }
+    }
+    const uint32_t q = block_append(nnext, pend);
+    if (pend) next[q] = s;
+  }
+  block_add64(moved, nmoved);
+}
+
+}  // namespace
+
+void launch_rd_prepare(const uint32_t* count, const uint32_t* sidx, const uint32_t* boff, int64_t nc,
+                       uint32_t cap, uint32_t* src_cell, uint32_t* src_off, uint32_t* src_head,
+                       uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend, uint32_t* ctr,
+                       hipStream_t st) {
+  k_rd_totals<<<1, 1, 0, st>>>(count, sidx, boff, nc, cap, ctr);
+  k_rd_src_init<<<rd_grid(nc), kRdBlock, 0, st>>>(count, sidx, boff, nc, cap, src_cell, src_off, src_head,
+                                                  src_lim, src_fill, pend);
+}
+
+void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n, uint32_t cap,
+                       const uint32_t* sidx, int64_t nsrc, const uint32_t* src_cell,
+                       const uint32_t* src_off, uint32_t* src_fill, const uint32_t* rank,
+                       uint32_t* brank, uint32_t* bslot, uint32_t* trank, uint32_t* tslot,
+                       uint32_t* big, uint32_t* ctr, hipStream_t st) {
+  if (n > 0)
+    k_rd_bucket<<<rd_grid(n), kRdBlock, 0, st>>>(cellkey, count, n, cap, sidx, src_off, src_fill, rank,
+                                                 brank, bslot);
+  if (nsrc <= 0) return;
+  k_rd_select<<<rd_grid(nsrc), kRdBlock, 0, st>>>(nsrc, src_cell, src_off, count, cap, brank, bslot, big,
+                                                  ctr + 4);
+  k_rd_sort_big<<<256, 64, 0, st>>>(ctr + 4, big, src_cell, src_off, count, brank, bslot, trank, tslot);
+}
+
+void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
+                      const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
+                      uint8_t* src_rmask, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+                      int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
+                      uint32_t epoch, int32_t* dest, uint32_t* ctr, hipStream_t st) {
+  if (max_pending <= 0) return;
+  const uint32_t* in = parity ? pend_b : pend_a;
+  uint32_t* out = parity ? pend_a : pend_b;
+  uint32_t* nin = ctr + 2 + parity;
+  uint32_t* nout = ctr + 3 - parity;
+  int64_t blocks = (max_pending + kRdBlock - 1) / kRdBlock;
+  if (blocks > 4096) blocks = 4096;
+  k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, src_rmask,
+                                                       brank, count, mask, nx, ny, nz, cap, dn, resv,
+                                                       epoch);
+  k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_cell, src_head, src_lim,
+                                                      src_rmask, brank, count, mask, nx, ny, nz, cap, dn,
+                                                      resv, epoch, dest, (unsigned long long*)(ctr + 6));
+}
+
+}  // namespace sdr

@@ -495,6 +495,8 @@ struct Solver : SolverBase {
   Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
   Buf cellkey, act_rank, act_slot, act_rank_sorted, act_slot_sorted, dest;
   Buf counters, aos_pos, aos_vel, fill_cells, rd_resv, rd_ctr;
+  Buf rd_sidx, rd_boff, rd_src_cell, rd_src_off, rd_src_head, rd_src_lim, rd_src_fill, rd_pend_a, rd_pend_b,
+      rd_big, rd_rmask;
   PinnedBuf pin;
   Projector<T> proj;
 
@@ -602,7 +604,7 @@ struct Solver : SolverBase {
       keys64_alt.alloc((size_t)nc * 8);
     }
     size_t tb = std::max({sort_pairs_u32_temp(nc), sort_pairs_u64_temp(nc), sort_keys_u32_temp(nc),
-                          scan_u32_temp(nc)});
+                          scan_u32_temp(nc), scan_over_cap_temp(ncells())});
     temp.alloc(tb);
     cap = nc;
   }
@@ -788,64 +790,71 @@ struct Solver : SolverBase {
     sorted_valid = false;
   }
   // The order-dependent decision pass of redistribute (solver.rs:476-555) by
-  // deterministic reservation rounds (kernels.hip, k_rd_*): the active particles
-  // (reference order a = 0..nact-1) are regrouped by cell keeping that order, every
-  // source cell gets its candidate range, and rounds run until no source has candidates
-  // left.  Chosen destinations land in dest[a]; returns the number of moves.
-  uint64_t decide(int64_t nact, uint32_t capc, const RedistDirs& dn) {
+  // deterministic reservation rounds over cell buckets (redist.hip): sources (cells over
+  // the cap) are numbered and given buckets by two scans over the cells, every particle
+  // of a source lands in its bucket, the first count - cap entries of each bucket are put
+  // in reference order, and rounds run until no source has candidates left.  Chosen
+  // destinations land in dest[bucket entry]; returns (bucket entries, moves).
+  std::pair<int64_t, uint64_t> decide(uint32_t capc, const RedistDirs& dn) {
     const int64_t nc = ncells();
     if (rd_resv.bytes < (size_t)nc * 8) {
       rd_resv.alloc((size_t)nc * 8);
       HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
       rd_epoch = 0;
+      for (Buf* b : {&rd_sidx, &rd_boff, &rd_src_cell, &rd_src_off, &rd_src_head, &rd_src_lim, &rd_src_fill,
+                     &rd_pend_a, &rd_pend_b, &rd_big})
+        b->alloc((size_t)nc * 4);
+      rd_rmask.alloc((size_t)nc);
     }
     if (!rd_ctr.p) rd_ctr.alloc(64);
-    uint32_t* ctr = rd_ctr.as<uint32_t>();  // [0] sources, [1] pending
-    unsigned long long* moved = (unsigned long long*)(ctr + 4);
-    // cells of the active particles in reference order, then grouped by cell (stable)
-    launch_gather_cells(act_slot_sorted.as<uint32_t>(), cellkey.as<int32_t>(), nact,
-                        act_rank.as<int32_t>(), st);
-    launch_iota_u32(act_slot.as<uint32_t>(), nact, st);
-    HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), keys.as<uint32_t>(),
-                          act_slot.as<uint32_t>(), vals.as<uint32_t>(), nact, bits_for((uint64_t)nc), st));
-    HIPCHK(hipMemsetAsync(rd_ctr.p, 0, rd_ctr.bytes, st));
-    HIPCHK(hipMemsetAsync(dest.p, 0xff, (size_t)nact * 4, st));
-    uint32_t* src_cell = keys_alt.as<uint32_t>();
-    uint32_t* src_head = vals_alt.as<uint32_t>();
-    uint32_t* src_lim = cullflag.as<uint32_t>();
-    uint8_t* src_rmask = culled_ranks.as<uint8_t>();
-    launch_rd_sources(keys.as<uint32_t>(), nact, count.as<uint32_t>(), capc, src_cell, src_head, src_lim,
-                      ctr, st);
+    uint32_t* ctr = rd_ctr.as<uint32_t>();
+    const uint32_t* cnt = count.as<uint32_t>();
+    HIPCHK(exclusive_sum_over_cap(temp.p, temp.bytes, cnt, capc, false, rd_sidx.as<uint32_t>(), nc, st));
+    HIPCHK(exclusive_sum_over_cap(temp.p, temp.bytes, cnt, capc, true, rd_boff.as<uint32_t>(), nc, st));
+    launch_rd_prepare(cnt, rd_sidx.as<uint32_t>(), rd_boff.as<uint32_t>(), nc, capc,
+                      rd_src_cell.as<uint32_t>(), rd_src_off.as<uint32_t>(), rd_src_head.as<uint32_t>(),
+                      rd_src_lim.as<uint32_t>(), rd_src_fill.as<uint32_t>(), rd_pend_a.as<uint32_t>(), ctr, st);
     uint32_t* hc = (uint32_t*)pin.p;
-    HIPCHK(hipMemcpyAsync(hc, ctr, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hc, ctr, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const int64_t nsrc = hc[0];
+    const int64_t nsrc = hc[0], nact = hc[1];
+    if (nsrc == 0) return {0, 0};
+    uint32_t* brank = act_rank.as<uint32_t>();
+    uint32_t* bslot = act_slot.as<uint32_t>();
+    launch_rd_buckets(cellkey.as<int32_t>(), cnt, n, capc, rd_sidx.as<uint32_t>(), nsrc,
+                      rd_src_cell.as<uint32_t>(), rd_src_off.as<uint32_t>(), rd_src_fill.as<uint32_t>(),
+                      part[cur][6].as<uint32_t>(), brank, bslot, act_rank_sorted.as<uint32_t>(),
+                      act_slot_sorted.as<uint32_t>(), rd_big.as<uint32_t>(), ctr, st);
+    HIPCHK(hipMemsetAsync(dest.p, 0xff, (size_t)nact * 4, st));
     // Each round commits at least the lowest pending candidate, so the candidates (at
-    // most nact) bound the rounds; the pending count is read every few rounds.
-    const int check_every = 4;
-    int64_t rounds = 0;
-    while (nsrc > 0) {
-      if (++rd_epoch == 0) {  // epochs wrapped: no stale reservation may outrank a new one
-        HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
-        rd_epoch = 1;
+    // most nact) bound the rounds; the pending count is read every few rounds and sizes
+    // the launches that follow.
+    const int check_every = 8;
+    int64_t rounds = 0, pending = nsrc;
+    while (pending > 0) {
+      for (int r = 0; r < check_every; ++r) {
+        if (++rd_epoch == 0) {  // epochs wrapped: no stale reservation may outrank a new one
+          HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
+          rd_epoch = 1;
+        }
+        launch_rd_round2(pending, (int)(rounds & 1), rd_pend_a.as<uint32_t>(), rd_pend_b.as<uint32_t>(),
+                         rd_src_cell.as<uint32_t>(), rd_src_head.as<uint32_t>(), rd_src_lim.as<uint32_t>(),
+                         rd_rmask.as<uint8_t>(), brank, count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz,
+                         capc, dn, rd_resv.as<unsigned long long>(), rd_epoch, dest.as<int32_t>(), ctr, st);
+        ++rounds;
       }
-      launch_rd_round(nsrc, ctr, src_cell, src_head, src_lim, src_rmask, vals.as<uint32_t>(),
-                      count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, dn,
-                      rd_resv.as<unsigned long long>(), rd_epoch, dest.as<int32_t>(), ctr + 1, moved, st);
-      ++rounds;
-      if (rounds % check_every == 0) {
-        HIPCHK(hipMemcpyAsync(hc, ctr + 1, 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        if (hc[0] == 0) break;
-        if (rounds > nact + check_every) throw Error(SDR_E_STATE, "redistribute: rounds did not converge");
-      }
+      // the list written by the last round
+      HIPCHK(hipMemcpyAsync(hc, ctr + 2 + (rounds & 1), 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      pending = hc[0];
+      if (rounds > nact + check_every) throw Error(SDR_E_STATE, "redistribute: rounds did not converge");
     }
     redist_rounds += (uint64_t)rounds;
-    HIPCHK(hipMemcpyAsync(hc + 2, moved, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hc + 2, ctr + 6, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     uint64_t m;
     memcpy(&m, hc + 2, 8);
-    return m;
+    return {nact, m};
   }
 
   // redistribute (solver.rs:454-557)
@@ -865,31 +874,37 @@ struct Solver : SolverBase {
     for (int sweep = 0; sweep < 4; ++sweep) {
       HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
       launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
+      if (!redist_sequential) {
+        auto [nact, moved] = decide(capc, dn);
+        if (nact == 0) break;  // no cell over the cap: later sweeps see the same state
+        redist_active += (uint64_t)nact;
+        redist_moved += moved;
+        launch_redist_apply<T>(parts(cur), act_slot.as<uint32_t>(), dest.as<int32_t>(), nact, grid(),
+                               cfaces(), sdf(), st);
+        continue;
+      }
+      // The single-thread reference loop over the active particles in reference order.
       launch_active_flags(cellkey.as<int32_t>(), count.as<uint32_t>(), n, capc, keep.as<uint32_t>(), st);
       HIPCHK(exclusive_sum_u32(temp.p, temp.bytes, keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, st));
       HIPCHK(hipMemcpyAsync(hn, scanpos.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(hn + 1, keep.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       int64_t nact = (int64_t)hn[0] + hn[1];
-      if (nact == 0) break;  // no cell over the cap: later sweeps see the same state
+      if (nact == 0) break;
       launch_active_compact(keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, part[cur][6].as<uint32_t>(),
                             act_rank.as<uint32_t>(), act_slot.as<uint32_t>(), st);
       HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(),
                             act_slot.as<uint32_t>(), act_slot_sorted.as<uint32_t>(), nact, 32, st));
       redist_active += (uint64_t)nact;
-      if (redist_sequential) {
-        unsigned long long* mv = counters.as<unsigned long long>() + 3;
-        launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(),
-                             count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x,
-                             gdir.y, gdir.z, dest.as<int32_t>(), mv, st);
-        HIPCHK(hipMemcpyAsync((unsigned long long*)pin.p + 4, mv, 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        redist_moved += ((unsigned long long*)pin.p)[4];
-      } else {
-        redist_moved += decide(nact, capc, dn);
-      }
-      launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact,
-                             grid(), cfaces(), sdf(), st);
+      unsigned long long* mv = counters.as<unsigned long long>() + 3;
+      launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(), count.as<uint32_t>(),
+                           mask.as<uint8_t>(), nx, ny, nz, capc, gdir.x, gdir.y, gdir.z, dest.as<int32_t>(), mv,
+                           st);
+      HIPCHK(hipMemcpyAsync((unsigned long long*)pin.p + 4, mv, 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      redist_moved += ((unsigned long long*)pin.p)[4];
+      launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact, grid(),
+                             cfaces(), sdf(), st);
     }
     sorted_valid = false;
   }

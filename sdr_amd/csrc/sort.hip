@@ -10,6 +10,15 @@
 
 namespace sdr {
 
+namespace {
+struct OverCap {
+  uint32_t cap;
+  bool sizes;
+  __host__ __device__ uint32_t operator()(uint32_t c) const { return c > cap ? (sizes ? c : 1u) : 0u; }
+};
+using OverCapIt = hipcub::TransformInputIterator<uint32_t, OverCap, const uint32_t*>;
+}  // namespace
+
 size_t sort_pairs_u32_temp(int64_t n) {
   size_t bytes = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -26,6 +35,12 @@ size_t scan_u32_temp(int64_t n) {
   size_t bytes = 0;
   (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                    (int)n);
+  return bytes;
+}
+size_t scan_over_cap_temp(int64_t n) {
+  size_t bytes = 0;
+  OverCapIt it((const uint32_t*)nullptr, OverCap{0, false});
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, (uint32_t*)nullptr, (int)n);
   return bytes;
 }
 size_t sort_keys_u32_temp(int64_t n) {
@@ -58,6 +73,13 @@ hipError_t exclusive_sum_u32(void* temp, size_t temp_bytes, const uint32_t* in, 
                              int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int)n, st);
+}
+
+hipError_t exclusive_sum_over_cap(void* temp, size_t temp_bytes, const uint32_t* count, uint32_t cap,
+                                  bool sizes, uint32_t* out, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  OverCapIt it(count, OverCap{cap, sizes});
+  return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, it, out, (int)n, st);
 }
 
 }  // namespace sdr
