@@ -5,6 +5,7 @@
 // bitwise reproducible; integer atomics (histograms, counters, max on non-negative
 // bit patterns) are order-independent.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include "kernels.hpp"
 
@@ -1123,6 +1124,10 @@ void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
     if (comp == 1) k_p2g<T, 1, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 2) k_p2g<T, 2, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
   } else {
+    // SDR_P2G_PERFACE=1 selects the face-by-face gather (same sums, same order)
+    const char* pf = getenv("SDR_P2G_PERFACE");
+    const bool perface = pf && atoi(pf) != 0;
+    if (!perface && launch_p2g_tiled<T>(comp, g, p, cstart, cend, out, st)) return;
     if (comp == 0) k_p2g<T, 0, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 1) k_p2g<T, 1, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 2) k_p2g<T, 2, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);

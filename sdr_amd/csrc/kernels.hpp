@@ -63,6 +63,10 @@ void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const ui
 template <class T>
 void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
                 const uint32_t* cend, T* out, bool exact, hipStream_t st);
+// P2G as an LDS-staged gather (p2g.hip); false when the grid is too large for it.
+template <class T>
+bool launch_p2g_tiled(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend,
+                      T* out, hipStream_t st);
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st);
 template <class T>

@@ -631,6 +631,523 @@ __global__ void __launch_bounds__(64 * TI * TK)
   }
 }
 
+// ------------------------------------------------------------------ multi-plane solve sweeps
+// As k_mic_solve, but a thread owns KT consecutive planes of its x-pencil: at step s its
+// plane t (of the wave's KT) is at row J - t, so the z neighbour of plane t > 0 is the
+// thread's own plane t - 1 from the previous step (a register), and only the wave's
+// first plane (last, backward) takes its z neighbour from another wave (LDS) or tile
+// (halo words).  A workgroup is TK such waves, one per SIMD, and tiles are TK * KT
+// planes deep, so a tile needs few waves for many planes and few hops cross tiles.
+// Tiles can be placed by XCD (`xcd`: the k-tiles of XCD x are a contiguous range, with
+// every i-tile, assuming workgroup b runs on XCD b % xcd), so that neighbouring tiles
+// share an XCD.  Halo words are read at agent scope either way: a narrower scope may be
+// served by a stale vector-L1 line.
+template <class T, int TK, int KT>
+struct Solve2Depth {
+  static constexpr int n = (sizeof(T) == 8 && KT >= 8) ? 4 : 8;
+};
+
+// Logical tile (forward dependency order) of workgroup b.
+__device__ inline void tile_of(int b, int nti, int ntk, int xcd, int* itl, int* ktl) {
+  if (xcd > 0) {
+    const int x = b % xcd, q = b / xcd, per = ntk / xcd;
+    *itl = q % nti;
+    *ktl = x * per + q / nti;
+  } else {
+    *itl = b % nti;
+    *ktl = b / nti;
+  }
+}
+
+template <class T, bool REV, int TK, int KT>
+__global__ void __launch_bounds__(64 * TK)
+    k_mic_solve2(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
+                 const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
+                 MicPipeArgs<T> pa) {
+  if (sc->done) return;
+  constexpr int W = Words<T>::n;
+  constexpr uint32_t SZ = sizeof(T);
+  constexpr int PF = Solve2Depth<T, TK, KT>::n;
+  constexpr int P = TK * KT;  // planes per tile
+  static_assert(PF % 2 == 0, "the LDS parity of a step is its ring slot's parity");
+  static_assert(P <= 64, "one fetch lane per plane of the tile");
+  __shared__ T zbuf[2][TK][64];
+  __shared__ T xlds[2][P];
+
+  const int lane = threadIdx.x & 63;
+  const int wk = threadIdx.x >> 6;
+  const int nti = pa.ntile_i, ntk = pa.ntile_k;
+  int itl, ktl;
+  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
+  const int it = REV ? nti - 1 - itl : itl;
+  const int kt = REV ? ntk - 1 - ktl : ktl;
+  const int i = it * 64 + lane;
+  const int k0 = kt * P + wk * KT;  // plane of this wave's first pencil
+  const int NJ = sh.NJ, nx = sh.nx, ny = sh.ny, nz = sh.nz;
+  const int nxp = nti * 64;
+  const uint32_t ncell = (uint32_t)sh.size();
+  const rsrc_t rf = make_rsrc(flags, ncell * 2u), rp = make_rsrc(precon, ncell * SZ),
+               ri = make_rsrc(in, ncell * SZ), ro = make_rsrc(out, ncell * SZ),
+               rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
+  const uint32_t tag = pa.tag;
+  const int dJ = REV ? -1 : 1;
+  // the plane that takes its z neighbour from outside the wave, and the one that gives
+  const int tzin = REV ? KT - 1 : 0, tzout = REV ? 0 : KT - 1;
+  const bool z_lds = REV ? wk + 1 < TK : wk > 0;
+  const bool z_in = !z_lds && (REV ? kt + 1 < ntk : kt > 0);
+  const bool x_tile_in = REV ? it + 1 < nti : it > 0;
+  // One wave fetches the x halo words of every plane of the tile (lane l: plane l) a
+  // step ahead and hands them over through LDS.
+  const bool x_fetch = x_tile_in && wk == (REV ? 0 : TK - 1);
+  const int kl = kt * P + lane;
+  const bool x_lane = x_fetch && lane < P && kl < nz;
+  const bool z_out = (REV ? wk == 0 : wk == TK - 1) && (REV ? kt > 0 : kt + 1 < ntk);
+  const bool x_out = REV ? it > 0 : it + 1 < nti;
+  const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
+  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
+  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
+  const int zsrc = REV ? wk + 1 : wk - 1;
+  const int zrow = W * nxp;
+  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
+  const int xl_in = (xb_in * nz + kl) * NJ * W;
+  int base[KT], xh_out[KT];
+  bool tok[KT];
+#pragma unroll
+  for (int t = 0; t < KT; ++t) {
+    const int k = k0 + t;
+    tok[t] = i < nx && k < nz;
+    base[t] = k * NJ * nx + i;
+    xh_out[t] = x_out && lane == edge_out && k < nz ? (xb_out * nz + k) * NJ * W : -1;
+  }
+
+  const int nsteps = NJ + P - 1;
+  const int nsteps_pad = (nsteps + PF - 1) / PF * PF;
+  // row of plane 0 of this wave at step 0 (plane t is at row J - t)
+  const int J0 = REV ? (NJ - 1) + (P - 1 - wk * KT) : -(wk * KT);
+  const int JX0 = REV ? (NJ - 1) + (P - 1 - lane) : -lane;
+
+  uint32_t cf[PF][KT], co[PF][KT];
+  T cp[PF][KT], ci[PF][KT];
+  uint64_t hz[PF][W], hx[PF][W];
+  auto refill = [&](int u, int J, int JX) {
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+      const int R = J - t;
+      const bool v = tok[t] && (unsigned)(R - i) < (unsigned)ny;
+      const uint32_t c = (uint32_t)(base[t] + R * nx);
+      const uint32_t ot = v ? c * SZ : kOob;
+      co[u][t] = ot;
+      cf[u][t] = __builtin_amdgcn_raw_buffer_load_b16(rf, v ? c * 2u : kOob, 0, 0);
+      cp[u][t] = bld<T>(rp, ot);
+      ci[u][t] = bld<T>(ri, ot);
+    }
+    if (z_in) {
+      const int R = J - tzin;
+      const uint32_t o = (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
+#pragma unroll
+      for (int q = 0; q < W; ++q) hz[u][q] = bld64<kAgent>(rhz, o + (uint32_t)(q * nxp) * 8u);
+    }
+    if (x_fetch) {
+      const uint32_t o = x_lane && (unsigned)JX < (unsigned)NJ ? (uint32_t)(xl_in + JX * W) * 8u : kOob;
+#pragma unroll
+      for (int q = 0; q < W; ++q) hx[u][q] = bld64<kAgent>(rhx, o + (uint32_t)q * 8u);
+    }
+  };
+  auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
+    long spins = 0;
+    while (!tags_ok<W>(w, tag)) {
+      if (++spins > pa.spin_limit ||
+          __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
+    }
+  };
+  // A stale word: wait for the window's newest word, then reload the window (rows are
+  // published in order), so later steps find fresh words instead of each paying one
+  // more round trip.
+  auto resync = [&](uint64_t(&ring)[PF][W], int u, int Jh, rsrc_t r, int hbase, int rowsz,
+                    uint32_t stride) {
+    int Jf = Jh + (PF - 1) * dJ;
+    Jf = Jf < 0 ? 0 : (Jf >= NJ ? NJ - 1 : Jf);
+    uint64_t w[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) w[q] = 0;  // tag 0 is never a sweep's tag
+    settle(w, r, (uint32_t)(hbase + Jf * rowsz) * 8u, stride);
+#pragma unroll
+    for (int v = 0; v < PF; ++v) {
+      const int R = Jh + v * dJ;
+      const uint32_t o = (unsigned)R < (unsigned)NJ ? (uint32_t)(hbase + R * rowsz) * 8u : kOob;
+#pragma unroll
+      for (int q = 0; q < W; ++q) ring[(u + v) % PF][q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
+    }
+    settle(ring[u], r, (uint32_t)(hbase + Jh * rowsz) * 8u, stride);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+  };
+
+  zbuf[1][wk][lane] = T(0);  // read by step 0
+  if (threadIdx.x < P) xlds[1][threadIdx.x] = T(0);
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+#pragma unroll
+    for (int q = 0; q < W; ++q) hz[u][q] = hx[u][q] = 0;
+    refill(u, J0 + u * dJ, JX0 + u * dJ);
+  }
+  __syncthreads();
+
+  T prev[KT];
+#pragma unroll
+  for (int t = 0; t < KT; ++t) prev[t] = T(0);
+  int J = J0, JX = JX0;
+  for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int cb = u & 1, pb = cb ^ 1;
+      // ---- z neighbour of the wave's boundary plane
+      T zb = z_lds ? zbuf[pb][zsrc][lane] : T(0);
+      const int rz = J - tzin;
+      if (z_in && (unsigned)rz < (unsigned)NJ) {
+        if (!tags_ok<W>(hz[u], tag)) resync(hz, u, rz, rhz, zh_in, zrow, (uint32_t)nxp * 8u);
+        zb = words_value<T>(hz[u]);
+      }
+      T pubn[KT];
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const T xs = wave_shift<REV>(prev[t]);
+        const T xe = x_tile_in ? xlds[pb][wk * KT + t] : T(0);
+        const T xin = lane == edge_in ? xe : xs;
+        const T zin = t == tzin ? zb : prev[REV ? (t + 1) % KT : (t + KT - 1) % KT];
+        const T yin = prev[t];
+        const uint32_t f = cf[u][t];
+        const T pcu = cp[u][t], inu = ci[u][t];
+        const bool live = (f & kFlFluid) != 0;
+        T res, pub;
+        if (!REV) {
+          // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
+          const T q = (((inu - xin) - yin) - zin) * pcu;
+          res = live ? q : T(0);
+          pub = live ? (-pcu) * q : T(0);
+        } else {
+          const T npc = -pcu;
+          T x = inu;
+          x = x - ((f & kFlPx) ? npc * xin : T(0));
+          x = x - ((f & kFlPy) ? npc * yin : T(0));
+          x = x - ((f & kFlPz) ? npc * zin : T(0));
+          const T z = x * pcu;
+          res = live ? z : T(0);
+          pub = res;
+        }
+        bst<T>(ro, co[u][t], res);
+        pubn[t] = pub;
+      }
+      zbuf[cb][wk][lane] = pubn[tzout];
+      // ---- halo publication
+      if (z_out) {
+        const int R = J - tzout;
+        if ((unsigned)R < (unsigned)NJ) {
+          uint32_t wv[W];
+          __builtin_memcpy(wv, &pubn[tzout], sizeof(T));
+          const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
+#pragma unroll
+          for (int q = 0; q < W; ++q)
+            bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
+        }
+      }
+      if (x_out) {
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          const int R = J - t;
+          uint32_t wv[W];
+          __builtin_memcpy(wv, &pubn[t], sizeof(T));
+          const uint32_t o = xh_out[t] >= 0 && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out[t] + R * W) * 8u : kOob;
+#pragma unroll
+          for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
+        }
+      }
+      if (x_fetch) {
+        T xv = T(0);
+        if (x_lane && (unsigned)JX < (unsigned)NJ) {
+          if (!tags_ok<W>(hx[u], tag)) resync(hx, u, JX, rhx, xl_in, W, 8u);
+          xv = words_value<T>(hx[u]);
+        }
+        if (lane < P) xlds[cb][lane] = xv;
+      }
+#pragma unroll
+      for (int t = 0; t < KT; ++t) prev[t] = pubn[t];
+      refill(u, J + PF * dJ, JX + PF * dJ);
+      J += dJ;
+      JX += dJ;
+      __syncthreads();
+    }
+  }
+}
+
+template <class T, int TK, int KT>
+void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
+                   const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
+  const int grid = pa.ntile_i * pa.ntile_k;
+  if (op == kMicFwd)
+    k_mic_solve2<T, false, TK, KT><<<grid, 64 * TK, 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else
+    k_mic_solve2<T, true, TK, KT><<<grid, 64 * TK, 0, st>>>(sh, flags, precon, in, out, sc, pa);
+}
+
+// ------------------------------------------------------------------ lean solve sweeps with an I/O wave
+// The solves again, with the per-step work of the compute waves cut to the cell itself:
+//   * TK compute waves (one plane each) plus one I/O wave per workgroup.  The I/O wave
+//     does all the cross-tile traffic: it fetches (a step ahead, from its prefetch ring)
+//     the z halo row of the tile's boundary plane and the x halo value of every plane,
+//     and publishes (a step late, from LDS) the tile's last plane and the edge lane of
+//     every plane.  Compute waves never wait on halo words and never issue
+//     write-through stores (which would hold back their own prefetch waits).
+//   * Cell loads and stores address a buffer descriptor per plane (the plane's sheared
+//     slots, NJ * nx values): a lane's offset is (J * nx + i) * size, so rows before the
+//     plane (J < 0) wrap to huge unsigned offsets and rows past it exceed the record
+//     count, and the hardware drops them; lanes with i >= nx start at an offset that is
+//     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) hold
+//     flags 0, so they compute and store 0.  No bounds tests remain in the step.
+template <class T, bool REV, int TK>
+__global__ void __launch_bounds__(64 * (TK + 1))
+    k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
+                 const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
+                 MicPipeArgs<T> pa) {
+  if (sc->done) return;
+  constexpr int W = Words<T>::n;
+  constexpr uint32_t SZ = sizeof(T);
+  constexpr int PF = 8;
+  // zbuf[.][w] for w < TK: compute wave w's values of a step; zbuf[.][TK]: the fetched z
+  // halo row (the boundary wave's z neighbour)
+  __shared__ T zbuf[2][TK + 1][64];
+  __shared__ T xlds[2][TK];
+
+  const int lane = threadIdx.x & 63;
+  // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool io = wave == TK;
+  const int nti = pa.ntile_i, ntk = pa.ntile_k;
+  int itl, ktl;
+  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
+  const int it = REV ? nti - 1 - itl : itl;
+  const int kt = REV ? ntk - 1 - ktl : ktl;
+  const int i = it * 64 + lane;
+  const int NJ = sh.NJ, nx = sh.nx, nz = sh.nz;
+  const int nxp = nti * 64;
+  const int dJ = REV ? -1 : 1;
+  const uint32_t tag = pa.tag;
+  // row of compute wave w at step s
+  auto row = [&](int w, int s) { return REV ? (NJ - 1) - s + (TK - 1 - w) : s - w; };
+  const int nsteps = NJ + TK - 1;  // compute steps; the I/O wave runs one more to publish
+  const int nsteps_pad = (nsteps + 1 + PF - 1) / PF * PF;
+  const int bwave = REV ? TK - 1 : 0;       // boundary wave: its z neighbour is in the tile before
+  const int lwave = REV ? 0 : TK - 1;       // last wave: the tile after takes its z neighbour from it
+  const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
+
+  if (!io) {
+    // ------------------------------------------------------------ compute wave
+    const int wk = wave;
+    const int k = kt * TK + wk;
+    const int plane = k < nz ? k : 0;
+    const uint32_t recs = k < nz ? (uint32_t)(NJ * nx) : 0u;
+    const T* const pbase = precon + (int64_t)plane * NJ * nx;
+    const rsrc_t rf = make_rsrc(flags + (int64_t)plane * NJ * nx, recs * 2u),
+                 rp = make_rsrc(pbase, recs * SZ),
+                 ri = make_rsrc(in + (int64_t)plane * NJ * nx, recs * SZ),
+                 ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ);
+    const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
+    // element offset of (row J, lane i); lanes past the grid start out of range for good
+    // (2^28 * size stays below 2^32 and above every plane's record count)
+    const uint32_t lbase = i < nx ? (uint32_t)i : 0x10000000u;
+    const int J0 = row(wk, 0);
+    uint32_t cur = lbase + (uint32_t)(J0 * nx);            // element offset of this step's row
+    uint32_t nxt = lbase + (uint32_t)((J0 + PF * dJ) * nx);  // of the row PF steps ahead
+    const uint32_t dstep = (uint32_t)(dJ * nx);
+    uint32_t cf[PF];
+    T cp[PF], ci[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const uint32_t e = lbase + (uint32_t)((J0 + u * dJ) * nx);
+      cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
+      cp[u] = bld<T>(rp, e * SZ);
+      ci[u] = bld<T>(ri, e * SZ);
+    }
+    zbuf[1][wk][lane] = T(0);
+    T prev = T(0);
+    __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
+    for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int cb = u & 1, pb = cb ^ 1;
+        const T zin = zbuf[pb][zsrc][lane];
+        const T xe = xlds[pb][wk];
+        const T xs = wave_shift<REV>(prev);
+        const T xin = lane == edge_in ? xe : xs;
+        const uint32_t f = cf[u];
+        const T pcu = cp[u], inu = ci[u];
+        const bool live = (f & kFlFluid) != 0;
+        T res, pub;
+        if (!REV) {
+          // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
+          const T q = (((inu - xin) - prev) - zin) * pcu;
+          res = live ? q : T(0);
+          pub = live ? (-pcu) * q : T(0);
+        } else {
+          const T npc = -pcu;
+          T x = inu;
+          x = x - ((f & kFlPx) ? npc * xin : T(0));
+          x = x - ((f & kFlPy) ? npc * prev : T(0));
+          x = x - ((f & kFlPz) ? npc * zin : T(0));
+          const T z = x * pcu;
+          res = live ? z : T(0);
+          pub = res;
+        }
+        bst<T>(ro, cur * SZ, res);
+        zbuf[cb][wk][lane] = pub;
+        prev = pub;
+        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, nxt * 2u, 0, 0);
+        cp[u] = bld<T>(rp, nxt * SZ);
+        ci[u] = bld<T>(ri, nxt * SZ);
+        cur += dstep;
+        nxt += dstep;
+        __syncthreads();
+      }
+    }
+    return;
+  }
+
+  // -------------------------------------------------------------- I/O wave
+  const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
+  const bool z_in = REV ? kt + 1 < ntk : kt > 0;
+  const bool z_out = REV ? kt > 0 : kt + 1 < ntk;
+  const bool x_in = REV ? it + 1 < nti : it > 0;
+  const bool x_out = REV ? it > 0 : it + 1 < nti;
+  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
+  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
+  const int zrow = W * nxp;
+  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
+  // lane l < TK serves plane kt * TK + l for the x halo
+  const int kl = kt * TK + lane;
+  const bool xl = lane < TK && kl < nz;
+  const int xh_in = (xb_in * nz + (xl ? kl : 0)) * NJ * W;
+  const int xh_out = (xb_out * nz + (xl ? kl : 0)) * NJ * W;
+  const int klast = kt * TK + lwave;
+  const bool zl_ok = klast < nz;
+  // Inputs of target step t: the z halo row of the boundary wave at step t, and the x
+  // neighbour row of plane `lane` at step t (its row at step t, minus dJ).
+  auto zrow_of = [&](int t) { return row(bwave, t); };
+  auto xrow_of = [&](int t) { return row(lane, t) - dJ; };
+  auto zoff = [&](int R) -> uint32_t {
+    return z_in && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
+  };
+  auto xoff = [&](int R) -> uint32_t {
+    return x_in && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_in + R * W) * 8u : kOob;
+  };
+  uint64_t hz[PF][W], hx[PF][W];
+  auto fill = [&](int u, int t) {
+    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
+      hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
+    }
+  };
+  // A word is ready when it carries this sweep's tag, or when it is not needed (its
+  // offset is out of range: the load returned 0, and the value used is 0).
+  auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
+    if (o == kOob) return;
+    long spins = 0;
+    while (!tags_ok<W>(w, tag)) {
+      if (++spins > pa.spin_limit ||
+          __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
+    }
+  };
+  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return o == kOob || tags_ok<W>(w, tag); };
+  // Take the inputs of target step t from ring slot u into LDS buffer b.  A stale word
+  // means the producer is behind the prefetch window: wait for the window's newest word
+  // (rows are published in order), then reload the whole window.
+  auto take = [&](int u, int t, int b) {
+    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+    const bool ok = ready(hz[u], oz) && ready(hx[u], ox);
+    if (__ballot(!ok) != 0ull) {
+      const uint32_t nz_ = zoff(zrow_of(t + PF - 1)), nx_ = xoff(xrow_of(t + PF - 1));
+      uint64_t wz[W], wx[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) wz[q] = wx[q] = 0;  // tag 0 is never a sweep's tag
+      settle(wz, rhz, nz_, (uint32_t)nxp * 8u);
+      settle(wx, rhx, nx_, 8u);
+#pragma unroll
+      for (int v = 0; v < PF; ++v) fill((u + v) % PF, t + v);
+      settle(hz[u], rhz, oz, (uint32_t)nxp * 8u);
+      settle(hx[u], rhx, ox, 8u);
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
+    if (lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
+  };
+  // Publish the values compute waves wrote at step t into LDS buffer b.
+  auto publish = [&](int t, int b) {
+    if (z_out && zl_ok) {
+      const int R = row(lwave, t);
+      if ((unsigned)R < (unsigned)NJ) {
+        const T v = zbuf[b][lwave][lane];
+        uint32_t wv[W];
+        __builtin_memcpy(wv, &v, sizeof(T));
+        const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
+      }
+    }
+    if (x_out) {
+      const int R = row(lane < TK ? lane : 0, t);
+      const T v = zbuf[b][lane < TK ? lane : 0][edge_out];
+      uint32_t wv[W];
+      __builtin_memcpy(wv, &v, sizeof(T));
+      const uint32_t o = xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+#pragma unroll
+      for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
+    }
+  };
+  zbuf[0][TK][lane] = T(0);
+  if (lane < TK) xlds[0][lane] = T(0);
+#pragma unroll
+  for (int u = 0; u < PF; ++u) fill(u, u);
+  take(0, 0, 1);  // inputs of step 0
+  fill(0, PF);
+  __syncthreads();
+  for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int s = s0 + u;
+      const int cb = u & 1, pb = cb ^ 1;
+      // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
+      if (s > 0) publish(s - 1, pb);
+      const int un = (u + 1) % PF;
+      take(un, s + 1, cb);
+      fill(un, s + 1 + PF);
+      __syncthreads();
+    }
+  }
+}
+
+template <class T, int TK>
+void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
+                   const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
+  const int grid = pa.ntile_i * pa.ntile_k;
+  if (op == kMicFwd)
+    k_mic_sweep3<T, false, TK><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else
+    k_mic_sweep3<T, true, TK><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+}
+
 template <class T, int OP, int TI, int TK>
 void launch_one(Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
                 const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
@@ -653,24 +1170,68 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 
 }  // namespace
 
-MicPipeShape mic_pipe_shape(Shear sh, int variant) {
-  // variants: (TI, TK) tiles of 64*TI values of i by TK planes
-  static const int table[][2] = {{1, 16}, {1, 8}, {2, 8}, {4, 4}, {1, 4}, {2, 4}};
-  int v = variant < 0 || variant >= (int)(sizeof table / sizeof table[0]) ? 0 : variant;
+// Tile shapes.  Variants 0-5: k_mic_solve tiles of 64*TI values of i by TK planes.
+// Variants 6-16: k_mic_solve2 tiles of 64 values of i by TK waves of KT planes, with the
+// XCD placement (x) where the tile grid allows it; 17-19: k_mic_sweep3 tiles of 64
+// values of i by TK planes (plus the I/O wave).
+namespace {
+struct VariantRow {
+  int ti, tk, kt, kind, xcd;
+};
+const VariantRow kVariants[] = {
+    {1, 16, 1, 0, 0}, {1, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 4, 1, 0, 0}, {1, 4, 1, 0, 0},
+    {2, 4, 1, 0, 0},  {1, 4, 4, 1, 0}, {1, 4, 2, 1, 0}, {1, 4, 8, 1, 0}, {1, 8, 2, 1, 0},
+    {1, 4, 4, 1, 8},  {1, 4, 2, 1, 8}, {1, 4, 8, 1, 8}, {1, 2, 8, 1, 8}, {1, 8, 4, 1, 8},
+    {1, 2, 4, 1, 8},  {1, 8, 2, 1, 8},  {1, 15, 1, 2, 0}, {1, 7, 1, 2, 0}, {1, 3, 1, 2, 0},
+};
+constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
+}  // namespace
+
+int mic_pipe_variants() { return kNumVariants; }
+
+MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
+  const int v = variant < 0 || variant >= kNumVariants ? 0 : variant;
+  const VariantRow& r = kVariants[v];
   MicPipeShape s;
-  s.ti = table[v][0];
-  s.tk = table[v][1];
+  s.ti = r.ti;
+  s.tk = r.tk;
+  s.kt = r.kt;
+  s.kind = r.kind;
+  const int planes = r.tk * r.kt;
   s.ntile_i = (sh.nx + 64 * s.ti - 1) / (64 * s.ti);
-  s.ntile_k = (sh.nz + s.tk - 1) / s.tk;
+  s.ntile_k = (sh.nz + planes - 1) / planes;
+  // XCD placement needs whole k-tile ranges per XCD and every tile resident at once
+  s.xcd = (r.xcd > 0 && s.ntile_k % r.xcd == 0 && s.ntile_i * s.ntile_k <= ncu) ? r.xcd : 0;
   return s;
 }
 
 template <class T>
 void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* precon, const T* in,
                      T* out, const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
-  MicPipeShape s = mic_pipe_shape(sh, variant);
+  MicPipeShape s = mic_pipe_shape(sh, variant, pa.ncu);
   pa.ntile_i = s.ntile_i;
   pa.ntile_k = s.ntile_k;
+  pa.xcd = s.xcd;
+  if (s.kind == 2) {
+    switch (s.tk) {
+      case 15: launch_sweep3<T, 15>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 7: launch_sweep3<T, 7>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 3: launch_sweep3<T, 3>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      default: break;
+    }
+  }
+  if (s.kind == 1) {
+    switch (s.tk * 100 + s.kt) {
+      case 404: launch_solve2<T, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 402: launch_solve2<T, 4, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 408: launch_solve2<T, 4, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 802: launch_solve2<T, 8, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 208: launch_solve2<T, 2, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 804: launch_solve2<T, 8, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 204: launch_solve2<T, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      default: break;
+    }
+  }
   switch (variant) {
     case 1: dispatch_op<T, 1, 8>(op, sh, flags, precon, in, out, sc, pa, st); break;
     case 2: dispatch_op<T, 2, 8>(op, sh, flags, precon, in, out, sc, pa, st); break;

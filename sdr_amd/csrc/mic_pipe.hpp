@@ -23,13 +23,20 @@ struct MicPipeArgs {
   int ntile_i, ntile_k;
   uint32_t tag;
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
+  int ncu;       // compute units of the device (bounds the tiles placed by XCD)
+  int xcd;       // XCD placement of the tiles (0: none); set by launch_mic_pipe
 };
 
+// kind 0: k_mic_solve tiles of 64*ti values of i by tk planes; kind 1: k_mic_solve2
+// tiles of 64 values of i by tk waves of kt planes; kind 2: k_mic_sweep3 tiles of 64
+// values of i by tk planes with an I/O wave.  The factor (build) always runs the kind-0
+// kernel.
 struct MicPipeShape {
-  int ti, tk, ntile_i, ntile_k;
+  int ti, tk, kt, kind, ntile_i, ntile_k, xcd;
 };
 
-MicPipeShape mic_pipe_shape(Shear sh, int variant);
+MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu);
+int mic_pipe_variants();
 
 // Sizes (64-bit words) of the halo buffers for a variant, with 2 values per cell
 // (enough for every op) of up to 2 words each.

@@ -197,6 +197,7 @@ struct Projector {
   uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
+  int ncu = 0, dev = 0;  // compute units of the solver's device
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
   void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
@@ -209,15 +210,17 @@ struct Projector {
     partials.alloc((size_t)nb * sizeof(double));
     scal.alloc(sizeof(PcgScalars));
     host_scal.ensure(sizeof(PcgScalars) + 16);
-    // f64 runs 512-thread tiles (1x8 planes), f32 1024-thread tiles (1x16 planes)
-    mic_variant = sizeof(T) == 8 ? 1 : 0;
+    // both precisions: 7-plane tiles with an I/O wave (k_mic_sweep3)
+    mic_variant = 18;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
     size_t hx = 0, hz = 0;
-    for (int v = 0; v < 6; ++v) {
-      MicPipeShape ps = mic_pipe_shape(sh, v);
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int v = 0; v < mic_pipe_variants(); ++v) {
+      MicPipeShape ps = mic_pipe_shape(sh, v, ncu);
       size_t a, c;
       mic_pipe_halo_sizes(sh, ps, &a, &c);
       hx = std::max(hx, a), hz = std::max(hz, c);
@@ -242,7 +245,10 @@ struct Projector {
         launch_mic_plane<T>(op, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sgm, sc(), st);
       return;
     }
-    MicPipeShape ps = mic_pipe_shape(sh, mic_variant);
+    // the factor runs the one-plane-per-wave kernel (kind 0)
+    int variant = mic_variant;
+    if (op == kMicBuild && mic_pipe_shape(sh, variant, ncu).kind != 0) variant = sizeof(T) == 8 ? 1 : 0;
+    MicPipeShape ps = mic_pipe_shape(sh, variant, ncu);
     if (++sweep_tag == 0) {  // tags wrapped: clear the halo words so no stale tag matches
       HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
       HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
@@ -250,8 +256,8 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes};
-    launch_mic_pipe<T>(op, mic_variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
+                      (uint32_t)halo_z.bytes, ncu, 0};
+    launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
   // A bounded wait of a pipelined sweep expired: the solve result is not valid.

@@ -102,6 +102,34 @@ def test_fast_mode_within_tolerance(dev, orc):
         assert ok, err
 
 
+@pytest.mark.parametrize("precision", [A.PRECISION_F32, A.PRECISION_F64])
+def test_p2g_tiled_matches_face_gather(dev, precision, monkeypatch):
+    """The LDS-staged P2G (slabs staged once, several z-levels accumulated at a time)
+    sums every face in the same order as the face-by-face gather, so the two agree bit
+    for bit; a 5000-particle clump makes some slabs overflow the staging area (several
+    passes)."""
+    sc = scenes.cavity_scene("c2")
+    g = np.random.default_rng(5)
+    out = []
+    for perface in ("0", "1"):
+        d = scenes.make_solver(sc, precision=precision, **FAST)
+        if not out:
+            pos = scenes.fill_particles(d, sc, d.cells().reshape(d.nz, d.ny, d.nx) != 0, stride=1)
+            c = np.array(d.cell_center(d.nx // 2, d.ny // 3, d.nz // 2))
+            clump = c + g.uniform(-0.45, 0.45, (5000, 3)) * d.dx
+            pos = np.concatenate([pos, clump])
+            vel = g.standard_normal(pos.shape)
+        d.set_particles(pos, vel)
+        monkeypatch.setenv("SDR_P2G_PERFACE", perface)
+        d.run_stage("classify", sc.dt)
+        d.run_stage("p2g", sc.dt)
+        out.append(d.grid())
+        d.close()
+    for a, b in zip(*out):
+        assert np.abs(a).max() > 0
+        assert bits_equal(a, b)
+
+
 def test_deterministic_run_to_run(dev):
     """No float atomics anywhere: two device runs agree bit for bit, fast mode included."""
     out = []
@@ -265,6 +293,45 @@ def test_mic_sweep_variants_bit_exact(dev, orc, variant, monkeypatch):
     assert ro.iters == rd.iters and ro.residual == rd.residual
     assert bits_equal(po, pd)
     for a, b in zip(fo, fd):
+        assert bits_equal(a, b)
+
+
+_MULTI_GRIDS = {}
+
+
+def _multi_case(orc, shape):
+    """Seeded projection case on `shape` and the oracle's result (computed once)."""
+    if shape not in _MULTI_GRIDS:
+        nx, ny, nz = shape
+        g = np.random.default_rng(11)
+        r = g.random((nz, ny, nx))
+        cells = np.where(r < 0.8, 1, np.where(r < 0.9, 0, 2)).astype(np.uint8)
+        faces = (g.standard_normal((nz, ny, nx + 1)), g.standard_normal((nz, ny + 1, nx)),
+                 g.standard_normal((nz + 1, ny, nx)))
+        u, v, w = (a.copy() for a in faces)
+        rep, p = project_capturing_with_source(nx, ny, nz, 0.01, (0, 0, 0), u, v, w, cells, 1000.0,
+                                               1e-3, SolveParams(25, 1e-9), lib=orc)
+        _MULTI_GRIDS[shape] = (cells, faces, (rep, p, u, v, w))
+    return _MULTI_GRIDS[shape]
+
+
+@pytest.mark.parametrize("shape", [(150, 37, 41), (130, 19, 256)])
+@pytest.mark.parametrize("variant", [str(v) for v in range(6, 20)])
+def test_mic_multiplane_sweeps_bit_exact(dev, orc, variant, shape, monkeypatch):
+    """The multi-plane sweeps (a thread owns several planes of its pencil), with and
+    without XCD placement of the tiles, and the sweeps with an I/O wave reproduce the
+    oracle's projection bit for bit.
+    On the 130x19x256 grid every tile depth divides into whole k-tile ranges per XCD, so
+    the XCD-placed variants run placed (the 150x37x41 grid runs them unplaced)."""
+    monkeypatch.setenv("SDR_MIC_VARIANT", variant)
+    nx, ny, nz = shape
+    cells, faces, (ro, po, *fo) = _multi_case(orc, shape)
+    u, v, w = (a.copy() for a in faces)
+    rd, pd = project_capturing_with_source(nx, ny, nz, 0.01, (0, 0, 0), u, v, w, cells, 1000.0, 1e-3,
+                                           SolveParams(25, 1e-9), lib=dev)
+    assert ro.iters == rd.iters and ro.residual == rd.residual
+    assert bits_equal(po, pd)
+    for a, b in zip(fo, (u, v, w)):
         assert bits_equal(a, b)
 
 
