@@ -324,6 +324,38 @@ __device__ inline void for_cells(Shear sh, F&& f) {
   }
 }
 
+// The in-grid run of sheared row `row` (= k * NJ + J): slots base + i, i in [imin, imin + len).
+struct RowRun {
+  int64_t base;
+  int imin, len, J, k;
+};
+__device__ inline RowRun row_run(Shear sh, int row) {
+  RowRun r;
+  r.J = row % sh.NJ;
+  r.k = row / sh.NJ;
+  r.imin = r.J - sh.ny + 1 > 0 ? r.J - sh.ny + 1 : 0;
+  const int imax = r.J < sh.nx - 1 ? r.J : sh.nx - 1;
+  r.len = imax - r.imin + 1;
+  r.base = (int64_t)row * sh.nx;
+  return r;
+}
+// Rows are taken two at a time per wavefront (row and row + stride), so each lane has
+// two independent cells' loads in flight: f(a_ok, a_run, a_i, b_ok, b_run, b_i).
+template <class F>
+__device__ inline void for_row_pairs(Shear sh, F&& f) {
+  const int lane = threadIdx.x & 63;
+  const int nrows = sh.nz * sh.NJ;
+  const int wstride = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
+  int row = __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  for (; row < nrows; row += 2 * wstride) {
+    const RowRun a = row_run(sh, row);
+    RowRun b = row_run(sh, row + wstride < nrows ? row + wstride : row);
+    if (row + wstride >= nrows) b.len = 0;
+    const int n = a.len > b.len ? a.len : b.len;
+    for (int c = lane; c < n; c += 64) f(c < a.len, a, a.imin + c, c < b.len, b, b.imin + c);
+  }
+}
+
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -355,8 +387,7 @@ __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
   const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
   double acc = 0.0;  // dot(x, A x) of the stored values (fast reduction mode)
-  for_cells(sh, [&](int64_t t, int i, int J, int k) {
-    const uint16_t f = flags[t];
+  auto cell = [&](int64_t t, int i, int J, int k, uint16_t f) -> T {
     T v = T(0);
     if (f & kFlFluid) {
       const int j = J - i;
@@ -370,7 +401,14 @@ __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restri
       if (k >= 1) v += ((f & kFlMz) ? T(-1) : T(0)) * x[t - plane];
       acc += (double)x[t] * (double)v;
     }
-    out[t] = v;
+    return v;
+  };
+  for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
+    const int64_t ta = a.base + ai, tb = b.base + bi;
+    const uint16_t fa = ao ? flags[ta] : 0, fb = bo ? flags[tb] : 0;
+    const T va = cell(ta, ai, a.J, a.k, fa), vb = cell(tb, bi, b.J, b.k, fb);
+    if (ao) out[ta] = va;
+    if (bo) out[tb] = vb;
   });
   if (partials) block_partial<0>(acc, partials);
 }
@@ -498,30 +536,50 @@ __global__ void __launch_bounds__(256) k_maxabs(Shear sh, const T* __restrict__ 
   block_partial<1>(m, partials);
 }
 
-// p += alpha s ; r -= alpha As (pressure.rs:371-375), fused with max|r| partials.
-template <class T>
-__global__ void __launch_bounds__(256) k_axpy_pr(Shear sh, T* __restrict__ p, T* __restrict__ r,
-                                                 const T* __restrict__ s, const T* __restrict__ as,
-                                                 const PcgScalars* sc, double* partials) {
-  if (sc->done) return;
-  const T alpha = (T)sc->alpha;
-  double m = 0.0;
-  for_cells(sh, [&](int64_t t, int, int, int) {
-    p[t] += alpha * s[t];
-    const T rv = r[t] - alpha * as[t];
-    r[t] = rv;
-    m = fmax(m, fabs((double)rv));
-  });
-  block_partial<1>(m, partials);
-}
-
 // s = z + beta s (pressure.rs:386-388).
 template <class T>
 __global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, const T* __restrict__ z,
                                                   const PcgScalars* sc) {
   if (sc->done) return;
   const T beta = (T)sc->beta;
-  for_cells(sh, [&](int64_t t, int, int, int) { s[t] = z[t] + beta * s[t]; });
+  for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
+    const int64_t ta = a.base + ai, tb = b.base + bi;
+    T za = T(0), sa = T(0), zb = T(0), sb = T(0);
+    if (ao) za = z[ta], sa = s[ta];
+    if (bo) zb = z[tb], sb = s[tb];
+    if (ao) s[ta] = za + beta * sa;
+    if (bo) s[tb] = zb + beta * sb;
+  });
+}
+
+// p += alpha s ; r -= alpha As (pressure.rs:371-375) with max|r| partials, two rows per
+// wavefront step (the loads of both cells are issued before either is stored).
+template <class T>
+__global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T* __restrict__ r,
+                                                  const T* __restrict__ s, const T* __restrict__ as,
+                                                  const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  const T alpha = (T)sc->alpha;
+  double m = 0.0;
+  for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
+    const int64_t ta = a.base + ai, tb = b.base + bi;
+    T pa = T(0), sa = T(0), ra = T(0), qa = T(0), pb = T(0), sb = T(0), rb = T(0), qb = T(0);
+    if (ao) pa = p[ta], sa = s[ta], ra = r[ta], qa = as[ta];
+    if (bo) pb = p[tb], sb = s[tb], rb = r[tb], qb = as[tb];
+    if (ao) {
+      p[ta] = pa + alpha * sa;
+      const T rv = ra - alpha * qa;
+      r[ta] = rv;
+      m = fmax(m, fabs((double)rv));
+    }
+    if (bo) {
+      p[tb] = pb + alpha * sb;
+      const T rv = rb - alpha * qb;
+      r[tb] = rv;
+      m = fmax(m, fabs((double)rv));
+    }
+  });
+  block_partial<1>(m, partials);
 }
 
 // ------------------------------------------------------------------ MIC(0) (pressure.rs:442-570)
@@ -1193,7 +1251,7 @@ void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
 template <class T>
 void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScalars* sc,
                     double* partials, int nblocks, hipStream_t st) {
-  k_axpy_pr<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials);
+  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials);
 }
 template <class T>
 void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st) {

@@ -909,7 +909,10 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 //     count, and the hardware drops them; lanes with i >= nx start at an offset that is
 //     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) hold
 //     flags 0, so they compute and store 0.  No bounds tests remain in the step.
-template <class T, bool REV, int TK>
+// DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
+// (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
+// workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
+template <class T, bool REV, int TK, bool DOT>
 __global__ void __launch_bounds__(64 * (TK + 1))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
@@ -922,6 +925,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   // halo row (the boundary wave's z neighbour)
   __shared__ T zbuf[2][TK + 1][64];
   __shared__ T xlds[2][TK];
+  __shared__ double dsum[TK];
 
   const int lane = threadIdx.x & 63;
   // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
@@ -955,7 +959,8 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     const rsrc_t rf = make_rsrc(flags + (int64_t)plane * NJ * nx, recs * 2u),
                  rp = make_rsrc(pbase, recs * SZ),
                  ri = make_rsrc(in + (int64_t)plane * NJ * nx, recs * SZ),
-                 ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ);
+                 ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ),
+                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u);
     const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
     // element offset of (row J, lane i); lanes past the grid start out of range for good
     // (2^28 * size stays below 2^32 and above every plane's record count)
@@ -965,14 +970,16 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     uint32_t nxt = lbase + (uint32_t)((J0 + PF * dJ) * nx);  // of the row PF steps ahead
     const uint32_t dstep = (uint32_t)(dJ * nx);
     uint32_t cf[PF];
-    T cp[PF], ci[PF];
+    T cp[PF], ci[PF], cr[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const uint32_t e = lbase + (uint32_t)((J0 + u * dJ) * nx);
       cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
+      if (DOT) cr[u] = bld<T>(rr, e * SZ);
     }
+    double dacc = 0.0;
     zbuf[1][wk][lane] = T(0);
     T prev = T(0);
     __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
@@ -1006,13 +1013,25 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         bst<T>(ro, cur * SZ, res);
         zbuf[cb][wk][lane] = pub;
         prev = pub;
+        if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
         cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, nxt * 2u, 0, 0);
         cp[u] = bld<T>(rp, nxt * SZ);
         ci[u] = bld<T>(ri, nxt * SZ);
+        if (DOT) cr[u] = bld<T>(rr, nxt * SZ);
         cur += dstep;
         nxt += dstep;
         __syncthreads();
       }
+    }
+    if (DOT) {
+      for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o);
+      if (lane == 0) dsum[wk] = dacc;
+    }
+    __syncthreads();  // matched by the I/O wave
+    if (DOT && wk == 0 && lane == 0) {
+      double t = 0.0;
+      for (int w = 0; w < TK; ++w) t += dsum[w];
+      pa.dot_partials[blockIdx.x] = t;
     }
     return;
   }
@@ -1136,6 +1155,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       __syncthreads();
     }
   }
+  __syncthreads();  // matches the compute waves' final barrier
 }
 
 template <class T, int TK>
@@ -1143,9 +1163,11 @@ void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, con
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
   if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, false, TK, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else if (pa.dot_r)
+    k_mic_sweep3<T, true, TK, true><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
-    k_mic_sweep3<T, true, TK><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }
 
 template <class T, int OP, int TI, int TK>

@@ -25,6 +25,10 @@ struct MicPipeArgs {
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
   int ncu;       // compute units of the device (bounds the tiles placed by XCD)
   int xcd;       // XCD placement of the tiles (0: none); set by launch_mic_pipe
+  // Backward solves of kind 2 only: with dot_r set, the sweep also leaves one partial of
+  // dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in dot_partials.
+  const T* dot_r;
+  double* dot_partials;
 };
 
 // kind 0: k_mic_solve tiles of 64*ti values of i by tk planes; kind 1: k_mic_solve2
@@ -36,6 +40,7 @@ struct MicPipeShape {
 };
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu);
+inline int mic_pipe_dot_partials(const MicPipeShape& s) { return s.ntile_i * s.ntile_k; }
 int mic_pipe_variants();
 
 // Sizes (64-bit words) of the halo buffers for a variant, with 2 values per cell

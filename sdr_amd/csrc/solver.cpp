@@ -207,7 +207,6 @@ struct Projector {
     for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
     flags.alloc(n * sizeof(uint16_t));
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
-    partials.alloc((size_t)nb * sizeof(double));
     scal.alloc(sizeof(PcgScalars));
     host_scal.ensure(sizeof(PcgScalars) + 16);
     // both precisions: 7-plane tiles with an I/O wave (k_mic_sweep3)
@@ -226,6 +225,8 @@ struct Projector {
       hx = std::max(hx, a), hz = std::max(hz, c);
     }
     if (hx * sizeof(uint64_t) >= lim || hz * sizeof(uint64_t) >= lim) mic_variant = -1;
+    for (int v = 0; v < mic_pipe_variants(); ++v) nb = std::max(nb, mic_pipe_dot_partials(mic_pipe_shape(sh, v, ncu)));
+    partials.alloc((size_t)nb * sizeof(double));
     if (mic_variant < 0) hx = hz = 1;
     halo_x.alloc(hx * sizeof(uint64_t));
     halo_z.alloc(hz * sizeof(uint64_t));
@@ -238,7 +239,10 @@ struct Projector {
 
   PcgScalars* sc() const { return scal.as<PcgScalars>(); }
 
-  void mic_sweep(int op, const T* in, T* out) {
+  // The fast reduction mode fuses dot(z, r) into the backward solve (kind-2 sweeps only).
+  bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 2; }
+
+  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr) {
     if (mic_variant < 0) {
       const int planes = sh.NJ + sh.nz - 1;
       for (int sgm = 0; sgm < planes; ++sgm)
@@ -256,7 +260,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, 0};
+                      (uint32_t)halo_z.bytes, ncu, 0, dot_r, partials.as<double>()};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
@@ -315,6 +319,39 @@ struct Projector {
     dot(z.as<T>(), r, kOpSigma0, tol);
     const int mb = maxabs_blocks(sh);
     const uint64_t check_every = 8;
+    if (fused()) {
+      const int ndot = mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu));
+      for (uint64_t it = 0; it < max_iters; ++it) {
+        {
+          Scope sc1(*timer, "spmv_dot");
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(),
+                         st);
+          launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
+        }
+        {
+          Scope sc1(*timer, "axpy_norm");
+          launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+          launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
+        }
+        {
+          Scope sc1(*timer, "mic_fwd");
+          mic_sweep(kMicFwd, r, q.as<T>());
+        }
+        {
+          Scope sc1(*timer, "mic_bwd_dot");
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
+          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
+        }
+        {
+          Scope sc1(*timer, "update_s");
+          launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
+        }
+        if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
+          if (read_scalars().done) break;
+        }
+      }
+      return read_scalars();
+    }
     for (uint64_t it = 0; it < max_iters; ++it) {
       {
         Scope sc_(*timer, "spmv_dot");
