@@ -228,6 +228,23 @@ def solve_error(s, scene, rel_ref, rel_tol, f32):
     }
 
 
+def serving_run(count: int, slots: int) -> dict:
+    """The serving form: `count` irrigation scenarios (seeded Poisson arrivals at 8/s,
+    grids 32^3-128^3, needle angle +-30 deg, 1-6 ml/s, 0.5-2 ml) through the native
+    serving loop under a 200 GB footprint budget; p50/p99 latency is finish - arrival."""
+    from sdr_amd import serving
+
+    specs = serving.serving_workload(count)
+    t = time.perf_counter()
+    recs = serving.serve(specs, budget_bytes=200e9, max_slots=slots)
+    out = serving.summarize(recs)
+    out.update({"slots": slots, "budget_gb": 200, "wall_s": round(time.perf_counter() - t, 2),
+                "workload": f"{count} scenarios, Poisson 8/s, n^3 with n in [32,128], f32 + f64 "
+                            f"refinement to 1e-6, 8-40 substeps each (dose/flow scaled)"})
+    print(f"[bench] serving: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +262,9 @@ def main() -> int:
     ap.add_argument("--no-error", action="store_true", help="skip the error-vs-f64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the 64-scenario serving run")
+    ap.add_argument("--serve-count", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the substep runs "
@@ -315,6 +335,12 @@ def main() -> int:
             cpu = cpu_baseline(scene, scene.sdf, float(np.mean(iters)), 20.0, threads)
         except Exception as e:  # the GPU number stands on its own
             cpu = {"error": repr(e)}
+    serve = None
+    if not args.no_serve:
+        try:
+            serve = serving_run(args.serve_count, args.serve_slots)
+        except Exception as e:  # the throughput number stands on its own
+            serve = {"error": repr(e)}
     prec_txt = ("f32 storage + f64 dots/norms/refinement" if prec == A.PRECISION_F32 else "f64")
     out = {
         "metric": METRIC,
@@ -344,6 +370,7 @@ def main() -> int:
         "regions_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(timing.items())},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     if cpu and "value" in cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -234,6 +234,58 @@ int sdr_project_capturing_with_source(const sdr_dims* grid, double* u, double* v
                                       double* pressure_out, sdr_solve_report* report,
                                       const sdr_options* opts);
 
+/* ---------------------------------------------------------------------------------
+ * Serving (no reference counterpart: the reference runs one scenario per process,
+ * main.rs:215-386).  Many irrigation scenarios of different sizes share one GPU: each
+ * request is admitted when a slot is free and its estimated HBM footprint fits the
+ * budget (first come, first served), runs on its own solver and HIP stream from its own
+ * host thread, so the work of admitted scenarios overlaps on the device, and is
+ * released when it finishes.  Each scenario's result depends only on its own inputs
+ * (no float atomics, no shared state): it is the same alone and in the mix. */
+typedef struct sdr_scenario {
+  sdr_sdf solid;            /* host SDF, read while the scenario is being set up */
+  double sim_dx;
+  sdr_fluid_params fluid;
+  sdr_needle needle;
+  sdr_options opts;
+  sdr_solve_params solve;
+  int32_t has_outlet;
+  sdr_aabb outlet;
+  uint32_t fill_per_cell;   /* sdr_solver_seed_fill arguments */
+  double fill_y_lo, fill_y_hi;
+  uint64_t fill_seed;
+  double dt;                /* fixed substep */
+  uint32_t substeps;        /* work of the request */
+  double arrival_s;         /* arrival time relative to the start of sdr_serve */
+} sdr_scenario;
+
+typedef struct sdr_serve_config {
+  double hbm_budget_bytes;  /* admission budget over the estimated footprints */
+  uint32_t max_slots;       /* scenarios in flight at once */
+} sdr_serve_config;
+
+typedef struct sdr_scenario_result {
+  int32_t status;           /* SDR_OK, or the failure (SDR_E_OOM: larger than the budget) */
+  uint32_t slot;
+  double arrival_s, admit_s, finish_s;  /* seconds from the start of sdr_serve */
+  uint64_t footprint_bytes;             /* the admission estimate */
+  uint64_t particles;                   /* resident at the end */
+  uint64_t particle_substeps;           /* sum over substeps of the particle count */
+  uint64_t pcg_iters;                   /* sum over substeps */
+  uint64_t checksum;        /* FNV-1a of the final positions and velocities (f64 bits) */
+} sdr_scenario_result;
+
+/* Estimated device footprint (bytes) of a solver of the given grid, particle capacity
+ * and precision (the admission estimate). */
+int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particles, int32_t precision,
+                         uint64_t* bytes);
+
+/* Serve `n` scenarios (arrival order need not be sorted); blocks until all finish.
+ * out[i] belongs to scenario i.  Returns SDR_OK when every scenario ran (individual
+ * failures are in out[i].status). */
+int sdr_serve(const sdr_scenario* scenarios, uint32_t n, const sdr_serve_config* cfg,
+              sdr_scenario_result* out);
+
 #ifdef __cplusplus
 }
 #endif

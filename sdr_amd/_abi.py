@@ -87,6 +87,25 @@ class Dims(C.Structure):
                 ("origin", Vec3)]
 
 
+class Scenario(C.Structure):
+    _fields_ = [("solid", SdfDesc), ("sim_dx", C.c_double), ("fluid", FluidParams),
+                ("needle", NeedleC), ("opts", Options), ("solve", SolveParamsC),
+                ("has_outlet", C.c_int32), ("outlet", Aabb), ("fill_per_cell", C.c_uint32),
+                ("fill_y_lo", C.c_double), ("fill_y_hi", C.c_double), ("fill_seed", C.c_uint64),
+                ("dt", C.c_double), ("substeps", C.c_uint32), ("arrival_s", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_slots", C.c_uint32)]
+
+
+class ScenarioResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("slot", C.c_uint32), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("finish_s", C.c_double), ("footprint_bytes", C.c_uint64),
+                ("particles", C.c_uint64), ("particle_substeps", C.c_uint64),
+                ("pcg_iters", C.c_uint64), ("checksum", C.c_uint64)]
+
+
 P = C.c_void_p
 # name -> argtypes (restype is always c_int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -126,6 +145,8 @@ DEVICE_ONLY = {
     "solver_get_timing": [P, C.c_char_p, C.c_size_t, c_double_p, c_u64_p, C.c_uint64, c_u64_p],
     "solver_seed_fill": [P, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Vec3),
                          c_u64_p],
+    "solver_footprint": [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, c_u64_p],
+    "serve": [C.POINTER(Scenario), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(ScenarioResult)],
 }
 
 

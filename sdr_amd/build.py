@@ -28,7 +28,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["kernels.hip", "mic_pipe.hip", "redist.hip", "p2g.hip", "sort.hip", "solver.cpp"]
+SOURCES = ["kernels.hip", "mic_pipe.hip", "redist.hip", "p2g.hip", "sort.hip", "solver.cpp", "serve.cpp"]
 
 
 def _hipcc() -> str:

@@ -912,7 +912,7 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 // DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
-template <class T, bool REV, int TK, bool DOT>
+template <class T, bool REV, int TK, bool DOT, int PFH>
 __global__ void __launch_bounds__(64 * (TK + 1))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
@@ -1063,7 +1063,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   auto xoff = [&](int R) -> uint32_t {
     return x_in && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_in + R * W) * 8u : kOob;
   };
-  uint64_t hz[PF][W], hx[PF][W];
+  uint64_t hz[PFH][W], hx[PFH][W];
   auto fill = [&](int u, int t) {
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
 #pragma unroll
@@ -1096,14 +1096,14 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
     const bool ok = ready(hz[u], oz) && ready(hx[u], ox);
     if (__ballot(!ok) != 0ull) {
-      const uint32_t nz_ = zoff(zrow_of(t + PF - 1)), nx_ = xoff(xrow_of(t + PF - 1));
+      const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
       uint64_t wz[W], wx[W];
 #pragma unroll
       for (int q = 0; q < W; ++q) wz[q] = wx[q] = 0;  // tag 0 is never a sweep's tag
       settle(wz, rhz, nz_, (uint32_t)nxp * 8u);
       settle(wx, rhx, nx_, 8u);
 #pragma unroll
-      for (int v = 0; v < PF; ++v) fill((u + v) % PF, t + v);
+      for (int v = 0; v < PFH; ++v) fill((u + v) % PFH, t + v);
       settle(hz[u], rhz, oz, (uint32_t)nxp * 8u);
       settle(hx[u], rhx, ox, 8u);
       __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -1138,9 +1138,9 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   zbuf[0][TK][lane] = T(0);
   if (lane < TK) xlds[0][lane] = T(0);
 #pragma unroll
-  for (int u = 0; u < PF; ++u) fill(u, u);
+  for (int u = 0; u < PFH; ++u) fill(u, u);
   take(0, 0, 1);  // inputs of step 0
-  fill(0, PF);
+  fill(0, PFH);
   __syncthreads();
   for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
 #pragma unroll
@@ -1149,25 +1149,25 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       const int cb = u & 1, pb = cb ^ 1;
       // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
       if (s > 0) publish(s - 1, pb);
-      const int un = (u + 1) % PF;
+      const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
-      fill(un, s + 1 + PF);
+      fill(un, s + 1 + PFH);
       __syncthreads();
     }
   }
   __syncthreads();  // matches the compute waves' final barrier
 }
 
-template <class T, int TK>
+template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
   if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
-    k_mic_sweep3<T, true, TK, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }
 
 template <class T, int OP, int TI, int TK>
@@ -1194,8 +1194,9 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 
 // Tile shapes.  Variants 0-5: k_mic_solve tiles of 64*TI values of i by TK planes.
 // Variants 6-16: k_mic_solve2 tiles of 64 values of i by TK waves of KT planes, with the
-// XCD placement (x) where the tile grid allows it; 17-19: k_mic_sweep3 tiles of 64
-// values of i by TK planes (plus the I/O wave).
+// XCD placement (x) where the tile grid allows it; 17+: k_mic_sweep3 tiles of 64
+// values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
+// (kind 4) or 2 (kind 6).
 namespace {
 struct VariantRow {
   int ti, tk, kt, kind, xcd;
@@ -1205,6 +1206,7 @@ const VariantRow kVariants[] = {
     {2, 4, 1, 0, 0},  {1, 4, 4, 1, 0}, {1, 4, 2, 1, 0}, {1, 4, 8, 1, 0}, {1, 8, 2, 1, 0},
     {1, 4, 4, 1, 8},  {1, 4, 2, 1, 8}, {1, 4, 8, 1, 8}, {1, 2, 8, 1, 8}, {1, 8, 4, 1, 8},
     {1, 2, 4, 1, 8},  {1, 8, 2, 1, 8},  {1, 15, 1, 2, 0}, {1, 7, 1, 2, 0}, {1, 3, 1, 2, 0},
+    {1, 7, 1, 4, 0},  {1, 7, 1, 6, 0},  {1, 15, 1, 4, 0}, {1, 3, 1, 4, 0},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
@@ -1234,11 +1236,16 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
   pa.ntile_i = s.ntile_i;
   pa.ntile_k = s.ntile_k;
   pa.xcd = s.xcd;
-  if (s.kind == 2) {
-    switch (s.tk) {
-      case 15: launch_sweep3<T, 15>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 7: launch_sweep3<T, 7>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 3: launch_sweep3<T, 3>(op, sh, flags, precon, in, out, sc, pa, st); return;
+  if (s.kind == 2 || s.kind == 4 || s.kind == 6) {
+    const int pfh = s.kind == 2 ? 8 : (s.kind == 4 ? 4 : 2);
+    switch (s.tk * 10 + pfh) {
+      case 158: launch_sweep3<T, 15, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 78: launch_sweep3<T, 7, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 38: launch_sweep3<T, 3, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 154: launch_sweep3<T, 15, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 74: launch_sweep3<T, 7, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 34: launch_sweep3<T, 3, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 72: launch_sweep3<T, 7, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
       default: break;
     }
   }

@@ -209,8 +209,8 @@ struct Projector {
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
     scal.alloc(sizeof(PcgScalars));
     host_scal.ensure(sizeof(PcgScalars) + 16);
-    // both precisions: 7-plane tiles with an I/O wave (k_mic_sweep3)
-    mic_variant = 18;
+    // both precisions: 7-plane tiles with an I/O wave, halo prefetch depth 4 (k_mic_sweep3)
+    mic_variant = 20;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
@@ -218,7 +218,8 @@ struct Projector {
     size_t hx = 0, hz = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    for (int v = 0; v < mic_pipe_variants(); ++v) {
+    // halo words of the solve variant and of the factor's variant (both share them)
+    for (int v : {mic_variant < 0 ? 0 : mic_variant, sizeof(T) == 8 ? 1 : 0}) {
       MicPipeShape ps = mic_pipe_shape(sh, v, ncu);
       size_t a, c;
       mic_pipe_halo_sizes(sh, ps, &a, &c);
@@ -240,7 +241,7 @@ struct Projector {
   PcgScalars* sc() const { return scal.as<PcgScalars>(); }
 
   // The fast reduction mode fuses dot(z, r) into the backward solve (kind-2 sweeps only).
-  bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 2; }
+  bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind >= 2; }
 
   void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr) {
     if (mic_variant < 0) {
@@ -1134,6 +1135,25 @@ static int with(sdr_solver* s, F&& fn) {
 extern "C" {
 
 const char* sdr_last_error(void) { return sdr::g_err.c_str(); }
+
+int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particles, int32_t precision,
+                         uint64_t* bytes) {
+  return sdr::guard([&] {
+    if (!bytes) throw sdr::Error(SDR_E_INVALID, "null argument");
+    const double S = precision == SDR_PRECISION_F32 ? 4.0 : 8.0;
+    const double nc = (double)nx * ny * nz;
+    const double nsh = (double)nx * (nx + ny - 1) * nz;  // sheared solve layout
+    // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch
+    const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 12.0 + (precision == SDR_PRECISION_F64 ? 16.0 : 0.0);
+    // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
+    // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
+    // vectors (f32 path), halo words of the pipelined sweeps
+    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 49.0;
+    const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
+    const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
+    *bytes = (uint64_t)est;
+  });
+}
 
 int sdr_device_info(int32_t* count, char* arch, size_t arch_len) {
   return guard([&] {
