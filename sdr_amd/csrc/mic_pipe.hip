@@ -1158,6 +1158,306 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   __syncthreads();  // matches the compute waves' final barrier
 }
 
+// ------------------------------------------------------------------ multi-plane sweeps with an I/O wave
+// k_mic_sweep3 with KT planes per compute wave: plane t of a wave is at row J - t, so
+// the z neighbour of its plane t > 0 (t < KT - 1 backward) is the wave's own plane
+// t - 1 (t + 1) from the previous step, a register; only the wave's boundary plane
+// takes it from LDS (the wave before) or the halo (the tile before).  A tile is TK * KT
+// planes deep for the same TK + 1 waves, so the tiles, and the hops between them, are
+// KT times fewer; the KT cells of a step are independent of each other.  The cell ring
+// is PF steps deep (3 loads and a store per plane and step all count against the
+// wave's 63 outstanding memory operations, so PF shrinks as KT grows).
+template <class T, bool REV, int TK, int KT, int PF, bool DOT, int PFH>
+__global__ void __launch_bounds__(64 * (TK + 1))
+    k_mic_sweep4(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
+                 const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
+                 MicPipeArgs<T> pa) {
+  if (sc->done) return;
+  constexpr int W = Words<T>::n;
+  constexpr uint32_t SZ = sizeof(T);
+  constexpr int P = TK * KT;  // planes per tile
+  static_assert(P <= 64, "one I/O lane per plane of the tile");
+  // the step loop is unrolled by U: ring slots are compile-time (u % PF, (u+1) % PFH) and
+  // the LDS parity of step s0 + u is u & 1
+  constexpr int U = PF > PFH ? PF : PFH;
+  static_assert(U % PF == 0 && U % PFH == 0 && U % 2 == 0, "ring depths must divide the unroll");
+  // zbuf[.][w] for w < TK: the boundary plane of compute wave w (its last plane forward,
+  // first backward) at a step; zbuf[.][TK]: the fetched z halo row of the tile's first
+  // (last) plane.  edgev: every plane's edge lane (published as x halo by the I/O wave).
+  __shared__ T zbuf[2][TK + 1][64];
+  __shared__ __align__(16) T xlds[2][P];
+  __shared__ __align__(16) T edgev[2][P];
+  __shared__ double dsum[TK];
+
+  const int lane = threadIdx.x & 63;
+  // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool io = wave == TK;
+  const int nti = pa.ntile_i, ntk = pa.ntile_k;
+  int itl, ktl;
+  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
+  const int it = REV ? nti - 1 - itl : itl;
+  const int kt = REV ? ntk - 1 - ktl : ktl;
+  const int i = it * 64 + lane;
+  const int NJ = sh.NJ, nx = sh.nx, nz = sh.nz;
+  const int nxp = nti * 64;
+  const int dJ = REV ? -1 : 1;
+  const uint32_t tag = pa.tag;
+  // row of tile plane p at step s
+  auto row = [&](int p, int s) { return REV ? (NJ - 1) - s + (P - 1 - p) : s - p; };
+  const int nsteps = NJ + P - 1;  // compute steps; the I/O wave runs one more to publish
+  const int nsteps_pad = (nsteps + 1 + U - 1) / U * U;
+  const int bwave = REV ? TK - 1 : 0;       // boundary wave: its z neighbour is in the tile before
+  const int lwave = REV ? 0 : TK - 1;       // last wave: the tile after takes its z neighbour from it
+  const int bplane = REV ? P - 1 : 0, lplane = REV ? 0 : P - 1;
+  const int tzin = REV ? KT - 1 : 0, tzout = REV ? 0 : KT - 1;  // a wave's boundary planes
+  const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
+
+  if (!io) {
+    // ------------------------------------------------------------ compute wave
+    const int wk = wave;
+    // One descriptor per array for the wave's KT consecutive planes (record count: the
+    // planes inside the grid).  Plane t's cell (row J, lane i) is element
+    // (t * NJ + J) * nx + i; a row outside [0, NJ) gets the out-of-range offset kFar
+    // instead, as do lanes with i >= nx, so no bounds test is left in the step.  Both
+    // together give 2^28 elements, whose byte offset (times 8 at most) stays below 2^32,
+    // and kFar exceeds every record count (KT * NJ * nx elements, checked on the host).
+    const int k0 = kt * P + wk * KT;
+    const int np_ = nz - k0 < 0 ? 0 : (nz - k0 > KT ? KT : nz - k0);
+    const int64_t pl = (int64_t)(np_ > 0 ? k0 : 0) * NJ * nx;
+    const uint32_t recs = (uint32_t)np_ * (uint32_t)(NJ * nx);
+    const rsrc_t rf = make_rsrc(flags + pl, recs * 2u), rp = make_rsrc(precon + pl, recs * SZ),
+                 ri = make_rsrc(in + pl, recs * SZ), ro = make_rsrc(out + pl, recs * SZ),
+                 rr = make_rsrc(DOT ? pa.dot_r + pl : nullptr, DOT ? recs * SZ : 0u);
+    constexpr uint32_t kFar = 1u << 27;
+    const uint32_t li = i < nx ? (uint32_t)i : kFar;
+    const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
+    const int J0 = row(wk * KT, 0);
+    // scalar part of plane t's element offset at plane-0 row J
+    auto soff = [&](int t, int J) -> uint32_t {
+      const int R = J - t;
+      return (unsigned)R < (unsigned)NJ ? (uint32_t)((t * NJ + R) * nx) : kFar;
+    };
+    uint32_t cf[PF][KT];
+    T cp[PF][KT], ci[PF][KT], cr[PF][KT];
+    auto load = [&](int u, int J) {
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const uint32_t e = li + soff(t, J);
+        cf[u][t] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
+        cp[u][t] = bld<T>(rp, e * SZ);
+        ci[u][t] = bld<T>(ri, e * SZ);
+        if (DOT) cr[u][t] = bld<T>(rr, e * SZ);
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u) load(u, J0 + u * dJ);
+    int J = J0;
+    double dacc = 0.0;
+    zbuf[1][wk][lane] = T(0);
+    T prev[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) prev[t] = T(0);
+    __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
+    for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const int u = uu % PF;
+        const int cb = uu & 1, pb = cb ^ 1;
+        const T zb = zbuf[pb][zsrc][lane];
+        T pubn[KT];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          const T xe = xlds[pb][wk * KT + t];
+          const T xs = wave_shift<REV>(prev[t]);
+          const T xin = lane == edge_in ? xe : xs;
+          const T zin = t == tzin ? zb : prev[REV ? (t + 1) % KT : (t + KT - 1) % KT];
+          const T yin = prev[t];
+          const uint32_t f = cf[u][t];
+          const T pcu = cp[u][t], inu = ci[u][t];
+          const bool live = (f & kFlFluid) != 0;
+          T res, pub;
+          if (!REV) {
+            // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
+            const T q = (((inu - xin) - yin) - zin) * pcu;
+            res = live ? q : T(0);
+            pub = live ? (-pcu) * q : T(0);
+          } else {
+            const T npc = -pcu;
+            T x = inu;
+            x = x - ((f & kFlPx) ? npc * xin : T(0));
+            x = x - ((f & kFlPy) ? npc * yin : T(0));
+            x = x - ((f & kFlPz) ? npc * zin : T(0));
+            const T z = x * pcu;
+            res = live ? z : T(0);
+            pub = res;
+          }
+          bst<T>(ro, (li + soff(t, J)) * SZ, res);
+          if (DOT) dacc += (double)res * (double)cr[u][t];  // 0 outside the grid and off-fluid
+          pubn[t] = pub;
+        }
+        zbuf[cb][wk][lane] = pubn[tzout];
+        if (lane == edge_out) {
+#pragma unroll
+          for (int t = 0; t < KT; ++t) edgev[cb][wk * KT + t] = pubn[t];
+        }
+#pragma unroll
+        for (int t = 0; t < KT; ++t) prev[t] = pubn[t];
+        load(u, J + PF * dJ);
+        J += dJ;
+        __syncthreads();
+      }
+    }
+    if (DOT) {
+      for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o);
+      if (lane == 0) dsum[wk] = dacc;
+    }
+    __syncthreads();  // matched by the I/O wave
+    if (DOT && wk == 0 && lane == 0) {
+      double t = 0.0;
+      for (int w = 0; w < TK; ++w) t += dsum[w];
+      pa.dot_partials[blockIdx.x] = t;
+    }
+    return;
+  }
+
+  // -------------------------------------------------------------- I/O wave
+  const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
+  const bool z_in = REV ? kt + 1 < ntk : kt > 0;
+  const bool z_out = REV ? kt > 0 : kt + 1 < ntk;
+  const bool x_in = REV ? it + 1 < nti : it > 0;
+  const bool x_out = REV ? it > 0 : it + 1 < nti;
+  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
+  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
+  const int zrow = W * nxp;
+  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
+  // lane l < P serves plane kt * P + l for the x halo
+  const int kl = kt * P + lane;
+  const bool xl = lane < P && kl < nz;
+  const int xh_in = (xb_in * nz + (xl ? kl : 0)) * NJ * W;
+  const int xh_out = (xb_out * nz + (xl ? kl : 0)) * NJ * W;
+  const int klast = kt * P + lplane;
+  const bool zl_ok = klast < nz;
+  // Inputs of target step t: the z halo row of the boundary wave at step t, and the x
+  // neighbour row of plane `lane` at step t (its row at step t, minus dJ).
+  auto zrow_of = [&](int t) { return row(bplane, t); };
+  auto xrow_of = [&](int t) { return row(lane, t) - dJ; };
+  auto zoff = [&](int R) -> uint32_t {
+    return z_in && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
+  };
+  auto xoff = [&](int R) -> uint32_t {
+    return x_in && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_in + R * W) * 8u : kOob;
+  };
+  uint64_t hz[PFH][W], hx[PFH][W];
+  auto fill = [&](int u, int t) {
+    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
+      hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
+    }
+  };
+  // A word is ready when it carries this sweep's tag, or when it is not needed (its
+  // offset is out of range: the load returned 0, and the value used is 0).
+  auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
+    if (o == kOob) return;
+    long spins = 0;
+    while (!tags_ok<W>(w, tag)) {
+      if (++spins > pa.spin_limit ||
+          __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
+    }
+  };
+  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return o == kOob || tags_ok<W>(w, tag); };
+  // Take the inputs of target step t from ring slot u into LDS buffer b.  A stale word
+  // means the producer is behind the prefetch window: wait for the window's newest word
+  // (rows are published in order), then reload the whole window.
+  auto take = [&](int u, int t, int b) {
+    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+    const bool ok = ready(hz[u], oz) && ready(hx[u], ox);
+    if (__ballot(!ok) != 0ull) {
+      const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
+      uint64_t wz[W], wx[W];
+#pragma unroll
+      for (int q = 0; q < W; ++q) wz[q] = wx[q] = 0;  // tag 0 is never a sweep's tag
+      settle(wz, rhz, nz_, (uint32_t)nxp * 8u);
+      settle(wx, rhx, nx_, 8u);
+#pragma unroll
+      for (int v = 0; v < PFH; ++v) fill((u + v) % PFH, t + v);
+      settle(hz[u], rhz, oz, (uint32_t)nxp * 8u);
+      settle(hx[u], rhx, ox, 8u);
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
+    if (lane < P) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
+  };
+  // Publish the values compute waves wrote at step t into LDS buffer b.
+  auto publish = [&](int t, int b) {
+    if (z_out && zl_ok) {
+      const int R = row(lplane, t);
+      if ((unsigned)R < (unsigned)NJ) {
+        const T v = zbuf[b][lwave][lane];
+        uint32_t wv[W];
+        __builtin_memcpy(wv, &v, sizeof(T));
+        const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+          bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
+      }
+    }
+    if (x_out) {
+      const int R = row(lane < P ? lane : 0, t);
+      const T v = edgev[b][lane < P ? lane : 0];
+      uint32_t wv[W];
+      __builtin_memcpy(wv, &v, sizeof(T));
+      const uint32_t o = xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+#pragma unroll
+      for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
+    }
+  };
+  zbuf[0][TK][lane] = T(0);
+  if (lane < P) xlds[0][lane] = T(0);
+#pragma unroll
+  for (int u = 0; u < PFH; ++u) fill(u, u);
+  take(0, 0, 1);  // inputs of step 0
+  fill(0, PFH);
+  __syncthreads();
+  for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      const int cb = u & 1, pb = cb ^ 1;
+      // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
+      if (s > 0) publish(s - 1, pb);
+      const int un = (u + 1) % PFH;
+      take(un, s + 1, cb);
+      fill(un, s + 1 + PFH);
+      __syncthreads();
+    }
+  }
+  __syncthreads();  // matches the compute waves' final barrier
+}
+
+template <class T, int TK, int KT, int PF, int PFH>
+void launch_sweep4(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
+                   const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
+  const int grid = pa.ntile_i * pa.ntile_k;
+  if (op == kMicFwd)
+    k_mic_sweep4<T, false, TK, KT, PF, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
+                                                                                 sc, pa);
+  else if (pa.dot_r)
+    k_mic_sweep4<T, true, TK, KT, PF, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
+                                                                               sc, pa);
+  else
+    k_mic_sweep4<T, true, TK, KT, PF, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
+                                                                                sc, pa);
+}
+
 template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
@@ -1196,7 +1496,8 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 // Variants 6-16: k_mic_solve2 tiles of 64 values of i by TK waves of KT planes, with the
 // XCD placement (x) where the tile grid allows it; 17+: k_mic_sweep3 tiles of 64
 // values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
-// (kind 4) or 2 (kind 6).
+// (kind 4) or 2 (kind 6); 24+: k_mic_sweep4 tiles of TK waves of KT planes (plus the
+// I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2) or 4 (kind 9).
 namespace {
 struct VariantRow {
   int ti, tk, kt, kind, xcd;
@@ -1207,6 +1508,8 @@ const VariantRow kVariants[] = {
     {1, 4, 4, 1, 8},  {1, 4, 2, 1, 8}, {1, 4, 8, 1, 8}, {1, 2, 8, 1, 8}, {1, 8, 4, 1, 8},
     {1, 2, 4, 1, 8},  {1, 8, 2, 1, 8},  {1, 15, 1, 2, 0}, {1, 7, 1, 2, 0}, {1, 3, 1, 2, 0},
     {1, 7, 1, 4, 0},  {1, 7, 1, 6, 0},  {1, 15, 1, 4, 0}, {1, 3, 1, 4, 0},
+    {1, 7, 4, 8, 0},  {1, 7, 2, 8, 0},  {1, 3, 4, 8, 0},  {1, 7, 4, 9, 0}, {1, 3, 8, 8, 0},
+    {1, 15, 2, 8, 0},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
@@ -1214,7 +1517,9 @@ constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 int mic_pipe_variants() { return kNumVariants; }
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
-  const int v = variant < 0 || variant >= kNumVariants ? 0 : variant;
+  int v = variant < 0 || variant >= kNumVariants ? 0 : variant;
+  // k_mic_sweep4 addresses a wave's planes below 2^27 elements (kFar)
+  if (kVariants[v].kind >= 8 && (int64_t)kVariants[v].kt * sh.NJ * sh.nx >= (1 << 27)) v = 20;
   const VariantRow& r = kVariants[v];
   MicPipeShape s;
   s.ti = r.ti;
@@ -1246,6 +1551,17 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
       case 74: launch_sweep3<T, 7, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 34: launch_sweep3<T, 3, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 72: launch_sweep3<T, 7, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      default: break;
+    }
+  }
+  if (s.kind == 8 || s.kind == 9) {  // k_mic_sweep4, halo depth 4; cell ring 2 (8) or 4 (9)
+    switch (s.tk * 100 + s.kt * 10 + (s.kind - 8)) {
+      case 740: launch_sweep4<T, 7, 4, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 741: launch_sweep4<T, 7, 4, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 720: launch_sweep4<T, 7, 2, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 340: launch_sweep4<T, 3, 4, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 380: launch_sweep4<T, 3, 8, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 1520: launch_sweep4<T, 15, 2, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
       default: break;
     }
   }

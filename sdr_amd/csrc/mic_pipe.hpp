@@ -25,7 +25,7 @@ struct MicPipeArgs {
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
   int ncu;       // compute units of the device (bounds the tiles placed by XCD)
   int xcd;       // XCD placement of the tiles (0: none); set by launch_mic_pipe
-  // Backward solves of kinds 2, 4, 6 only: with dot_r set, the sweep also leaves one partial of
+  // Backward solves of kinds 2, 4, 6, 8, 9 only: with dot_r set, the sweep also leaves one partial of
   // dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in dot_partials.
   const T* dot_r;
   double* dot_partials;
@@ -33,7 +33,8 @@ struct MicPipeArgs {
 
 // kind 0: k_mic_solve tiles of 64*ti values of i by tk planes; kind 1: k_mic_solve2
 // tiles of 64 values of i by tk waves of kt planes; kinds 2, 4, 6: k_mic_sweep3 tiles
-// of 64 values of i by tk planes with an I/O wave (halo prefetch depth 8, 4, 2).  The factor (build) always runs the kind-0
+// of 64 values of i by tk planes with an I/O wave (halo prefetch depth 8, 4, 2); kinds
+// 8, 9: k_mic_sweep4 tiles of tk waves of kt planes with an I/O wave.  The factor (build) always runs the kind-0
 // kernel.
 struct MicPipeShape {
   int ti, tk, kt, kind, ntile_i, ntile_k, xcd;
