@@ -228,6 +228,40 @@ def solve_error(s, scene, rel_ref, rel_tol, f32):
     }
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One plain device-to-device copy of `nbytes` (HIP runtime copy kernel): under
+    rocprofv3 --pmc its FETCH_SIZE and WRITE_SIZE per dispatch read `nbytes`, which
+    gives the factor from counter units to bytes."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def pmc_traffic(kernel_region: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration), if it covers
+    this region; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    r = d.get("regions", {}).get(kernel_region)
+    if not r:
+        return None
+    return {"bytes_per_launch": r["bytes_per_launch"], "fetch_bytes": r["fetch_bytes"],
+            "write_bytes": r["write_bytes"], "source": "profiles/pmc_traffic.json"}
+
+
 def serving_run(count: int, slots: int) -> dict:
     """The serving form: `count` irrigation scenarios (seeded Poisson arrivals at 8/s,
     grids 32^3-128^3, needle angle +-30 deg, 1-6 ml/s, 0.5-2 ml) through the native
@@ -263,6 +297,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the 64-scenario serving run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 256 MiB device-to-device copy (the reference kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     ap.add_argument("--serve-count", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
@@ -277,6 +314,8 @@ def main() -> int:
     from sdr_amd import scenes
 
     ndev, arch = sdr_amd.device_info()
+    if args.copy_calibration:
+        copy_calibration(256 << 20)
     if ndev < 1:
         print("no HIP device", file=sys.stderr)
         return 3
@@ -307,8 +346,10 @@ def main() -> int:
         per_call_s = ms / calls / 1e3
         by = region_bytes(best, sz, nc, nc, npart, nsdf)
         ach = by / per_call_s / 1e9
+        tr = pmc_traffic(best)
         roof = {"bound": "hbm", "kernel": best, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": tr["bytes_per_launch"] if tr else None, "traffic_detail": tr,
                 "bytes_per_call": int(by), "avg_call_ms": round(per_call_s * 1e3, 4)}
     err = None
     if not args.no_error and smode == A.SOLVE_RELATIVE:
