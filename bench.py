@@ -32,6 +32,7 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
         "mic_fwd": nc * (3 * s + flags),        # read r, precon, flags; write q
         "mic_bwd": nc * (3 * s + flags),        # read q, precon, flags; write z
         "mic_bwd_dot": nc * (4 * s + flags),    # ... and read r for dot(z, r)
+        "mic_fwd_axpy": nc * (8 * s + flags),   # read r, As, p, s, precon, flags; write r, p, q
         "update_s": nc * 3 * s,                 # s = z + b s
         "mic_build": nc * (s + flags),          # read flags; write precon
         "spmv_dot": nc * (4 * s + flags),       # spmv: read s, flags, write As; dot: read s, As
@@ -153,7 +154,7 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
     return s, elapsed, iters, particle_steps, timing
 
 
-SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_bwd", "mic_bwd_dot", "spmv_dot",
+SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot", "spmv_dot",
                  "axpy_norm", "dot_update", "update_s", "gradient", "refine"]
 
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 
+#include "bufrsrc.hpp"
 #include "kernels.hpp"
 
 namespace sdr {
@@ -410,6 +411,88 @@ __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restri
     if (ao) out[ta] = va;
     if (bo) out[tb] = vb;
   });
+  if (partials) block_partial<0>(acc, partials);
+}
+
+// The same product over work units of 64 consecutive slots of a sheared row (unit q is
+// chunk q % nchunk of row q / nchunk).  Each wavefront takes a contiguous span of units,
+// and the spans of the workgroups that one XCD runs are contiguous too (workgroup b runs
+// on XCD b % 8), so the rows that a cell's z neighbours live in are mostly read by the
+// same XCD and its L2.  U units are in flight per wavefront step: the flags and all
+// seven x values of a lane are loaded before any is used (a neighbour outside the grid,
+// and every slot outside it, is addressed past the buffer: no load).  The arithmetic and
+// its order are those of k_spmv.
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __restrict__ flags,
+                                                    const T* __restrict__ x, T* __restrict__ out,
+                                                    const PcgScalars* sc, int it, double* partials,
+                                                    int span) {
+  if (sc->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
+  constexpr uint32_t SZ = sizeof(T);
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ);
+  const int nx = sh.nx, ny = sh.ny, nz = sh.nz, NJ = sh.NJ;
+  const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)NJ;
+  const int nchunk = (nx + 63) >> 6;
+  const int nunits = nz * NJ * nchunk;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int lb = (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
+  const int gw = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int q0 = gw * span;
+  const int q1 = q0 + span < nunits ? q0 + span : nunits;
+  double acc = 0.0;
+  for (int q = q0; q < q1; q += U) {
+    uint32_t et[U];
+    int ci[U], cj[U], ck[U];
+    uint16_t f[U];
+    T xc[U], xn[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int qq = q + u;
+      const int r = qq / nchunk;
+      const int i = (qq - r * nchunk) * 64 + lane;
+      const int J = r % NJ, k = r / NJ;
+      const int j = J - i;
+      const bool v = qq < q1 && i < nx && (unsigned)j < (unsigned)ny;
+      const uint32_t t = (uint32_t)r * row + (uint32_t)i;
+      et[u] = v ? t : kFarElem;
+      ci[u] = i, cj[u] = j, ck[u] = k;
+      f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, et[u] * 2u, 0, 0);
+      xc[u] = bld<T>(rx, et[u] * SZ);
+      xn[u][0] = bld<T>(rx, (v && i + 1 < nx ? t + row + 1 : kFarElem) * SZ);
+      xn[u][1] = bld<T>(rx, (v && i >= 1 ? t - row - 1 : kFarElem) * SZ);
+      xn[u][2] = bld<T>(rx, (v && j + 1 < ny ? t + row : kFarElem) * SZ);
+      xn[u][3] = bld<T>(rx, (v && j >= 1 ? t - row : kFarElem) * SZ);
+      xn[u][4] = bld<T>(rx, (v && k + 1 < nz ? t + plane : kFarElem) * SZ);
+      xn[u][5] = bld<T>(rx, (v && k >= 1 ? t - plane : kFarElem) * SZ);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint16_t fu = f[u];
+      const int i = ci[u], j = cj[u], k = ck[u];
+      T val = T(fl_diag(fu)) * xc[u];
+      // a neighbour inside the grid adds (-1 or +0) * x, one outside it adds nothing
+      const T a0 = val + ((fu & kFlPx) ? T(-1) : T(0)) * xn[u][0];
+      val = i + 1 < nx ? a0 : val;
+      const T a1 = val + ((fu & kFlMx) ? T(-1) : T(0)) * xn[u][1];
+      val = i >= 1 ? a1 : val;
+      const T a2 = val + ((fu & kFlPy) ? T(-1) : T(0)) * xn[u][2];
+      val = j + 1 < ny ? a2 : val;
+      const T a3 = val + ((fu & kFlMy) ? T(-1) : T(0)) * xn[u][3];
+      val = j >= 1 ? a3 : val;
+      const T a4 = val + ((fu & kFlPz) ? T(-1) : T(0)) * xn[u][4];
+      val = k + 1 < nz ? a4 : val;
+      const T a5 = val + ((fu & kFlMz) ? T(-1) : T(0)) * xn[u][5];
+      val = k >= 1 ? a5 : val;
+      const bool fl = (fu & kFlFluid) != 0;
+      const T res = fl ? val : T(0);
+      acc += fl ? (double)xc[u] * (double)res : 0.0;
+      bst<T>(ro, et[u] * SZ, res);
+    }
+  }
   if (partials) block_partial<0>(acc, partials);
 }
 
@@ -1221,7 +1304,17 @@ int cell_blocks(Shear sh) {
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
                  double* partials, hipStream_t st) {
-  k_spmv<T><<<cell_blocks(sh), 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
+  const int nb = cell_blocks(sh);
+  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+  if (sh.size() * (int64_t)sizeof(T) < ((int64_t)1 << 31) && nunits < ((int64_t)1 << 30)) {
+    // spans of whole U-unit steps, over the nb * 4 wavefronts
+    constexpr int U = 4;
+    const int64_t waves = (int64_t)nb * 4;
+    const int span = (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
+    k_spmv_units<T, U><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials, span);
+    return;
+  }
+  k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
 }
 int dot_blocks(Shear sh, bool exact) {
   if (exact) {

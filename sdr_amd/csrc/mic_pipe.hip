@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include "bufrsrc.hpp"
 #include "kernels.hpp"
 #include "mic_pipe.hpp"
 
@@ -347,59 +348,6 @@ done:;
 //   * halo traffic only in the waves on a tile face (wave-uniform branches); a wait that
 //     expires sets the error flag and the sweep runs on to its end, so no wave leaves
 //     the per-step barrier early and every wait after the flag is raised ends at once.
-typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-
-// Offset past every buffer's record count (record counts are kept below 2^31 bytes).
-constexpr uint32_t kOob = 0x80000000u;
-// Cache-policy bits: sc1 = agent-scope load (misses the XCD's own L2, which another
-// XCD's stores do not update); sc0|sc1 = system-scope store, written through the L2 so a
-// consumer on another XCD sees it at once instead of when the dirty line is evicted.
-constexpr int kAgent = 16;
-constexpr int kWriteThrough = 17;
-
-__device__ inline rsrc_t make_rsrc(const void* p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
-template <int AUX>
-__device__ inline uint64_t bld64(rsrc_t r, uint32_t off) {
-  const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX);
-  return (uint64_t)v.x | ((uint64_t)v.y << 32);
-}
-template <int AUX>
-__device__ inline void bst64(rsrc_t r, uint32_t off, uint64_t x) {
-  u32x2_t v;
-  v.x = (uint32_t)x;
-  v.y = (uint32_t)(x >> 32);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, AUX);
-}
-template <class T>
-__device__ inline T bld(rsrc_t r, uint32_t off) {
-  if constexpr (sizeof(T) == 8) {
-    const uint64_t b = bld64<0>(r, off);
-    T v;
-    __builtin_memcpy(&v, &b, 8);
-    return v;
-  } else {
-    const uint32_t b = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
-    T v;
-    __builtin_memcpy(&v, &b, 4);
-    return v;
-  }
-}
-template <class T>
-__device__ inline void bst(rsrc_t r, uint32_t off, T v) {
-  if constexpr (sizeof(T) == 8) {
-    uint64_t b;
-    __builtin_memcpy(&b, &v, 8);
-    bst64<0>(r, off, b);
-  } else {
-    uint32_t b;
-    __builtin_memcpy(&b, &v, 4);
-    __builtin_amdgcn_raw_buffer_store_b32(b, r, off, 0, 0);
-  }
-}
-
 template <int W>
 __device__ inline bool tags_ok(const uint64_t (&w)[W], uint32_t tag) {
   bool ok = true;
@@ -907,12 +855,17 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 //     slots, NJ * nx values): a lane's offset is (J * nx + i) * size, so rows before the
 //     plane (J < 0) wrap to huge unsigned offsets and rows past it exceed the record
 //     count, and the hardware drops them; lanes with i >= nx start at an offset that is
-//     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) hold
-//     flags 0, so they compute and store 0.  No bounds tests remain in the step.
+//     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) are
+//     loaded from a far offset (not fetched: half the sheared slots), read as flags 0,
+//     and compute and store 0; that select is the only bounds test in the step.
+// AXPY (forward solve): the wave first applies p += alpha s and r -= alpha As to its
+// cell (pressure.rs:371-375, r updated in place) and solves with the new r; the lanes'
+// max|r| (pressure.rs:377) leaves one partial per workgroup.  The update rides on a
+// latency-bound sweep instead of costing a pass of its own.
 // DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
-template <class T, bool REV, int TK, bool DOT, int PFH>
+template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY>
 __global__ void __launch_bounds__(64 * (TK + 1))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
@@ -920,7 +873,14 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   if (sc->done) return;
   constexpr int W = Words<T>::n;
   constexpr uint32_t SZ = sizeof(T);
-  constexpr int PF = 8;
+  // the cell ring: 3 loads and a store per step, 6 loads and 3 stores with AXPY (the
+  // wave's outstanding memory operations stay below 64)
+  constexpr int PF = AXPY ? 4 : 8;
+  static_assert(!(AXPY && REV) && !(AXPY && DOT), "AXPY is a forward-sweep option");
+  // the step loop is unrolled by U: ring slots are compile-time (u % PF for the cells,
+  // (u + 1) % PFH for the halo words) and the LDS parity of step s0 + u is u & 1
+  constexpr int U = PF > PFH ? PF : PFH;
+  static_assert(U % PF == 0 && U % PFH == 0 && U % 2 == 0, "ring depths must divide the unroll");
   // zbuf[.][w] for w < TK: compute wave w's values of a step; zbuf[.][TK]: the fetched z
   // halo row (the boundary wave's z neighbour)
   __shared__ T zbuf[2][TK + 1][64];
@@ -944,7 +904,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   // row of compute wave w at step s
   auto row = [&](int w, int s) { return REV ? (NJ - 1) - s + (TK - 1 - w) : s - w; };
   const int nsteps = NJ + TK - 1;  // compute steps; the I/O wave runs one more to publish
-  const int nsteps_pad = (nsteps + 1 + PF - 1) / PF * PF;
+  const int nsteps_pad = (nsteps + 1 + U - 1) / U * U;
   const int bwave = REV ? TK - 1 : 0;       // boundary wave: its z neighbour is in the tile before
   const int lwave = REV ? 0 : TK - 1;       // last wave: the tile after takes its z neighbour from it
   const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
@@ -960,7 +920,11 @@ __global__ void __launch_bounds__(64 * (TK + 1))
                  rp = make_rsrc(pbase, recs * SZ),
                  ri = make_rsrc(in + (int64_t)plane * NJ * nx, recs * SZ),
                  ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ),
-                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u);
+                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u),
+                 rap = make_rsrc(AXPY ? pa.ax_p + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u),
+                 ras = make_rsrc(AXPY ? pa.ax_s + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u),
+                 raq = make_rsrc(AXPY ? pa.ax_as + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u);
+    const T alpha = AXPY ? (T)sc->alpha : T(0);
     const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
     // element offset of (row J, lane i); lanes past the grid start out of range for good
     // (2^28 * size stays below 2^32 and above every plane's record count)
@@ -969,30 +933,46 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     uint32_t cur = lbase + (uint32_t)(J0 * nx);            // element offset of this step's row
     uint32_t nxt = lbase + (uint32_t)((J0 + PF * dJ) * nx);  // of the row PF steps ahead
     const uint32_t dstep = (uint32_t)(dJ * nx);
+    // j of the row PF steps ahead: rows outside the grid (j < 0 or j >= ny, half the
+    // sheared slots) are loaded from the far offset, i.e. not fetched at all
+    int jn = J0 + PF * dJ - i;
     uint32_t cf[PF];
-    T cp[PF], ci[PF], cr[PF];
+    T cp[PF], ci[PF], cr[PF], cap[PF], cas[PF], caq[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-      const uint32_t e = lbase + (uint32_t)((J0 + u * dJ) * nx);
+      const uint32_t e = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny ? lbase + (uint32_t)((J0 + u * dJ) * nx)
+                                                                       : kFarElem;
       cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
       if (DOT) cr[u] = bld<T>(rr, e * SZ);
+      if (AXPY) cap[u] = bld<T>(rap, e * SZ), cas[u] = bld<T>(ras, e * SZ), caq[u] = bld<T>(raq, e * SZ);
     }
-    double dacc = 0.0;
+    double dacc = 0.0;  // dot(z, r) (DOT) or max|r| (AXPY)
     zbuf[1][wk][lane] = T(0);
     T prev = T(0);
     __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
-    for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
+    for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        const int cb = u & 1, pb = cb ^ 1;
+      for (int uu = 0; uu < U; ++uu) {
+        const int u = uu % PF;
+        const int cb = uu & 1, pb = cb ^ 1;
         const T zin = zbuf[pb][zsrc][lane];
         const T xe = xlds[pb][wk];
         const T xs = wave_shift<REV>(prev);
         const T xin = lane == edge_in ? xe : xs;
         const uint32_t f = cf[u];
-        const T pcu = cp[u], inu = ci[u];
+        T inu = ci[u];
+        const T pcu = cp[u];
+        if (AXPY) {
+          // p += alpha s; r -= alpha As (pressure.rs:371-375), then max|r| (:377)
+          // slots outside the grid keep their p and r (the loads gave 0 there)
+          const uint32_t ec = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny ? cur : kFarElem;
+          bst<T>(rap, ec * SZ, cap[u] + alpha * cas[u]);
+          inu = inu - alpha * caq[u];
+          bst<T>(ri, ec * SZ, inu);
+          dacc = fmax(dacc, fabs((double)inu));
+        }
         const bool live = (f & kFlFluid) != 0;
         T res, pub;
         if (!REV) {
@@ -1014,12 +994,15 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         zbuf[cb][wk][lane] = pub;
         prev = pub;
         if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
-        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, nxt * 2u, 0, 0);
-        cp[u] = bld<T>(rp, nxt * SZ);
-        ci[u] = bld<T>(ri, nxt * SZ);
-        if (DOT) cr[u] = bld<T>(rr, nxt * SZ);
+        const uint32_t en = (unsigned)jn < (unsigned)sh.ny ? nxt : kFarElem;
+        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en * 2u, 0, 0);
+        cp[u] = bld<T>(rp, en * SZ);
+        ci[u] = bld<T>(ri, en * SZ);
+        if (DOT) cr[u] = bld<T>(rr, en * SZ);
+        if (AXPY) cap[u] = bld<T>(rap, en * SZ), cas[u] = bld<T>(ras, en * SZ), caq[u] = bld<T>(raq, en * SZ);
         cur += dstep;
         nxt += dstep;
+        jn += dJ;
         __syncthreads();
       }
     }
@@ -1027,10 +1010,14 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o);
       if (lane == 0) dsum[wk] = dacc;
     }
+    if (AXPY) {
+      for (int o = 32; o > 0; o >>= 1) dacc = fmax(dacc, __shfl_xor(dacc, o));
+      if (lane == 0) dsum[wk] = dacc;
+    }
     __syncthreads();  // matched by the I/O wave
-    if (DOT && wk == 0 && lane == 0) {
+    if ((DOT || AXPY) && wk == 0 && lane == 0) {
       double t = 0.0;
-      for (int w = 0; w < TK; ++w) t += dsum[w];
+      for (int w = 0; w < TK; ++w) t = DOT ? t + dsum[w] : fmax(t, dsum[w]);
       pa.dot_partials[blockIdx.x] = t;
     }
     return;
@@ -1142,9 +1129,9 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   take(0, 0, 1);  // inputs of step 0
   fill(0, PFH);
   __syncthreads();
-  for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
+  for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
-    for (int u = 0; u < PF; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       const int cb = u & 1, pb = cb ^ 1;
       // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
@@ -1462,12 +1449,18 @@ template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  if (op == kMicFwd && pa.ax_p)
+    k_mic_sweep3<T, false, TK, false, PFH, true><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+                                                                              pa);
+  else if (op == kMicFwd)
+    k_mic_sweep3<T, false, TK, false, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+                                                                               pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, true, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+                                                                             pa);
   else
-    k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, false, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+                                                                              pa);
 }
 
 template <class T, int OP, int TI, int TK>

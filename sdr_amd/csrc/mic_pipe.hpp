@@ -29,6 +29,12 @@ struct MicPipeArgs {
   // dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in dot_partials.
   const T* dot_r;
   double* dot_partials;
+  // Forward solves of kinds 2, 4, 6 only: with ax_p set, `in` is r and the sweep first applies
+  // p += alpha ax_s, r -= alpha ax_as (alpha from the PCG scalars), leaving one max|r|
+  // partial per workgroup in dot_partials.
+  T* ax_p;
+  const T* ax_s;
+  const T* ax_as;
 };
 
 // kind 0: k_mic_solve tiles of 64*ti values of i by tk planes; kind 1: k_mic_solve2

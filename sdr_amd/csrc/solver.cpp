@@ -197,6 +197,7 @@ struct Projector {
   uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
+  bool axpy_fuse = false;  // the forward sweep applies the PCG axpys (sweep kinds 2, 4, 6)
   int ncu = 0, dev = 0;  // compute units of the solver's device
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
@@ -212,6 +213,7 @@ struct Projector {
     // both precisions: 7-plane tiles with an I/O wave, halo prefetch depth 4 (k_mic_sweep3)
     mic_variant = 20;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
+    if (const char* e = getenv("SDR_MIC_AXPY")) axpy_fuse = atoi(e) != 0;
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -243,7 +245,15 @@ struct Projector {
   // The fast reduction mode fuses dot(z, r) into the backward solve (kind-2 sweeps only).
   bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind >= 2; }
 
-  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr) {
+  // the forward solve also applies the PCG axpys (kinds 2, 4, 6)
+  bool fused_axpy() const {
+    if (!axpy_fuse || !fused()) return false;
+    const int k = mic_pipe_shape(sh, mic_variant, ncu).kind;
+    return k == 2 || k == 4 || k == 6;
+  }
+
+  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr, T* ax_p = nullptr,
+                 const T* ax_s = nullptr, const T* ax_as = nullptr) {
     if (mic_variant < 0) {
       const int planes = sh.NJ + sh.nz - 1;
       for (int sgm = 0; sgm < planes; ++sgm)
@@ -261,7 +271,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, 0, dot_r, partials.as<double>()};
+                      (uint32_t)halo_z.bytes, ncu, 0, dot_r, partials.as<double>(), ax_p, ax_s, ax_as};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
@@ -329,12 +339,16 @@ struct Projector {
                          st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
         }
-        {
-          Scope sc1(*timer, "axpy_norm");
-          launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
-          launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
-        }
-        {
+        if (fused_axpy()) {
+          Scope sc1(*timer, "mic_fwd_axpy");
+          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, p.as<T>(), s.as<T>(), as.as<T>());
+          launch_dot_finish(partials.as<double>(), ndot, false, kOpResidual, sc(), tol, st);
+        } else {
+          {
+            Scope sc1(*timer, "axpy_norm");
+            launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+            launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
+          }
           Scope sc1(*timer, "mic_fwd");
           mic_sweep(kMicFwd, r, q.as<T>());
         }

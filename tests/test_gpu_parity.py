@@ -335,6 +335,34 @@ def test_mic_multiplane_sweeps_bit_exact(dev, orc, variant, shape, monkeypatch):
         assert bits_equal(a, b)
 
 
+@pytest.mark.parametrize("axpy", ["0", "1"])
+@pytest.mark.parametrize("shape", [(150, 37, 41), (64, 64, 64)])
+def test_fused_fast_solve_matches_unfused(dev, shape, axpy, monkeypatch):
+    """Fast reduction mode with the default sweeps (dot(z, r) fused into the backward
+    sweep; with SDR_MIC_AXPY=1 the axpys also fused into the forward one) against the
+    unfused kernels (variant 0): the same relative residual, and pressures equal to the
+    solve's tolerance."""
+    monkeypatch.setenv("SDR_MIC_AXPY", axpy)
+    nx, ny, nz = shape
+    g = np.random.default_rng(3)
+    r = g.random((nz, ny, nx))
+    cells = np.where(r < 0.85, 1, np.where(r < 0.93, 0, 2)).astype(np.uint8)
+    faces = (g.standard_normal((nz, ny, nx + 1)), g.standard_normal((nz, ny + 1, nx)),
+             g.standard_normal((nz + 1, ny, nx)))
+    out = {}
+    for variant in ("20", "0"):
+        monkeypatch.setenv("SDR_MIC_VARIANT", variant)
+        u, v, w = (a.copy() for a in faces)
+        rep, p = project_capturing_with_source(nx, ny, nz, 0.01, (0, 0, 0), u, v, w, cells, 1000.0, 1e-3,
+                                               SolveParams(5000, 0.0), dot_mode=A.DOT_FAST,
+                                               solve_mode=A.SOLVE_RELATIVE, refine_tol=1e-10)
+        out[variant] = (rep, p)
+    (r20, p20), (r0, p0) = out["20"], out["0"]
+    assert abs(r20.iters - r0.iters) <= max(3, r0.iters // 20), (r20.iters, r0.iters)
+    scale = np.abs(p0).max()
+    assert np.abs(p20 - p0).max() <= 1e-8 * scale
+
+
 def _random_clumps(solver, seed, lo=0, hi=16):
     """Particles with a random count in [lo, hi] per cavity cell, at random offsets inside
     the cell and in shuffled reference order, so sources and receivers interleave and
