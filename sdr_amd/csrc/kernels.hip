@@ -422,6 +422,48 @@ __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restri
 // seven x values of a lane are loaded before any is used (a neighbour outside the grid,
 // and every slot outside it, is addressed past the buffer: no load).  The arithmetic and
 // its order are those of k_spmv.
+// Work units of the vector kernels: unit q is the 64 slots of chunk q % nchunk of sheared
+// row q / nchunk.  unit_span gives the calling wavefront its contiguous span [q0, q1)
+// (the spans of workgroups on one XCD being contiguous too) and unit_slot the slot of
+// this lane in unit qq, or kFarElem outside the grid or the span.
+struct UnitLane {
+  uint32_t t;  // slot, or kFarElem
+  int i, j, k;
+  bool v;
+};
+__device__ inline void unit_span(Shear sh, int span, int* q0, int* q1) {
+  const int nunits = sh.nz * sh.NJ * ((sh.nx + 63) >> 6);
+  const int G = gridDim.x, b = blockIdx.x;
+  const int lb = (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
+  const int gw = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  *q0 = gw * span;
+  *q1 = *q0 + span < nunits ? *q0 + span : nunits;
+}
+__device__ inline UnitLane unit_slot(Shear sh, int qq, int q1) {
+  const int nchunk = (sh.nx + 63) >> 6;
+  UnitLane L;
+  const int r = qq / nchunk;
+  L.i = (qq - r * nchunk) * 64 + (threadIdx.x & 63);
+  const int J = r % sh.NJ;
+  L.k = r / sh.NJ;
+  L.j = J - L.i;
+  L.v = qq < q1 && L.i < sh.nx && (unsigned)L.j < (unsigned)sh.ny;
+  L.t = L.v ? (uint32_t)r * (uint32_t)sh.nx + (uint32_t)L.i : kFarElem;
+  return L;
+}
+// unit kernels address with 32-bit byte offsets
+template <class T>
+bool units_fit(Shear sh) {
+  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+  return sh.size() * (int64_t)sizeof(T) < ((int64_t)1 << 31) && nunits < ((int64_t)1 << 30);
+}
+// span of whole U-unit steps over nblocks * 4 wavefronts
+inline int unit_span_len(Shear sh, int nblocks, int U) {
+  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+  const int64_t waves = (int64_t)nblocks * 4;
+  return (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
+}
+
 template <class T, int U>
 __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __restrict__ flags,
                                                     const T* __restrict__ x, T* __restrict__ out,
@@ -432,36 +474,23 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
   const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ);
-  const int nx = sh.nx, ny = sh.ny, nz = sh.nz, NJ = sh.NJ;
-  const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)NJ;
-  const int nchunk = (nx + 63) >> 6;
-  const int nunits = nz * NJ * nchunk;
-  const int G = gridDim.x;
-  const int b = blockIdx.x;
-  const int lb = (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
-  const int gw = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int q0 = gw * span;
-  const int q1 = q0 + span < nunits ? q0 + span : nunits;
+  const int nx = sh.nx, ny = sh.ny, nz = sh.nz;
+  const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)sh.NJ;
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
   double acc = 0.0;
   for (int q = q0; q < q1; q += U) {
-    uint32_t et[U];
-    int ci[U], cj[U], ck[U];
+    UnitLane L[U];
     uint16_t f[U];
     T xc[U], xn[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int qq = q + u;
-      const int r = qq / nchunk;
-      const int i = (qq - r * nchunk) * 64 + lane;
-      const int J = r % NJ, k = r / NJ;
-      const int j = J - i;
-      const bool v = qq < q1 && i < nx && (unsigned)j < (unsigned)ny;
-      const uint32_t t = (uint32_t)r * row + (uint32_t)i;
-      et[u] = v ? t : kFarElem;
-      ci[u] = i, cj[u] = j, ck[u] = k;
-      f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, et[u] * 2u, 0, 0);
-      xc[u] = bld<T>(rx, et[u] * SZ);
+      L[u] = unit_slot(sh, q + u, q1);
+      const uint32_t t = L[u].t;
+      const int i = L[u].i, j = L[u].j, k = L[u].k;
+      const bool v = L[u].v;
+      f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, t * 2u, 0, 0);
+      xc[u] = bld<T>(rx, t * SZ);
       xn[u][0] = bld<T>(rx, (v && i + 1 < nx ? t + row + 1 : kFarElem) * SZ);
       xn[u][1] = bld<T>(rx, (v && i >= 1 ? t - row - 1 : kFarElem) * SZ);
       xn[u][2] = bld<T>(rx, (v && j + 1 < ny ? t + row : kFarElem) * SZ);
@@ -472,7 +501,7 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint16_t fu = f[u];
-      const int i = ci[u], j = cj[u], k = ck[u];
+      const int i = L[u].i, j = L[u].j, k = L[u].k;
       T val = T(fl_diag(fu)) * xc[u];
       // a neighbour inside the grid adds (-1 or +0) * x, one outside it adds nothing
       const T a0 = val + ((fu & kFlPx) ? T(-1) : T(0)) * xn[u][0];
@@ -490,7 +519,7 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
       const bool fl = (fu & kFlFluid) != 0;
       const T res = fl ? val : T(0);
       acc += fl ? (double)xc[u] * (double)res : 0.0;
-      bst<T>(ro, et[u] * SZ, res);
+      bst<T>(ro, L[u].t * SZ, res);
     }
   }
   if (partials) block_partial<0>(acc, partials);
@@ -662,6 +691,67 @@ __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T
       m = fmax(m, fabs((double)rv));
     }
   });
+  block_partial<1>(m, partials);
+}
+
+// k_update_s and k_axpy_pr2 over work units (the addressing of k_spmv_units): the
+// loads of U units are in flight before the first store.
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_update_s_units(Shear sh, T* __restrict__ s, const T* __restrict__ z,
+                                                        const PcgScalars* sc, int span) {
+  if (sc->done) return;
+  constexpr uint32_t SZ = sizeof(T);
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rs = make_rsrc(s, n * SZ), rz = make_rsrc(z, n * SZ);
+  const T beta = (T)sc->beta;
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    T zv[U], sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      zv[u] = bld<T>(rz, t[u] * SZ);
+      sv[u] = bld<T>(rs, t[u] * SZ);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) bst<T>(rs, t[u] * SZ, zv[u] + beta * sv[u]);
+  }
+}
+
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__ p, T* __restrict__ r,
+                                                       const T* __restrict__ s, const T* __restrict__ as,
+                                                       const PcgScalars* sc, double* partials, int span) {
+  if (sc->done) return;
+  constexpr uint32_t SZ = sizeof(T);
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
+               rq = make_rsrc(as, n * SZ);
+  const T alpha = (T)sc->alpha;
+  double m = 0.0;
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    T pv[U], sv[U], rv[U], qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      pv[u] = bld<T>(rp, t[u] * SZ);
+      sv[u] = bld<T>(rs, t[u] * SZ);
+      rv[u] = bld<T>(rr, t[u] * SZ);
+      qv[u] = bld<T>(rq, t[u] * SZ);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bst<T>(rp, t[u] * SZ, pv[u] + alpha * sv[u]);
+      const T nr = rv[u] - alpha * qv[u];
+      bst<T>(rr, t[u] * SZ, nr);
+      m = fmax(m, fabs((double)nr));  // 0 outside the grid
+    }
+  }
   block_partial<1>(m, partials);
 }
 
@@ -1305,13 +1395,8 @@ template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
                  double* partials, hipStream_t st) {
   const int nb = cell_blocks(sh);
-  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
-  if (sh.size() * (int64_t)sizeof(T) < ((int64_t)1 << 31) && nunits < ((int64_t)1 << 30)) {
-    // spans of whole U-unit steps, over the nb * 4 wavefronts
-    constexpr int U = 4;
-    const int64_t waves = (int64_t)nb * 4;
-    const int span = (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
-    k_spmv_units<T, U><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials, span);
+  if (units_fit<T>(sh)) {
+    k_spmv_units<T, 4><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials, unit_span_len(sh, nb, 4));
     return;
   }
   k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
@@ -1344,11 +1429,20 @@ void launch_maxabs_partials(Shear sh, const T* a, double* partials, int nblocks,
 template <class T>
 void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScalars* sc,
                     double* partials, int nblocks, hipStream_t st) {
+  if (units_fit<T>(sh)) {
+    k_axpy_pr_units<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials, unit_span_len(sh, nblocks, 4));
+    return;
+  }
   k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials);
 }
 template <class T>
 void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st) {
-  k_update_s<T><<<cell_blocks(sh), 256, 0, st>>>(sh, s, z, sc);
+  const int nb = cell_blocks(sh);
+  if (units_fit<T>(sh)) {
+    k_update_s_units<T, 4><<<nb, 256, 0, st>>>(sh, s, z, sc, unit_span_len(sh, nb, 4));
+    return;
+  }
+  k_update_s<T><<<nb, 256, 0, st>>>(sh, s, z, sc);
 }
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,

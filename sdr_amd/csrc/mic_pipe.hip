@@ -947,6 +947,9 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       ci[u] = bld<T>(ri, e * SZ);
       if (DOT) cr[u] = bld<T>(rr, e * SZ);
       if (AXPY) cap[u] = bld<T>(rap, e * SZ), cas[u] = bld<T>(ras, e * SZ), caq[u] = bld<T>(raq, e * SZ);
+      // issue slot by slot, in the order of the step loop: the wait for a slot's loads
+      // at the top of the loop then also holds for the ones issued here
+      __builtin_amdgcn_sched_barrier(0);
     }
     double dacc = 0.0;  // dot(z, r) (DOT) or max|r| (AXPY)
     zbuf[1][wk][lane] = T(0);
@@ -1004,6 +1007,9 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         nxt += dstep;
         jn += dJ;
         __syncthreads();
+        // keep each step's instructions in its step: hoisted into an earlier step, the
+        // use of a ring slot makes the wave wait there for the newest loads in flight
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     if (DOT) {
@@ -1075,13 +1081,13 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
     }
   };
-  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return o == kOob || tags_ok<W>(w, tag); };
+  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return (o == kOob) | tags_ok<W>(w, tag); };
   // Take the inputs of target step t from ring slot u into LDS buffer b.  A stale word
   // means the producer is behind the prefetch window: wait for the window's newest word
   // (rows are published in order), then reload the whole window.
   auto take = [&](int u, int t, int b) {
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
-    const bool ok = ready(hz[u], oz) && ready(hx[u], ox);
+    const bool ok = ready(hz[u], oz) & ready(hx[u], ox);  // no branch between the waits
     if (__ballot(!ok) != 0ull) {
       const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
       uint64_t wz[W], wx[W];
@@ -1098,48 +1104,65 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
     if (lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
   };
-  // Publish the values compute waves wrote at step t into LDS buffer b.
+  // Publish the values compute waves wrote at step t into LDS buffer b.  The stores are
+  // issued on every step (to an out-of-range offset when there is nothing to publish):
+  // with a fixed number of memory operations per step, the wait for a prefetched halo
+  // word counts exactly the operations issued after it, and never waits for this
+  // step's write-through stores.
   auto publish = [&](int t, int b) {
-    if (z_out && zl_ok) {
+    {
       const int R = row(lwave, t);
-      if ((unsigned)R < (unsigned)NJ) {
-        const T v = zbuf[b][lwave][lane];
-        uint32_t wv[W];
-        __builtin_memcpy(wv, &v, sizeof(T));
-        const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
+      const T v = zbuf[b][lwave][lane];
+      uint32_t wv[W];
+      __builtin_memcpy(wv, &v, sizeof(T));
+      const uint32_t o = z_out && zl_ok && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
 #pragma unroll
-        for (int q = 0; q < W; ++q)
-          bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
-      }
+      for (int q = 0; q < W; ++q)
+        bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
-    if (x_out) {
+    {
       const int R = row(lane < TK ? lane : 0, t);
       const T v = zbuf[b][lane < TK ? lane : 0][edge_out];
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+      const uint32_t o = x_out && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
 #pragma unroll
-      for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
+      for (int q = 0; q < W; ++q)
+        bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
   };
   zbuf[0][TK][lane] = T(0);
   if (lane < TK) xlds[0][lane] = T(0);
+  // The prologue issues the memory operations of steps -PFH-1 .. -1 in the order of the
+  // step loop (the stores of those steps have no row, so nothing is stored): a wait
+  // at the top of the loop, which holds for every way into it, is then the same as in
+  // the steady state.
+  fill(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int u = 0; u < PFH; ++u) fill(u, u);
+  for (int u = 1; u < PFH; ++u) {
+    publish(-1, 0);
+    fill(u, u);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  publish(-1, 0);
   take(0, 0, 1);  // inputs of step 0
   fill(0, PFH);
   __syncthreads();
+  __builtin_amdgcn_sched_barrier(0);
   for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       const int cb = u & 1, pb = cb ^ 1;
-      // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
-      if (s > 0) publish(s - 1, pb);
+      // values of step s - 1 (in buffer pb; step -1 has no row in the grid, so nothing
+      // is stored), inputs of step s + 1 (into buffer cb)
+      publish(s - 1, pb);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
       fill(un, s + 1 + PFH);
       __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   __syncthreads();  // matches the compute waves' final barrier
