@@ -865,7 +865,7 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 // DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
-template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY>
+template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY, int PFC>
 __global__ void __launch_bounds__(64 * (TK + 1))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
@@ -873,9 +873,11 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   if (sc->done) return;
   constexpr int W = Words<T>::n;
   constexpr uint32_t SZ = sizeof(T);
-  // the cell ring: 3 loads and a store per step, 6 loads and 3 stores with AXPY (the
-  // wave's outstanding memory operations stay below 64)
-  constexpr int PF = AXPY ? 4 : 8;
+  // the cell ring, PFC steps deep: 3 loads and a store per step (4 and a store with
+  // DOT), 6 loads and 3 stores with AXPY; a step waits for the loads of PFC steps
+  // before, so a deeper ring hides more memory latency, up to the wave's 63
+  // outstanding memory operations
+  constexpr int PF = AXPY ? 4 : PFC;
   static_assert(!(AXPY && REV) && !(AXPY && DOT), "AXPY is a forward-sweep option");
   // the step loop is unrolled by U: ring slots are compile-time (u % PF for the cells,
   // (u + 1) % PFH for the halo words) and the LDS parity of step s0 + u is u & 1
@@ -1261,7 +1263,10 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       }
     };
 #pragma unroll
-    for (int u = 0; u < PF; ++u) load(u, J0 + u * dJ);
+    for (int u = 0; u < PF; ++u) {
+      load(u, J0 + u * dJ);
+      __builtin_amdgcn_sched_barrier(0);  // slot by slot, in the order of the step loop
+    }
     int J = J0;
     double dacc = 0.0;
     zbuf[1][wk][lane] = T(0);
@@ -1316,6 +1321,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         load(u, J + PF * dJ);
         J += dJ;
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);  // each step's instructions stay in the step
       }
     }
     if (DOT) {
@@ -1383,13 +1389,13 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
     }
   };
-  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return o == kOob || tags_ok<W>(w, tag); };
+  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return (o == kOob) | tags_ok<W>(w, tag); };
   // Take the inputs of target step t from ring slot u into LDS buffer b.  A stale word
   // means the producer is behind the prefetch window: wait for the window's newest word
   // (rows are published in order), then reload the whole window.
   auto take = [&](int u, int t, int b) {
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
-    const bool ok = ready(hz[u], oz) && ready(hx[u], ox);
+    const bool ok = ready(hz[u], oz) & ready(hx[u], ox);
     if (__ballot(!ok) != 0ull) {
       const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
       uint64_t wz[W], wx[W];
@@ -1406,48 +1412,59 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
     if (lane < P) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
   };
-  // Publish the values compute waves wrote at step t into LDS buffer b.
+  // Publish the values compute waves wrote at step t into LDS buffer b (the stores are
+  // issued on every step, out of range when there is nothing to publish, as in
+  // k_mic_sweep3).
   auto publish = [&](int t, int b) {
-    if (z_out && zl_ok) {
+    {
       const int R = row(lplane, t);
-      if ((unsigned)R < (unsigned)NJ) {
-        const T v = zbuf[b][lwave][lane];
-        uint32_t wv[W];
-        __builtin_memcpy(wv, &v, sizeof(T));
-        const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
+      const T v = zbuf[b][lwave][lane];
+      uint32_t wv[W];
+      __builtin_memcpy(wv, &v, sizeof(T));
+      const uint32_t o = z_out && zl_ok && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
 #pragma unroll
-        for (int q = 0; q < W; ++q)
-          bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
-      }
+      for (int q = 0; q < W; ++q)
+        bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
-    if (x_out) {
+    {
       const int R = row(lane < P ? lane : 0, t);
       const T v = edgev[b][lane < P ? lane : 0];
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+      const uint32_t o = x_out && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
 #pragma unroll
-      for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
+      for (int q = 0; q < W; ++q)
+        bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
   };
   zbuf[0][TK][lane] = T(0);
   if (lane < P) xlds[0][lane] = T(0);
+  // steps -PFH-1 .. -1 in the order of the step loop (see k_mic_sweep3)
+  fill(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int u = 0; u < PFH; ++u) fill(u, u);
+  for (int u = 1; u < PFH; ++u) {
+    publish(-1, 0);
+    fill(u, u);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  publish(-1, 0);
   take(0, 0, 1);  // inputs of step 0
   fill(0, PFH);
   __syncthreads();
+  __builtin_amdgcn_sched_barrier(0);
   for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       const int cb = u & 1, pb = cb ^ 1;
       // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
-      if (s > 0) publish(s - 1, pb);
+      publish(s - 1, pb);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
       fill(un, s + 1 + PFH);
       __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   __syncthreads();  // matches the compute waves' final barrier
@@ -1468,21 +1485,21 @@ void launch_sweep4(int op, Shear sh, const uint16_t* flags, const T* precon, con
                                                                                 sc, pa);
 }
 
-template <class T, int TK, int PFH>
+template <class T, int TK, int PFH, int PFC = 8>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
   if (op == kMicFwd && pa.ax_p)
-    k_mic_sweep3<T, false, TK, false, PFH, true><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, false, TK, false, PFH, true, 4><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                               pa);
   else if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, false, TK, false, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                                pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, true, TK, true, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                              pa);
   else
-    k_mic_sweep3<T, true, TK, false, PFH, false><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, true, TK, false, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                               pa);
 }
 
@@ -1510,13 +1527,14 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 
 // Tile shapes.  Variants 0-5: k_mic_solve tiles of 64*TI values of i by TK planes.
 // Variants 6-16: k_mic_solve2 tiles of 64 values of i by TK waves of KT planes, with the
-// XCD placement (x) where the tile grid allows it; 17+: k_mic_sweep3 tiles of 64
+// XCD placement (x) where the tile grid allows it; 17-23: k_mic_sweep3 tiles of 64
 // values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
-// (kind 4) or 2 (kind 6); 24+: k_mic_sweep4 tiles of TK waves of KT planes (plus the
-// I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2) or 4 (kind 9).
+// (kind 4) or 2 (kind 6), cell ring depth 8; 24-29: k_mic_sweep4 tiles of TK waves of
+// KT planes (plus the I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2)
+// or 4 (kind 9); 30+: k_mic_sweep3 with deeper cell rings (pf).
 namespace {
 struct VariantRow {
-  int ti, tk, kt, kind, xcd;
+  int ti, tk, kt, kind, xcd, pf;  // pf: k_mic_sweep3's cell ring depth (0: 8)
 };
 const VariantRow kVariants[] = {
     {1, 16, 1, 0, 0}, {1, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 4, 1, 0, 0}, {1, 4, 1, 0, 0},
@@ -1526,6 +1544,7 @@ const VariantRow kVariants[] = {
     {1, 7, 1, 4, 0},  {1, 7, 1, 6, 0},  {1, 15, 1, 4, 0}, {1, 3, 1, 4, 0},
     {1, 7, 4, 8, 0},  {1, 7, 2, 8, 0},  {1, 3, 4, 8, 0},  {1, 7, 4, 9, 0}, {1, 3, 8, 8, 0},
     {1, 15, 2, 8, 0},
+    {1, 7, 1, 4, 0, 12}, {1, 7, 1, 4, 0, 16}, {1, 3, 1, 4, 0, 16}, {1, 15, 1, 4, 0, 12}, {1, 7, 1, 6, 0, 12},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
@@ -1542,6 +1561,7 @@ MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
   s.tk = r.tk;
   s.kt = r.kt;
   s.kind = r.kind;
+  s.pf = r.pf > 0 ? r.pf : 8;
   const int planes = r.tk * r.kt;
   s.ntile_i = (sh.nx + 64 * s.ti - 1) / (64 * s.ti);
   s.ntile_k = (sh.nz + planes - 1) / planes;
@@ -1559,7 +1579,15 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
   pa.xcd = s.xcd;
   if (s.kind == 2 || s.kind == 4 || s.kind == 6) {
     const int pfh = s.kind == 2 ? 8 : (s.kind == 4 ? 4 : 2);
-    switch (s.tk * 10 + pfh) {
+    switch (s.tk * 1000 + s.pf * 10 + pfh) {
+      case 7124: launch_sweep3<T, 7, 4, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 7164: launch_sweep3<T, 7, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 3164: launch_sweep3<T, 3, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 15124: launch_sweep3<T, 15, 4, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 7122: launch_sweep3<T, 7, 2, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      default: break;
+    }
+    if (s.pf == 8) switch (s.tk * 10 + pfh) {
       case 158: launch_sweep3<T, 15, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 78: launch_sweep3<T, 7, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 38: launch_sweep3<T, 3, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;

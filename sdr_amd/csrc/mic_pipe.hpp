@@ -44,6 +44,7 @@ struct MicPipeArgs {
 // kernel.
 struct MicPipeShape {
   int ti, tk, kt, kind, ntile_i, ntile_k, xcd;
+  int pf;  // cell ring depth of the k_mic_sweep3 kinds
 };
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu);
