@@ -930,20 +930,24 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
     // element offset of (row J, lane i); lanes past the grid start out of range for good
     // (2^28 * size stays below 2^32 and above every plane's record count)
-    const uint32_t lbase = i < nx ? (uint32_t)i : 0x10000000u;
+    const uint32_t lbase = i < nx ? (uint32_t)i : kFarElem;
     const int J0 = row(wk, 0);
-    uint32_t cur = lbase + (uint32_t)(J0 * nx);            // element offset of this step's row
-    uint32_t nxt = lbase + (uint32_t)((J0 + PF * dJ) * nx);  // of the row PF steps ahead
-    const uint32_t dstep = (uint32_t)(dJ * nx);
-    // j of the row PF steps ahead: rows outside the grid (j < 0 or j >= ny, half the
-    // sheared slots) are loaded from the far offset, i.e. not fetched at all
+    // byte offsets (mod 2^32, advanced by a constant per step) of this step's row (oc)
+    // and of the row PF steps ahead (of2 in the flags, ofn in the values)
+    uint32_t oc = (lbase + (uint32_t)(J0 * nx)) * SZ;
+    uint32_t of2 = (lbase + (uint32_t)((J0 + PF * dJ) * nx)) * 2u;
+    uint32_t ofn = (lbase + (uint32_t)((J0 + PF * dJ) * nx)) * SZ;
+    const uint32_t d2 = (uint32_t)(dJ * nx) * 2u, dS = (uint32_t)(dJ * nx) * SZ;
+    // j of the row PF steps ahead: slots outside the grid (j < 0 or j >= ny, half the
+    // sheared slots) are not loaded (with AXPY not stored either, so they keep their p
+    // and r and add nothing to max|r|)
     int jn = J0 + PF * dJ - i;
     uint32_t cf[PF];
     T cp[PF], ci[PF], cr[PF], cap[PF], cas[PF], caq[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-      const uint32_t e = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny ? lbase + (uint32_t)((J0 + u * dJ) * nx)
-                                                                       : kFarElem;
+      const bool in = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny;
+      const uint32_t e = in ? lbase + (uint32_t)((J0 + u * dJ) * nx) : kFarElem;
       cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
@@ -969,13 +973,11 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         const uint32_t f = cf[u];
         T inu = ci[u];
         const T pcu = cp[u];
+        T np = T(0);
         if (AXPY) {
           // p += alpha s; r -= alpha As (pressure.rs:371-375), then max|r| (:377)
-          // slots outside the grid keep their p and r (the loads gave 0 there)
-          const uint32_t ec = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny ? cur : kFarElem;
-          bst<T>(rap, ec * SZ, cap[u] + alpha * cas[u]);
+          np = cap[u] + alpha * cas[u];
           inu = inu - alpha * caq[u];
-          bst<T>(ri, ec * SZ, inu);
           dacc = fmax(dacc, fabs((double)inu));
         }
         const bool live = (f & kFlFluid) != 0;
@@ -995,18 +997,28 @@ __global__ void __launch_bounds__(64 * (TK + 1))
           res = live ? z : T(0);
           pub = res;
         }
-        bst<T>(ro, cur * SZ, res);
+        // the value the next step's waves wait for goes to LDS first; the memory
+        // operations are issued while that write completes
         zbuf[cb][wk][lane] = pub;
+        __builtin_amdgcn_sched_barrier(0);
         prev = pub;
+        if (AXPY) {
+          const uint32_t ec = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny ? oc : kFarElem * SZ;
+          bst<T>(rap, ec, np);
+          bst<T>(ri, ec, inu);
+        }
+        bst<T>(ro, oc, res);
         if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
-        const uint32_t en = (unsigned)jn < (unsigned)sh.ny ? nxt : kFarElem;
-        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en * 2u, 0, 0);
-        cp[u] = bld<T>(rp, en * SZ);
-        ci[u] = bld<T>(ri, en * SZ);
-        if (DOT) cr[u] = bld<T>(rr, en * SZ);
-        if (AXPY) cap[u] = bld<T>(rap, en * SZ), cas[u] = bld<T>(ras, en * SZ), caq[u] = bld<T>(raq, en * SZ);
-        cur += dstep;
-        nxt += dstep;
+        const bool inn = (unsigned)jn < (unsigned)sh.ny;
+        const uint32_t en2 = inn ? of2 : kFarElem * 2u, enS = inn ? ofn : kFarElem * SZ;
+        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
+        cp[u] = bld<T>(rp, enS);
+        ci[u] = bld<T>(ri, enS);
+        if (DOT) cr[u] = bld<T>(rr, enS);
+        if (AXPY) cap[u] = bld<T>(rap, enS), cas[u] = bld<T>(ras, enS), caq[u] = bld<T>(raq, enS);
+        oc += dS;
+        of2 += d2;
+        ofn += dS;
         jn += dJ;
         __syncthreads();
         // keep each step's instructions in its step: hoisted into an earlier step, the
