@@ -111,9 +111,15 @@ def cpu_baseline(scene, sdf, gpu_iters: float, budget_s: float, threads: int):
     }
 
 
-def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, label):
+def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, label, regions=True,
+             breakdown_steps=0):
     """Seed the workload, run warmup + timed substeps; returns (solver, elapsed, iters,
-    particle-substeps, timing regions)."""
+    particle-substeps, sweep timing, breakdown).
+
+    The timed substeps carry HIP events around the MIC(0) sweeps only (the roofline
+    kernel; events around every region cost ~4% of a substep).  `breakdown_steps` more
+    substeps follow, untimed, with events around every region: their per-region times
+    and PCG iterations are the breakdown."""
     from sdr_amd import _abi as A
     from sdr_amd import scenes
     from sdr_amd.solver import SolveParams
@@ -133,7 +139,7 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
         s.synchronize()
         print(f"[bench] {label} warmup {k}: {time.perf_counter() - t:.3f}s iters {s.last_pcg_iters}",
               file=sys.stderr, flush=True)
-    s.enable_timing(True)
+    s.enable_timing(regions, sweeps_only=True)
     s.synchronize()
     particle_steps = 0
     iters = []
@@ -151,7 +157,19 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
     elapsed = time.perf_counter() - t_start
     timing = s.timing()
     s.enable_timing(False)
-    return s, elapsed, iters, particle_steps, timing
+    breakdown = None
+    if breakdown_steps > 0:
+        s.enable_timing(True)
+        bi = []
+        for k in range(breakdown_steps):
+            s.substep(dt)
+            bi.append(s.last_pcg_iters)
+        s.synchronize()
+        breakdown = {"steps": breakdown_steps, "pcg_iters": bi, "regions": s.timing()}
+        s.enable_timing(False)
+        print(f"[bench] {label} breakdown: {breakdown_steps} more substeps, pcg iters {bi}",
+              file=sys.stderr, flush=True)
+    return s, elapsed, iters, particle_steps, timing, breakdown
 
 
 SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot", "spmv_dot",
@@ -298,6 +316,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the 64-scenario serving run")
+    ap.add_argument("--no-regions", action="store_true",
+                    help="time the substeps without the per-region HIP events (no roofline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -324,16 +344,18 @@ def main() -> int:
     prec = A.PRECISION_F32 if args.precision == "f32" else A.PRECISION_F64
     dot_mode = A.DOT_FAST if args.dot == "fast" else A.DOT_EXACT
     smode = A.SOLVE_RELATIVE if args.solve == "relative" else A.SOLVE_REFERENCE
-    s, elapsed, iters, particle_steps, timing = run_path(
+    s, elapsed, iters, particle_steps, timing, bd = run_path(
         scene, prec, dot_mode, smode, args.tol, args.max_iters, args.steps, args.warmup,
-        args.precision)
+        args.precision, regions=not args.no_regions, breakdown_steps=0 if args.no_regions else args.steps)
     ms_per_step = elapsed * 1e3 / args.steps
     value = particle_steps / elapsed
     sz = 4 if prec == A.PRECISION_F32 else 8
     nc = s.nx * s.ny * s.nz
     nsdf = scene.sdf.nx * scene.sdf.ny * scene.sdf.nz
     npart = particle_steps / max(args.steps, 1)
-    pressure_ms = sum(timing.get(r, (0.0, 0))[0] for r in SOLVE_REGIONS) / args.steps
+    bd_regions = bd["regions"] if bd else {}
+    bd_steps = bd["steps"] if bd else 1
+    pressure_ms = sum(bd_regions.get(r, (0.0, 0))[0] for r in SOLVE_REGIONS) / bd_steps
     # Dominant kernel region by total time; roofline from its average call duration.
     best, best_t = None, -1.0
     for name, (ms, calls) in timing.items():
@@ -361,14 +383,16 @@ def main() -> int:
     s.close()
     fp64 = None
     if not args.no_fp64 and prec == A.PRECISION_F32:
-        s64, el64, it64, ps64, tm64 = run_path(scene, A.PRECISION_F64, dot_mode, smode, args.tol,
-                                               args.max_iters, args.steps, args.warmup, "f64")
+        s64, el64, it64, ps64, tm64, bd64 = run_path(scene, A.PRECISION_F64, dot_mode, smode, args.tol,
+                                                     args.max_iters, args.steps, args.warmup, "f64",
+                                                     breakdown_steps=args.steps)
         s64.close()
         fp64 = {"ms_per_step": round(el64 * 1e3 / args.steps, 3),
                 "value": round(ps64 / el64, 1),
-                "pressure_solve_ms": round(sum(tm64.get(r, (0.0, 0))[0] for r in SOLVE_REGIONS)
-                                           / args.steps, 3),
+                "pressure_solve_ms": round(sum(bd64["regions"].get(r, (0.0, 0))[0] for r in SOLVE_REGIONS)
+                                           / bd64["steps"], 3),
                 "pcg_iters": [int(i) for i in it64],
+                "breakdown_pcg_iters": [int(i) for i in bd64["pcg_iters"]],
                 "speedup_of_f32": round((ps64 / el64) and value / (ps64 / el64), 3)}
     cpu = None
     if not args.no_cpu:
@@ -407,9 +431,10 @@ def main() -> int:
                              f"reference tol {scene.solve.tol:g} / {scene.solve.max_iters}")},
         "pressure_solve_ms": round(pressure_ms, 3),
         "pcg_iters": [int(i) for i in iters],
+        "breakdown_pcg_iters": [int(i) for i in bd["pcg_iters"]] if bd else None,
         "fp64_gpu": fp64,
         "error_vs_fp64": err,
-        "regions_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(timing.items())},
+        "regions_ms_per_step": {k: round(v[0] / bd_steps, 4) for k, v in sorted(bd_regions.items())},
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serve,

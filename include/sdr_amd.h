@@ -213,7 +213,9 @@ int sdr_solver_set_state(sdr_solver* s, const sdr_state* st);
 int sdr_solver_synchronize(sdr_solver* s);
 
 /* Per-kernel timing of the last substep, filled when enabled (HIP events on the
- * solver's stream).  names/ms arrays of length cap; *count receives the number written. */
+ * solver's stream): on = 1 times every region, on = 2 only the MIC(0) sweep regions
+ * (mic_*; fewer events in the stream), 0 stops.  names/ms arrays of length cap; *count
+ * receives the number written. */
 int sdr_solver_enable_timing(sdr_solver* s, int32_t on);
 int sdr_solver_get_timing(sdr_solver* s, char* names, size_t name_len, double* ms,
                           uint64_t* calls, uint64_t cap, uint64_t* count);

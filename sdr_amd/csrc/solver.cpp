@@ -126,6 +126,7 @@ static int bits_for(uint64_t v) {
 // ------------------------------------------------------------------ timing
 struct Timer {
   bool on = false;
+  bool sweeps_only = false;  // time only the pipelined MIC(0) sweep regions (mic_*)
   hipStream_t st = nullptr;
   struct Pending {
     std::string name;
@@ -145,7 +146,7 @@ struct Timer {
     return e;
   }
   size_t begin(const char* name) {
-    if (!on) return 0;
+    if (!on || (sweeps_only && std::strncmp(name, "mic_", 4) != 0)) return 0;
     Pending p{name, get(), get()};
     HIPCHK(hipEventRecord(p.a, st));
     pending.push_back(p);
@@ -1280,7 +1281,8 @@ int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
 }
 int sdr_solver_synchronize(sdr_solver* s) { return WITH(s, HIPCHK(hipStreamSynchronize(S.st))); }
 int sdr_solver_enable_timing(sdr_solver* s, int32_t on) {
-  return WITH(s, S.timer.collect(); S.timer.on = on != 0; if (on) S.timer.acc.clear());
+  return WITH(s, S.timer.collect(); S.timer.on = on != 0; S.timer.sweeps_only = on == 2;
+              if (on) S.timer.acc.clear());
 }
 int sdr_solver_get_timing(sdr_solver* s, char* names, size_t name_len, double* ms, uint64_t* calls,
                           uint64_t cap, uint64_t* count) {

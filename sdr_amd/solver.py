@@ -288,8 +288,9 @@ class Solver:
     def synchronize(self):
         self.lib.solver_synchronize(self._h)
 
-    def enable_timing(self, on: bool = True):
-        self.lib.solver_enable_timing(self._h, 1 if on else 0)
+    def enable_timing(self, on: bool = True, sweeps_only: bool = False):
+        """Per-region HIP-event timing (sweeps_only: the MIC(0) sweep regions only)."""
+        self.lib.solver_enable_timing(self._h, (2 if sweeps_only else 1) if on else 0)
 
     def timing(self) -> dict:
         cap, nl = 64, 48
