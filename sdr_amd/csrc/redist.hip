@@ -21,6 +21,12 @@
 //     would and commits.  The lowest pending head always commits, so every round makes
 //     progress.  Pending sources are appended to the next round's list with one atomic
 //     per block, so a round costs in proportion to what is still pending.
+//   * a source that holds all of its open receivers also commits the particles after
+//     its head, in reference order, up to the rank of the earliest head that lost one
+//     of those receivers to it this round (the reservation keeps that rank too): no
+//     particle of another cell comes between them in the sequential loop and touches
+//     these receivers, so a large clump with uncontested neighbours drains in one
+//     round instead of one round per particle.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -230,9 +236,9 @@ __global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__
 __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* nnext,
     const uint32_t* __restrict__ src_cell, uint32_t* src_head, const uint32_t* __restrict__ src_lim,
-    uint8_t* src_rmask, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
+    uint16_t* src_rmask, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
     const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
-    unsigned long long* resv, uint32_t epoch) {
+    unsigned long long* resv, unsigned long long* second, uint32_t epoch) {
   const uint32_t nl = *nlist;
   if (blockIdx.x == 0 && threadIdx.x == 0) *nnext = 0;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
@@ -252,26 +258,33 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
       if (!in[q] || !mask[nb[q]] || count[nb[q]] >= cap) continue;
       rm |= (uint8_t)(1u << q);
       best = fmax(best, dn.d[q]);
-      atomicMax(resv + nb[q], val);
+      // of this head and the one it displaced (or that kept the receiver), the later
+      // loses; the earliest loser of each receiver is kept in `second`
+      const unsigned long long old = atomicMax(resv + nb[q], val);
+      const unsigned long long loser = old > val ? val : old;
+      if ((uint32_t)(loser >> 32) == epoch) atomicMax(second + nb[q], loser);
     }
     uint8_t fm = 0;
     for (int q = 0; q < 6; ++q)
       if (((rm >> q) & 1u) && dn.d[q] == best) fm |= (uint8_t)(1u << q);
-    src_rmask[s] = fm;
+    src_rmask[s] = (uint16_t)(fm | ((uint16_t)rm << 8));
     if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
   }
 }
 
 // Commit: a head that holds its best class moves (even if an earlier head holds one of
-// its other receivers: that head's particles never touch the class it holds).  Sources
-// still pending go to the next round's list.
+// its other receivers: that head's particles never touch the class it holds).  A source
+// that holds every open receiver goes on with the particles after the head while their
+// ranks stay below the earliest competing head's.  Sources still pending go to the
+// next round's list.
 __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* next,
     uint32_t* nnext, const uint32_t* __restrict__ src_cell, uint32_t* src_head,
-    const uint32_t* __restrict__ src_lim, const uint8_t* __restrict__ src_rmask,
+    const uint32_t* __restrict__ src_lim, const uint16_t* __restrict__ src_rmask,
     const uint32_t* __restrict__ brank, uint32_t* count, const uint8_t* __restrict__ mask, int nx,
     int ny, int nz, uint32_t cap, RedistDirs dn, const unsigned long long* __restrict__ resv,
-    uint32_t epoch, int32_t* dest, unsigned long long* moved) {
+    const unsigned long long* __restrict__ second, uint32_t epoch, int32_t* dest,
+    unsigned long long* moved) {
   const uint32_t nl = *nlist;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   uint32_t nmoved = 0;
@@ -285,25 +298,64 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
       if (head < lim) {
         const int64_t c = src_cell[s];
         const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
+        const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
         const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-        const uint8_t rm = src_rmask[s];
-        bool held = true;
-        for (int q = 0; q < 6; ++q)
-          if (((rm >> q) & 1u) && resv[nb[q]] != val) held = false;
+        const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+        const uint16_t mk = src_rmask[s];
+        const uint32_t fm = mk & 63u, rm = (mk >> 8) & 63u;
+        bool held = true, all = true;
+        unsigned long long comp = 0;  // earliest competing head on an open receiver
+        for (int q = 0; q < 6; ++q) {
+          if (!((rm >> q) & 1u)) continue;
+          if (resv[nb[q]] != val) {
+            all = false;
+            if ((fm >> q) & 1u) held = false;
+          }
+          const unsigned long long s2 = second[nb[q]];
+          if ((uint32_t)(s2 >> 32) == epoch && s2 > comp) comp = s2;
+        }
         if (!held) {
           pend = true;  // an earlier particle holds a receiver of the class
         } else {
-          const int64_t best = rd_choose(c, count, mask, nx, ny, nz, cap, dn);
-          if (best < 0) {
-            src_head[s] = lim;
-          } else {
-            count[c] -= 1;
-            count[best] += 1;
-            dest[head] = (int32_t)best;
-            ++nmoved;
-            src_head[s] = head + 1;
-            pend = head + 1 < lim;
+          // rooms as the sequential loop sees them (solver.rs:486-544); those of the
+          // receivers this source holds change only by its own moves below
+          uint32_t room[6];
+          for (int q = 0; q < 6; ++q) {
+            const bool open = in[q] && mask[nb[q]] && count[nb[q]] < cap;
+            room[q] = open ? cap - count[nb[q]] : 0u;
           }
+          // ranks below this one are committed without waiting (none: only the head)
+          const uint64_t rank_lim = !all ? 0ull : (comp ? (uint64_t)(0xFFFFFFFFu - (uint32_t)comp) : (1ull << 32));
+          uint32_t add[6] = {0, 0, 0, 0, 0, 0};
+          uint32_t p = head;
+          bool exhausted = false;
+          while (true) {
+            int bq = -1;
+            double bd = -INFINITY;
+            uint32_t br = 0;
+            for (int q = 0; q < 6; ++q) {
+              if (room[q] == 0) continue;
+              if (bq < 0 || dn.d[q] > bd || (dn.d[q] == bd && room[q] > br)) bq = q, bd = dn.d[q], br = room[q];
+            }
+            if (bq < 0) {
+              exhausted = true;  // rooms only shrink: no later particle can move
+              break;
+            }
+            room[bq] -= 1;
+            add[bq] += 1;
+            dest[p] = (int32_t)nb[bq];
+            ++p;
+            if (p >= lim || (uint64_t)brank[p] >= rank_lim) break;
+          }
+          const uint32_t nm = p - head;
+          nmoved += nm;
+          if (nm) {
+            count[c] -= nm;
+            for (int q = 0; q < 6; ++q)
+              if (add[q]) count[nb[q]] += add[q];
+          }
+          src_head[s] = exhausted ? lim : p;
+          pend = !exhausted && p < lim;
         }
       }
     }
@@ -340,9 +392,10 @@ void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n,
 
 void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
                       const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
-                      uint8_t* src_rmask, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+                      uint16_t* src_rmask, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
                       int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
-                      uint32_t epoch, int32_t* dest, uint32_t* ctr, hipStream_t st) {
+                      unsigned long long* second, uint32_t epoch, int32_t* dest, uint32_t* ctr,
+                      hipStream_t st) {
   if (max_pending <= 0) return;
   const uint32_t* in = parity ? pend_b : pend_a;
   uint32_t* out = parity ? pend_a : pend_b;
@@ -352,10 +405,10 @@ void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_
   if (blocks > 4096) blocks = 4096;
   k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, src_rmask,
                                                        brank, count, mask, nx, ny, nz, cap, dn, resv,
-                                                       epoch);
+                                                       second, epoch);
   k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_cell, src_head, src_lim,
                                                       src_rmask, brank, count, mask, nx, ny, nz, cap, dn,
-                                                      resv, epoch, dest, (unsigned long long*)(ctr + 6));
+                                                      resv, second, epoch, dest, (unsigned long long*)(ctr + 6));
 }
 
 }  // namespace sdr

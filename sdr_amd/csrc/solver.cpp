@@ -553,7 +553,7 @@ struct Solver : SolverBase {
   Buf temp;
   Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
   Buf cellkey, act_rank, act_slot, act_rank_sorted, act_slot_sorted, dest;
-  Buf counters, aos_pos, aos_vel, fill_cells, rd_resv, rd_ctr;
+  Buf counters, aos_pos, aos_vel, fill_cells, rd_resv, rd_second, rd_ctr;
   Buf rd_sidx, rd_boff, rd_src_cell, rd_src_off, rd_src_head, rd_src_lim, rd_src_fill, rd_pend_a, rd_pend_b,
       rd_big, rd_rmask;
   PinnedBuf pin;
@@ -858,12 +858,14 @@ struct Solver : SolverBase {
     const int64_t nc = ncells();
     if (rd_resv.bytes < (size_t)nc * 8) {
       rd_resv.alloc((size_t)nc * 8);
+      rd_second.alloc((size_t)nc * 8);
       HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
+      HIPCHK(hipMemsetAsync(rd_second.p, 0, rd_second.bytes, st));
       rd_epoch = 0;
       for (Buf* b : {&rd_sidx, &rd_boff, &rd_src_cell, &rd_src_off, &rd_src_head, &rd_src_lim, &rd_src_fill,
                      &rd_pend_a, &rd_pend_b, &rd_big})
         b->alloc((size_t)nc * 4);
-      rd_rmask.alloc((size_t)nc);
+      rd_rmask.alloc((size_t)nc * 2);
     }
     if (!rd_ctr.p) rd_ctr.alloc(64);
     uint32_t* ctr = rd_ctr.as<uint32_t>();
@@ -894,12 +896,14 @@ struct Solver : SolverBase {
       for (int r = 0; r < check_every; ++r) {
         if (++rd_epoch == 0) {  // epochs wrapped: no stale reservation may outrank a new one
           HIPCHK(hipMemsetAsync(rd_resv.p, 0, rd_resv.bytes, st));
+          HIPCHK(hipMemsetAsync(rd_second.p, 0, rd_second.bytes, st));
           rd_epoch = 1;
         }
         launch_rd_round2(pending, (int)(rounds & 1), rd_pend_a.as<uint32_t>(), rd_pend_b.as<uint32_t>(),
                          rd_src_cell.as<uint32_t>(), rd_src_head.as<uint32_t>(), rd_src_lim.as<uint32_t>(),
-                         rd_rmask.as<uint8_t>(), brank, count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz,
-                         capc, dn, rd_resv.as<unsigned long long>(), rd_epoch, dest.as<int32_t>(), ctr, st);
+                         rd_rmask.as<uint16_t>(), brank, count.as<uint32_t>(), mask.as<uint8_t>(), nx, ny, nz,
+                         capc, dn, rd_resv.as<unsigned long long>(), rd_second.as<unsigned long long>(), rd_epoch,
+                         dest.as<int32_t>(), ctr, st);
         ++rounds;
       }
       // the list written by the last round
@@ -1163,7 +1167,7 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
     // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
     // vectors (f32 path), halo words of the pipelined sweeps
-    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 49.0;
+    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 58.0;
     const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;
