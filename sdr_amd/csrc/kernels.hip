@@ -464,16 +464,25 @@ inline int unit_span_len(Shear sh, int nblocks, int U) {
   return (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
 }
 
-template <class T, int U>
+// UPD: the product of the previous iteration's search-direction update: x is s, and
+// the operand is s' = z + beta s (pressure.rs:386-388) formed at the cell and at each
+// neighbour (the same arithmetic as k_update_s, so the same bits); s' is stored at the
+// cell of another buffer (snew: x is still read by the neighbours' threads).  That
+// saves update_s's pass over s and z.
+template <class T, int U, bool UPD = false>
 __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __restrict__ flags,
                                                     const T* __restrict__ x, T* __restrict__ out,
                                                     const PcgScalars* sc, int it, double* partials,
-                                                    int span) {
+                                                    int span, const T* __restrict__ z = nullptr,
+                                                    T* __restrict__ snew = nullptr) {
   if (sc->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
-  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ);
+  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ),
+               rz = make_rsrc(UPD ? z : nullptr, UPD ? n * SZ : 0u),
+               rsn = make_rsrc(UPD ? snew : nullptr, UPD ? n * SZ : 0u);
+  const T beta = UPD ? (T)sc->beta : T(0);
   const int nx = sh.nx, ny = sh.ny, nz = sh.nz;
   const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)sh.NJ;
   int q0, q1;
@@ -482,21 +491,34 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
   for (int q = q0; q < q1; q += U) {
     UnitLane L[U];
     uint16_t f[U];
-    T xc[U], xn[U][6];
+    T xc[U], xn[U][6], zc[U], zn[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       L[u] = unit_slot(sh, q + u, q1);
       const uint32_t t = L[u].t;
       const int i = L[u].i, j = L[u].j, k = L[u].k;
       const bool v = L[u].v;
+      const uint32_t on[6] = {v && i + 1 < nx ? t + row + 1 : kFarElem, v && i >= 1 ? t - row - 1 : kFarElem,
+                              v && j + 1 < ny ? t + row : kFarElem,     v && j >= 1 ? t - row : kFarElem,
+                              v && k + 1 < nz ? t + plane : kFarElem,   v && k >= 1 ? t - plane : kFarElem};
       f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, t * 2u, 0, 0);
       xc[u] = bld<T>(rx, t * SZ);
-      xn[u][0] = bld<T>(rx, (v && i + 1 < nx ? t + row + 1 : kFarElem) * SZ);
-      xn[u][1] = bld<T>(rx, (v && i >= 1 ? t - row - 1 : kFarElem) * SZ);
-      xn[u][2] = bld<T>(rx, (v && j + 1 < ny ? t + row : kFarElem) * SZ);
-      xn[u][3] = bld<T>(rx, (v && j >= 1 ? t - row : kFarElem) * SZ);
-      xn[u][4] = bld<T>(rx, (v && k + 1 < nz ? t + plane : kFarElem) * SZ);
-      xn[u][5] = bld<T>(rx, (v && k >= 1 ? t - plane : kFarElem) * SZ);
+#pragma unroll
+      for (int d = 0; d < 6; ++d) xn[u][d] = bld<T>(rx, on[d] * SZ);
+      if (UPD) {
+        zc[u] = bld<T>(rz, t * SZ);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) zn[u][d] = bld<T>(rz, on[d] * SZ);
+      }
+    }
+    if (UPD) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xc[u] = zc[u] + beta * xc[u];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) xn[u][d] = zn[u][d] + beta * xn[u][d];
+        bst<T>(rsn, L[u].t * SZ, xc[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1401,6 +1423,15 @@ void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgS
   }
   k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
 }
+template <class T>
+bool launch_spmv_update(Shear sh, const uint16_t* flags, const T* s, const T* z, T* snew, T* out,
+                        const PcgScalars* sc, int it, double* partials, hipStream_t st) {
+  if (!units_fit<T>(sh)) return false;
+  const int nb = cell_blocks(sh);
+  k_spmv_units<T, 2, true><<<nb, 256, 0, st>>>(sh, flags, s, out, sc, it, partials, unit_span_len(sh, nb, 2), z,
+                                               snew);
+  return true;
+}
 int dot_blocks(Shear sh, bool exact) {
   if (exact) {
     int64_t n = (int64_t)sh.nx * sh.ny * sh.nz;
@@ -1616,6 +1647,8 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
                                     T*, uint16_t*, double*, hipStream_t);                          \
   template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
                                double*, hipStream_t);                                              \
+  template bool launch_spmv_update<T>(Shear, const uint16_t*, const T*, const T*, T*, T*,          \
+                                      const PcgScalars*, int, double*, hipStream_t);               \
   template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
                                        const PcgScalars*, hipStream_t);                            \
   template void launch_maxabs_partials<T>(Shear, const T*, double*, int, const PcgScalars*,        \

@@ -192,13 +192,14 @@ struct Projector {
   bool exact_dot = true;
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
-  Buf b, z, s, p, as, q, precon, flags, partials, scal;
+  Buf b, z, s, s2, p, as, q, precon, flags, partials, scal;
   Buf halo_x, halo_z, pipe_err, sink;
   Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
   bool axpy_fuse = false;  // the forward sweep applies the PCG axpys (sweep kinds 2, 4, 6)
+  bool spmv_upd = false;   // s = z + beta s formed inside the next product (measured slower)
   int ncu = 0, dev = 0;  // compute units of the solver's device
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
@@ -206,7 +207,7 @@ struct Projector {
     nx = x, ny = y, nz = zz, dx = h, st = stream, timer = t;
     sh = Shear{nx, ny, nz, nx + ny - 1};
     size_t n = (size_t)sh.size();
-    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
+    for (Buf* bb : {&b, &z, &s, &s2, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
     flags.alloc(n * sizeof(uint16_t));
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
     scal.alloc(sizeof(PcgScalars));
@@ -215,6 +216,7 @@ struct Projector {
     mic_variant = 20;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
     if (const char* e = getenv("SDR_MIC_AXPY")) axpy_fuse = atoi(e) != 0;
+    if (const char* e = getenv("SDR_SPMV_UPDATE")) spmv_upd = atoi(e) != 0;
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -333,21 +335,37 @@ struct Projector {
     const uint64_t check_every = 8;
     if (fused()) {
       const int ndot = mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu));
+      // the search direction lives in s or s2: with the update fused into the product,
+      // each iteration reads one and writes the other
+      T* scur = s.as<T>();
+      T* snext = s2.as<T>();
+      bool upd = spmv_upd;
       for (uint64_t it = 0; it < max_iters; ++it) {
         {
           Scope sc1(*timer, "spmv_dot");
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(),
-                         st);
+          bool done_ = false;
+          if (it > 0 && upd) {
+            done_ = launch_spmv_update<T>(sh, flags.as<uint16_t>(), scur, z.as<T>(), snext, as.as<T>(), sc(),
+                                          (int)it, partials.as<double>(), st);
+            if (done_) {
+              std::swap(scur, snext);
+            } else {  // sizes past the fused kernel's offsets: update, then the product
+              upd = false;
+              launch_update_s<T>(sh, scur, z.as<T>(), sc(), st);
+            }
+          }
+          if (!done_)
+            launch_spmv<T>(sh, flags.as<uint16_t>(), scur, as.as<T>(), sc(), (int)it, partials.as<double>(), st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
         }
         if (fused_axpy()) {
           Scope sc1(*timer, "mic_fwd_axpy");
-          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, p.as<T>(), s.as<T>(), as.as<T>());
+          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, p.as<T>(), scur, as.as<T>());
           launch_dot_finish(partials.as<double>(), ndot, false, kOpResidual, sc(), tol, st);
         } else {
           {
             Scope sc1(*timer, "axpy_norm");
-            launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+            launch_axpy_pr<T>(sh, p.as<T>(), r, scur, as.as<T>(), sc(), partials.as<double>(), mb, st);
             launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
           }
           Scope sc1(*timer, "mic_fwd");
@@ -358,9 +376,9 @@ struct Projector {
           mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
           launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
         }
-        {
+        if (!upd) {
           Scope sc1(*timer, "update_s");
-          launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
+          launch_update_s<T>(sh, scur, z.as<T>(), sc(), st);
         }
         if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
           if (read_scalars().done) break;
@@ -1165,10 +1183,10 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch
     const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 12.0 + (precision == SDR_PRECISION_F64 ? 16.0 : 0.0);
     // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
-    // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
+    // redistribution tables; per sheared slot: 8 solve vectors, flags, f64 refinement
     // vectors (f32 path), halo words of the pipelined sweeps
     const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 58.0;
-    const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
+    const double pslot = 8.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;
   });

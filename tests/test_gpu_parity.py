@@ -335,14 +335,17 @@ def test_mic_multiplane_sweeps_bit_exact(dev, orc, variant, shape, monkeypatch):
         assert bits_equal(a, b)
 
 
+@pytest.mark.parametrize("upd", ["0", "1"])
 @pytest.mark.parametrize("axpy", ["0", "1"])
 @pytest.mark.parametrize("shape", [(150, 37, 41), (64, 64, 64)])
-def test_fused_fast_solve_matches_unfused(dev, shape, axpy, monkeypatch):
+def test_fused_fast_solve_matches_unfused(dev, shape, axpy, upd, monkeypatch):
     """Fast reduction mode with the default sweeps (dot(z, r) fused into the backward
-    sweep; with SDR_MIC_AXPY=1 the axpys also fused into the forward one) against the
-    unfused kernels (variant 0): the same relative residual, and pressures equal to the
-    solve's tolerance."""
+    sweep; with SDR_MIC_AXPY=1 the axpys also fused into the forward one; with
+    SDR_SPMV_UPDATE=1 the search-direction update fused into the next product) against
+    the unfused kernels (variant 0): the same relative residual, and pressures equal to
+    the solve's tolerance."""
     monkeypatch.setenv("SDR_MIC_AXPY", axpy)
+    monkeypatch.setenv("SDR_SPMV_UPDATE", upd)
     nx, ny, nz = shape
     g = np.random.default_rng(3)
     r = g.random((nz, ny, nx))
