@@ -207,7 +207,7 @@ struct Projector {
     nx = x, ny = y, nz = zz, dx = h, st = stream, timer = t;
     sh = Shear{nx, ny, nz, nx + ny - 1};
     size_t n = (size_t)sh.size();
-    for (Buf* bb : {&b, &z, &s, &s2, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
+    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
     flags.alloc(n * sizeof(uint16_t));
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
     scal.alloc(sizeof(PcgScalars));
@@ -217,6 +217,7 @@ struct Projector {
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
     if (const char* e = getenv("SDR_MIC_AXPY")) axpy_fuse = atoi(e) != 0;
     if (const char* e = getenv("SDR_SPMV_UPDATE")) spmv_upd = atoi(e) != 0;
+    if (spmv_upd) s2.alloc(n * sizeof(T));  // the second search-direction buffer
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -1183,10 +1184,10 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch
     const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 12.0 + (precision == SDR_PRECISION_F64 ? 16.0 : 0.0);
     // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
-    // redistribution tables; per sheared slot: 8 solve vectors, flags, f64 refinement
+    // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
     // vectors (f32 path), halo words of the pipelined sweeps
     const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 58.0;
-    const double pslot = 8.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
+    const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;
   });
