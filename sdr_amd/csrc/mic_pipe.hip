@@ -1118,15 +1118,15 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
     if (lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
   };
-  // Publish the values compute waves wrote at step t into LDS buffer b.  The stores are
-  // issued on every step (to an out-of-range offset when there is nothing to publish):
-  // with a fixed number of memory operations per step, the wait for a prefetched halo
-  // word counts exactly the operations issued after it, and never waits for this
-  // step's write-through stores.
-  auto publish = [&](int t, int b) {
+  // Publish the values compute waves wrote at step t (vz: the last plane's row, vx: the
+  // edge lane of plane `lane`, both read from LDS at the top of the step).  The stores
+  // are issued on every step (to an out-of-range offset when there is nothing to
+  // publish): with a fixed number of memory operations per step, the wait for a
+  // prefetched halo word counts exactly the operations issued after it.
+  auto publish = [&](int t, T vz, T vx) {
     {
       const int R = row(lwave, t);
-      const T v = zbuf[b][lwave][lane];
+      const T v = vz;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
       const uint32_t o = z_out && zl_ok && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
@@ -1136,7 +1136,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     }
     {
       const int R = row(lane < TK ? lane : 0, t);
-      const T v = zbuf[b][lane < TK ? lane : 0][edge_out];
+      const T v = vx;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
       const uint32_t o = x_out && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
@@ -1155,12 +1155,12 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int u = 1; u < PFH; ++u) {
-    publish(-1, 0);
+    publish(-1, T(0), T(0));
     fill(u, u);
     __builtin_amdgcn_sched_barrier(0);
   }
-  publish(-1, 0);
   take(0, 0, 1);  // inputs of step 0
+  publish(-1, T(0), T(0));
   fill(0, PFH);
   __syncthreads();
   __builtin_amdgcn_sched_barrier(0);
@@ -1169,11 +1169,14 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       const int cb = u & 1, pb = cb ^ 1;
-      // values of step s - 1 (in buffer pb; step -1 has no row in the grid, so nothing
-      // is stored), inputs of step s + 1 (into buffer cb)
-      publish(s - 1, pb);
+      // values of step s - 1 (LDS buffer pb; step -1 has no row in the grid, so nothing
+      // is stored), inputs of step s + 1 (into buffer cb).  The two LDS reads go out
+      // first and complete while the inputs are checked and written to LDS.
+      const T vz = zbuf[pb][lwave][lane];
+      const T vx = zbuf[pb][lane < TK ? lane : 0][edge_out];
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
+      publish(s - 1, vz, vx);
       fill(un, s + 1 + PFH);
       __syncthreads();
       __builtin_amdgcn_sched_barrier(0);
