@@ -1064,11 +1064,12 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   // neighbour row of plane `lane` at step t (its row at step t, minus dJ).
   auto zrow_of = [&](int t) { return row(bwave, t); };
   auto xrow_of = [&](int t) { return row(lane, t) - dJ; };
+  // (conditions combined with & rather than &&: selects, not branches, in the step)
   auto zoff = [&](int R) -> uint32_t {
-    return z_in && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
+    return (z_in & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
   };
   auto xoff = [&](int R) -> uint32_t {
-    return x_in && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_in + R * W) * 8u : kOob;
+    return (x_in & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_in + R * W) * 8u : kOob;
   };
   uint64_t hz[PFH][W], hx[PFH][W];
   auto fill = [&](int u, int t) {
@@ -1129,7 +1130,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       const T v = vz;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = z_out && zl_ok && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
+      const uint32_t o = (z_out & zl_ok & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
 #pragma unroll
       for (int q = 0; q < W; ++q)
         bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
@@ -1139,7 +1140,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       const T v = vx;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = x_out && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+      const uint32_t o = (x_out & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_out + R * W) * 8u : kOob;
 #pragma unroll
       for (int q = 0; q < W; ++q)
         bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
@@ -1174,6 +1175,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       // first and complete while the inputs are checked and written to LDS.
       const T vz = zbuf[pb][lwave][lane];
       const T vx = zbuf[pb][lane < TK ? lane : 0][edge_out];
+      __builtin_amdgcn_sched_barrier(0);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
       publish(s - 1, vz, vx);
