@@ -455,7 +455,9 @@ __device__ inline UnitLane unit_slot(Shear sh, int qq, int q1) {
 template <class T>
 bool units_fit(Shear sh) {
   const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
-  return sh.size() * (int64_t)sizeof(T) < ((int64_t)1 << 31) && nunits < ((int64_t)1 << 30);
+  const int64_t plane = (int64_t)sh.nx * sh.NJ;
+  // byte offsets of kFarElem +- a plane stay past the records (kFarElem * 4 = 2^30)
+  return (sh.size() + 4 * plane) * (int64_t)sizeof(T) < ((int64_t)1 << 30) && nunits < ((int64_t)1 << 30);
 }
 // span of whole U-unit steps over nblocks * 4 wavefronts
 inline int unit_span_len(Shear sh, int nblocks, int U) {
@@ -498,9 +500,13 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
       const uint32_t t = L[u].t;
       const int i = L[u].i, j = L[u].j, k = L[u].k;
       const bool v = L[u].v;
-      const uint32_t on[6] = {v && i + 1 < nx ? t + row + 1 : kFarElem, v && i >= 1 ? t - row - 1 : kFarElem,
-                              v && j + 1 < ny ? t + row : kFarElem,     v && j >= 1 ? t - row : kFarElem,
-                              v && k + 1 < nz ? t + plane : kFarElem,   v && k >= 1 ? t - plane : kFarElem};
+      // Neighbour slots without bounds tests: a neighbour outside the grid is still a
+      // slot of the buffer, or past either end of it (an offset that wraps below 0 or
+      // passes the record count: the hardware returns 0), and the arithmetic below drops
+      // it.  Lanes outside the grid start from kFarElem, whose neighbours stay past the
+      // record count (units_fit keeps the buffer below 2^30 - 4 planes bytes).
+      (void)v;
+      const uint32_t on[6] = {t + row + 1, t - row - 1, t + row, t - row, t + plane, t - plane};
       f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, t * 2u, 0, 0);
       xc[u] = bld<T>(rx, t * SZ);
 #pragma unroll
