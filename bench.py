@@ -35,7 +35,7 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
         "mic_fwd_axpy": nc * (8 * s + flags),   # read r, As, p, s, precon, flags; write r, p, q
         "update_s": nc * 3 * s,                 # s = z + b s
         "mic_build": nc * (s + flags),          # read flags; write precon
-        "spmv_dot": nc * (4 * s + flags),       # spmv: read s, flags, write As; dot: read s, As
+        "spmv_dot": nc * (2 * s + flags),       # read s, flags; write As (dot(s, As) in the same pass)
         "axpy_norm": nc * 6 * s,                # p += a s; r -= a As; max|r|
         "dot_update": nc * 5 * s,               # dot(z, r); s = z + b s
         "p2g": np_ * 6 * s + 3 * nc * s,        # read x, v; write u, v, w
