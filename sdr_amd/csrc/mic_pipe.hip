@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include <type_traits>
+
 #include "bufrsrc.hpp"
 #include "kernels.hpp"
 #include "mic_pipe.hpp"
@@ -865,8 +867,8 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 // DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
-template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY, int PFC>
-__global__ void __launch_bounds__(64 * (TK + 1))
+template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY, int PFC, bool IO2>
+__global__ void __launch_bounds__(64 * (TK + 1 + IO2))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
                  MicPipeArgs<T> pa) {
@@ -892,7 +894,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   const int lane = threadIdx.x & 63;
   // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool io = wave == TK;
+  const bool io = wave >= TK;
   const int nti = pa.ntile_i, ntk = pa.ntile_k;
   int itl, ktl;
   tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
@@ -1043,12 +1045,17 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     return;
   }
 
-  // -------------------------------------------------------------- I/O wave
+  // -------------------------------------------------------------- I/O wave(s)
+  // ZX = 2: one wave serves both halos; with IO2, wave TK serves the z halo (ZX = 0)
+  // and wave TK + 1 the x halo (ZX = 1), each with half the instructions per step.
+  auto io_body = [&](auto zx) {
+  constexpr int ZX = decltype(zx)::value;
+  constexpr bool DZ = ZX != 1, DX = ZX != 0;
   const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
-  const bool z_in = REV ? kt + 1 < ntk : kt > 0;
-  const bool z_out = REV ? kt > 0 : kt + 1 < ntk;
-  const bool x_in = REV ? it + 1 < nti : it > 0;
-  const bool x_out = REV ? it > 0 : it + 1 < nti;
+  const bool z_in = DZ && (REV ? kt + 1 < ntk : kt > 0);
+  const bool z_out = DZ && (REV ? kt > 0 : kt + 1 < ntk);
+  const bool x_in = DX && (REV ? it + 1 < nti : it > 0);
+  const bool x_out = DX && (REV ? it > 0 : it + 1 < nti);
   const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
   const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
   const int zrow = W * nxp;
@@ -1072,12 +1079,16 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     return (x_in & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_in + R * W) * 8u : kOob;
   };
   uint64_t hz[PFH][W], hx[PFH][W];
+#pragma unroll
+  for (int u = 0; u < PFH; ++u)
+#pragma unroll
+    for (int q = 0; q < W; ++q) hz[u][q] = hx[u][q] = 0;
   auto fill = [&](int u, int t) {
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-      hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
-      hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
+      if (DZ) hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
+      if (DX) hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
     }
   };
   // A word is ready when it carries this sweep's tag, or when it is not needed (its
@@ -1116,8 +1127,8 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       settle(hx[u], rhx, ox, 8u);
       __builtin_amdgcn_s_waitcnt(0x0F70);
     }
-    zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
-    if (lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
+    if (DZ) zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
+    if (DX && lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
   };
   // Publish the values compute waves wrote at step t (vz: the last plane's row, vx: the
   // edge lane of plane `lane`, both read from LDS at the top of the step).  The stores
@@ -1125,7 +1136,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   // publish): with a fixed number of memory operations per step, the wait for a
   // prefetched halo word counts exactly the operations issued after it.
   auto publish = [&](int t, T vz, T vx) {
-    {
+    if (DZ) {
       const int R = row(lwave, t);
       const T v = vz;
       uint32_t wv[W];
@@ -1135,7 +1146,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       for (int q = 0; q < W; ++q)
         bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
-    {
+    if (DX) {
       const int R = row(lane < TK ? lane : 0, t);
       const T v = vx;
       uint32_t wv[W];
@@ -1146,8 +1157,8 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
   };
-  zbuf[0][TK][lane] = T(0);
-  if (lane < TK) xlds[0][lane] = T(0);
+  if (DZ) zbuf[0][TK][lane] = T(0);
+  if (DX && lane < TK) xlds[0][lane] = T(0);
   // The prologue issues the memory operations of steps -PFH-1 .. -1 in the order of the
   // step loop (the stores of those steps have no row, so nothing is stored): a wait
   // at the top of the loop, which holds for every way into it, is then the same as in
@@ -1173,8 +1184,8 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       // values of step s - 1 (LDS buffer pb; step -1 has no row in the grid, so nothing
       // is stored), inputs of step s + 1 (into buffer cb).  The two LDS reads go out
       // first and complete while the inputs are checked and written to LDS.
-      const T vz = zbuf[pb][lwave][lane];
-      const T vx = zbuf[pb][lane < TK ? lane : 0][edge_out];
+      const T vz = DZ ? zbuf[pb][lwave][lane] : T(0);
+      const T vx = DX ? zbuf[pb][lane < TK ? lane : 0][edge_out] : T(0);
       __builtin_amdgcn_sched_barrier(0);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
@@ -1184,6 +1195,13 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+  };
+  if (!IO2)
+    io_body(std::integral_constant<int, 2>{});
+  else if (wave == TK)
+    io_body(std::integral_constant<int, 0>{});
+  else
+    io_body(std::integral_constant<int, 1>{});
   __syncthreads();  // matches the compute waves' final barrier
 }
 
@@ -1502,21 +1520,21 @@ void launch_sweep4(int op, Shear sh, const uint16_t* flags, const T* precon, con
                                                                                 sc, pa);
 }
 
-template <class T, int TK, int PFH, int PFC = 8>
+template <class T, int TK, int PFH, int PFC = 8, bool IO2 = false>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
   if (op == kMicFwd && pa.ax_p)
-    k_mic_sweep3<T, false, TK, false, PFH, true, 4><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, false, TK, false, PFH, true, 4, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                               pa);
   else if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, false, TK, false, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                                pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, true, TK, true, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                              pa);
   else
-    k_mic_sweep3<T, true, TK, false, PFH, false, PFC><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc,
+    k_mic_sweep3<T, true, TK, false, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
                                                                               pa);
 }
 
@@ -1548,10 +1566,11 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 // values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
 // (kind 4) or 2 (kind 6), cell ring depth 8; 24-29: k_mic_sweep4 tiles of TK waves of
 // KT planes (plus the I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2)
-// or 4 (kind 9); 30+: k_mic_sweep3 with deeper cell rings (pf).
+// or 4 (kind 9); 30-34: k_mic_sweep3 with deeper cell rings (pf); 35+: k_mic_sweep3
+// with two I/O waves (z halo and x halo).
 namespace {
 struct VariantRow {
-  int ti, tk, kt, kind, xcd, pf;  // pf: k_mic_sweep3's cell ring depth (0: 8)
+  int ti, tk, kt, kind, xcd, pf, io2;  // pf: k_mic_sweep3's cell ring depth (0: 8); io2: two I/O waves
 };
 const VariantRow kVariants[] = {
     {1, 16, 1, 0, 0}, {1, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 4, 1, 0, 0}, {1, 4, 1, 0, 0},
@@ -1562,6 +1581,7 @@ const VariantRow kVariants[] = {
     {1, 7, 4, 8, 0},  {1, 7, 2, 8, 0},  {1, 3, 4, 8, 0},  {1, 7, 4, 9, 0}, {1, 3, 8, 8, 0},
     {1, 15, 2, 8, 0},
     {1, 7, 1, 4, 0, 12}, {1, 7, 1, 4, 0, 16}, {1, 3, 1, 4, 0, 16}, {1, 15, 1, 4, 0, 12}, {1, 7, 1, 6, 0, 12},
+    {1, 7, 1, 4, 0, 0, 1}, {1, 11, 1, 4, 0, 0, 1}, {1, 14, 1, 4, 0, 0, 1}, {1, 7, 1, 2, 0, 0, 1},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
@@ -1579,6 +1599,7 @@ MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
   s.kt = r.kt;
   s.kind = r.kind;
   s.pf = r.pf > 0 ? r.pf : 8;
+  s.io2 = r.io2;
   const int planes = r.tk * r.kt;
   s.ntile_i = (sh.nx + 64 * s.ti - 1) / (64 * s.ti);
   s.ntile_k = (sh.nz + planes - 1) / planes;
@@ -1596,7 +1617,14 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
   pa.xcd = s.xcd;
   if (s.kind == 2 || s.kind == 4 || s.kind == 6) {
     const int pfh = s.kind == 2 ? 8 : (s.kind == 4 ? 4 : 2);
-    switch (s.tk * 1000 + s.pf * 10 + pfh) {
+    if (s.io2) switch (s.tk * 10 + pfh) {
+      case 74: launch_sweep3<T, 7, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 114: launch_sweep3<T, 11, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 144: launch_sweep3<T, 14, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 78: launch_sweep3<T, 7, 8, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      default: break;
+    }
+    if (!s.io2) switch (s.tk * 1000 + s.pf * 10 + pfh) {
       case 7124: launch_sweep3<T, 7, 4, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 7164: launch_sweep3<T, 7, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 3164: launch_sweep3<T, 3, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;

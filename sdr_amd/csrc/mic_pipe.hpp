@@ -44,7 +44,8 @@ struct MicPipeArgs {
 // kernel.
 struct MicPipeShape {
   int ti, tk, kt, kind, ntile_i, ntile_k, xcd;
-  int pf;  // cell ring depth of the k_mic_sweep3 kinds
+  int pf;   // cell ring depth of the k_mic_sweep3 kinds
+  int io2;  // k_mic_sweep3 with two I/O waves
 };
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu);
