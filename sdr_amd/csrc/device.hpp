@@ -254,6 +254,11 @@ __device__ inline double jitter_mix(uint64_t k) {
 // contiguous rows (coalesced) while every cell still sees the reference operands.
 struct Shear {
   int nx, ny, nz, NJ;
+  // Work units of the vector kernels that hold in-grid slots, per plane: entry r is
+  // (J << 8) | chunk for the r-th (row J, 64-slot chunk) pair with an in-grid slot, in
+  // row order (built by the solver; null: every chunk of every row is a unit).
+  const uint32_t* units = nullptr;
+  int units_per_plane = 0;
   __host__ __device__ int64_t size() const { return (int64_t)nx * NJ * nz; }
   __host__ __device__ int64_t at(int i, int j, int k) const {
     return ((int64_t)k * NJ + (i + j)) * nx + i;

@@ -422,17 +422,23 @@ __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restri
 // seven x values of a lane are loaded before any is used (a neighbour outside the grid,
 // and every slot outside it, is addressed past the buffer: no load).  The arithmetic and
 // its order are those of k_spmv.
-// Work units of the vector kernels: unit q is the 64 slots of chunk q % nchunk of sheared
-// row q / nchunk.  unit_span gives the calling wavefront its contiguous span [q0, q1)
-// (the spans of workgroups on one XCD being contiguous too) and unit_slot the slot of
-// this lane in unit qq, or kFarElem outside the grid or the span.
+// Work units of the vector kernels: unit q is the 64 slots of a chunk of a sheared row;
+// with the solver's unit table (Shear::units) only the chunks that hold in-grid slots
+// are units (q = k * units_per_plane + r, table entry r gives row and chunk; a third of
+// the chunks of a cube's rows hold none), else chunk q % nchunk of row q / nchunk.
+// unit_span gives the calling wavefront its contiguous span [q0, q1) (the spans of
+// workgroups on one XCD being contiguous too) and unit_slot the slot of this lane in
+// unit qq, or kFarElem outside the grid or the span.
 struct UnitLane {
   uint32_t t;  // slot, or kFarElem
   int i, j, k;
   bool v;
 };
+__host__ __device__ inline int64_t unit_count(Shear sh) {
+  return sh.units ? (int64_t)sh.nz * sh.units_per_plane : (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+}
 __device__ inline void unit_span(Shear sh, int span, int* q0, int* q1) {
-  const int nunits = sh.nz * sh.NJ * ((sh.nx + 63) >> 6);
+  const int nunits = (int)unit_count(sh);
   const int G = gridDim.x, b = blockIdx.x;
   const int lb = (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
   const int gw = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -440,12 +446,23 @@ __device__ inline void unit_span(Shear sh, int span, int* q0, int* q1) {
   *q1 = *q0 + span < nunits ? *q0 + span : nunits;
 }
 __device__ inline UnitLane unit_slot(Shear sh, int qq, int q1) {
-  const int nchunk = (sh.nx + 63) >> 6;
   UnitLane L;
-  const int r = qq / nchunk;
-  L.i = (qq - r * nchunk) * 64 + (threadIdx.x & 63);
-  const int J = r % sh.NJ;
-  L.k = r / sh.NJ;
+  int r, J;
+  if (sh.units) {
+    // wave-uniform: the table entry comes through the scalar cache
+    const int qc = qq < q1 ? qq : q1 - 1;
+    L.k = qc / sh.units_per_plane;
+    const uint32_t e = sh.units[qc - L.k * sh.units_per_plane];
+    J = (int)(e >> 8);
+    L.i = (int)(e & 255u) * 64 + (threadIdx.x & 63);
+    r = L.k * sh.NJ + J;
+  } else {
+    const int nchunk = (sh.nx + 63) >> 6;
+    r = qq / nchunk;
+    L.i = (qq - r * nchunk) * 64 + (threadIdx.x & 63);
+    J = r % sh.NJ;
+    L.k = r / sh.NJ;
+  }
   L.j = J - L.i;
   L.v = qq < q1 && L.i < sh.nx && (unsigned)L.j < (unsigned)sh.ny;
   L.t = L.v ? (uint32_t)r * (uint32_t)sh.nx + (uint32_t)L.i : kFarElem;
@@ -454,14 +471,14 @@ __device__ inline UnitLane unit_slot(Shear sh, int qq, int q1) {
 // unit kernels address with 32-bit byte offsets
 template <class T>
 bool units_fit(Shear sh) {
-  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+  const int64_t nunits = unit_count(sh);
   const int64_t plane = (int64_t)sh.nx * sh.NJ;
   // byte offsets of kFarElem +- a plane stay past the records (kFarElem * 4 = 2^30)
   return (sh.size() + 4 * plane) * (int64_t)sizeof(T) < ((int64_t)1 << 30) && nunits < ((int64_t)1 << 30);
 }
 // span of whole U-unit steps over nblocks * 4 wavefronts
 inline int unit_span_len(Shear sh, int nblocks, int U) {
-  const int64_t nunits = (int64_t)sh.nz * sh.NJ * ((sh.nx + 63) / 64);
+  const int64_t nunits = unit_count(sh);
   const int64_t waves = (int64_t)nblocks * 4;
   return (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
 }

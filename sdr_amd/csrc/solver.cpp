@@ -192,7 +192,7 @@ struct Projector {
   bool exact_dot = true;
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
-  Buf b, z, s, s2, p, as, q, precon, flags, partials, scal;
+  Buf b, z, s, s2, p, as, q, precon, flags, partials, scal, units;
   Buf halo_x, halo_z, pipe_err, sink;
   Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
@@ -206,6 +206,19 @@ struct Projector {
   void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
     nx = x, ny = y, nz = zz, dx = h, st = stream, timer = t;
     sh = Shear{nx, ny, nz, nx + ny - 1};
+    // the vector kernels' work units: the (row, 64-slot chunk) pairs of a plane that hold
+    // in-grid slots, in row order
+    if (nx <= 256 * 64) {
+      std::vector<uint32_t> u;
+      for (int J = 0; J < sh.NJ; ++J) {
+        const int imin = std::max(0, J - ny + 1), imax = std::min(J, nx - 1);
+        for (int c = imin / 64; c <= imax / 64; ++c) u.push_back(((uint32_t)J << 8) | (uint32_t)c);
+      }
+      units.alloc(u.size() * sizeof(uint32_t));
+      HIPCHK(hipMemcpy(units.p, u.data(), u.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      sh.units = units.as<uint32_t>();
+      sh.units_per_plane = (int)u.size();
+    }
     size_t n = (size_t)sh.size();
     for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
     flags.alloc(n * sizeof(uint16_t));
