@@ -1566,8 +1566,8 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 // values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
 // (kind 4) or 2 (kind 6), cell ring depth 8; 24-29: k_mic_sweep4 tiles of TK waves of
 // KT planes (plus the I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2)
-// or 4 (kind 9); 30-34: k_mic_sweep3 with deeper cell rings (pf); 35+: k_mic_sweep3
-// with two I/O waves (z halo and x halo).
+// or 4 (kind 9); 30-34: k_mic_sweep3 with deeper cell rings (pf); 35-38: k_mic_sweep3
+// with two I/O waves (z halo and x halo); 39+: more k_mic_sweep3 tile depths.
 namespace {
 struct VariantRow {
   int ti, tk, kt, kind, xcd, pf, io2;  // pf: k_mic_sweep3's cell ring depth (0: 8); io2: two I/O waves
@@ -1582,6 +1582,7 @@ const VariantRow kVariants[] = {
     {1, 15, 2, 8, 0},
     {1, 7, 1, 4, 0, 12}, {1, 7, 1, 4, 0, 16}, {1, 3, 1, 4, 0, 16}, {1, 15, 1, 4, 0, 12}, {1, 7, 1, 6, 0, 12},
     {1, 7, 1, 4, 0, 0, 1}, {1, 11, 1, 4, 0, 0, 1}, {1, 14, 1, 4, 0, 0, 1}, {1, 7, 1, 2, 0, 0, 1},
+    {1, 9, 1, 4, 0}, {1, 11, 1, 4, 0}, {1, 5, 1, 4, 0}, {1, 11, 1, 6, 0},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
@@ -1640,6 +1641,10 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
       case 74: launch_sweep3<T, 7, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 34: launch_sweep3<T, 3, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
       case 72: launch_sweep3<T, 7, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 94: launch_sweep3<T, 9, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 114: launch_sweep3<T, 11, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 54: launch_sweep3<T, 5, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
+      case 112: launch_sweep3<T, 11, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
       default: break;
     }
   }

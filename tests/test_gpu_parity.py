@@ -316,7 +316,7 @@ def _multi_case(orc, shape):
 
 
 @pytest.mark.parametrize("shape", [(150, 37, 41), (130, 19, 256)])
-@pytest.mark.parametrize("variant", [str(v) for v in range(6, 39)])
+@pytest.mark.parametrize("variant", [str(v) for v in range(6, 43)])
 def test_mic_multiplane_sweeps_bit_exact(dev, orc, variant, shape, monkeypatch):
     """The multi-plane sweeps (a thread owns several planes of its pencil), with and
     without XCD placement of the tiles, and the sweeps with an I/O wave reproduce the
