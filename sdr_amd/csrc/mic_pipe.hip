@@ -1079,12 +1079,17 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
     return (x_in & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_in + R * W) * 8u : kOob;
   };
   uint64_t hz[PFH][W], hx[PFH][W];
+  uint32_t ozr[PFH], oxr[PFH];  // the offsets each ring slot was loaded from
 #pragma unroll
-  for (int u = 0; u < PFH; ++u)
+  for (int u = 0; u < PFH; ++u) {
+    ozr[u] = oxr[u] = kOob;
 #pragma unroll
     for (int q = 0; q < W; ++q) hz[u][q] = hx[u][q] = 0;
+  }
   auto fill = [&](int u, int t) {
     const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+    ozr[u] = oz;
+    oxr[u] = ox;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       if (DZ) hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
@@ -1112,7 +1117,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   // means the producer is behind the prefetch window: wait for the window's newest word
   // (rows are published in order), then reload the whole window.
   auto take = [&](int u, int t, int b) {
-    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+    const uint32_t oz = ozr[u], ox = oxr[u];  // slot u was filled for target step t
     const bool ok = ready(hz[u], oz) & ready(hx[u], ox);  // no branch between the waits
     if (__ballot(!ok) != 0ull) {
       const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
