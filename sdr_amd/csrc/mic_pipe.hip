@@ -1086,8 +1086,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
 #pragma unroll
     for (int q = 0; q < W; ++q) hz[u][q] = hx[u][q] = 0;
   }
-  auto fill = [&](int u, int t) {
-    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
+  auto fill_at = [&](int u, uint32_t oz, uint32_t ox) {
     ozr[u] = oz;
     oxr[u] = ox;
 #pragma unroll
@@ -1096,6 +1095,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       if (DX) hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
     }
   };
+  auto fill = [&](int u, int t) { fill_at(u, zoff(zrow_of(t)), xoff(xrow_of(t))); };
   // A word is ready when it carries this sweep's tag, or when it is not needed (its
   // offset is out of range: the load returned 0, and the value used is 0).
   auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
@@ -1140,27 +1140,34 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   // are issued on every step (to an out-of-range offset when there is nothing to
   // publish): with a fixed number of memory operations per step, the wait for a
   // prefetched halo word counts exactly the operations issued after it.
-  auto publish = [&](int t, T vz, T vx) {
+  auto zpub_off = [&](int R) -> uint32_t {
+    return (z_out & zl_ok & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
+  };
+  auto xpub_off = [&](int R) -> uint32_t {
+    return (x_out & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+  };
+  auto publish_at = [&](uint32_t oz, uint32_t ox, T vz, T vx) {
     if (DZ) {
-      const int R = row(lwave, t);
       const T v = vz;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = (z_out & zl_ok & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
+      const uint32_t o = oz;
 #pragma unroll
       for (int q = 0; q < W; ++q)
         bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
     if (DX) {
-      const int R = row(lane < TK ? lane : 0, t);
       const T v = vx;
       uint32_t wv[W];
       __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = (x_out & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+      const uint32_t o = ox;
 #pragma unroll
       for (int q = 0; q < W; ++q)
         bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
     }
+  };
+  auto publish = [&](int t, T vz, T vx) {
+    publish_at(zpub_off(row(lwave, t)), xpub_off(row(lane < TK ? lane : 0, t)), vz, vx);
   };
   if (DZ) zbuf[0][TK][lane] = T(0);
   if (DX && lane < TK) xlds[0][lane] = T(0);
@@ -1181,6 +1188,10 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   fill(0, PFH);
   __syncthreads();
   __builtin_amdgcn_sched_barrier(0);
+  // rows of the step's publish (step s - 1) and prefetch (target s + 1 + PFH), advanced
+  // by dJ per step: the offsets then cost a compare and a select each
+  int rzp = row(lwave, -1), rxp = row(lane < TK ? lane : 0, -1);
+  int rzf = zrow_of(1 + PFH), rxf = xrow_of(1 + PFH);
   for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1194,8 +1205,9 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       __builtin_amdgcn_sched_barrier(0);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
-      publish(s - 1, vz, vx);
-      fill(un, s + 1 + PFH);
+      publish_at(zpub_off(rzp), xpub_off(rxp), vz, vx);
+      fill_at(un, zoff(rzf), xoff(rxf));
+      rzp += dJ, rxp += dJ, rzf += dJ, rxf += dJ;
       __syncthreads();
       __builtin_amdgcn_sched_barrier(0);
     }
