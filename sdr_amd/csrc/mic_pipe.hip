@@ -1051,6 +1051,9 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   auto io_body = [&](auto zx) {
   constexpr int ZX = decltype(zx)::value;
   constexpr bool DZ = ZX != 1, DX = ZX != 0;
+  // the I/O wave's step is the longest of the workgroup's: let its instructions issue
+  // first on the SIMD it shares with a compute wave
+  __builtin_amdgcn_s_setprio(3);
   const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
   const bool z_in = DZ && (REV ? kt + 1 < ntk : kt > 0);
   const bool z_out = DZ && (REV ? kt > 0 : kt + 1 < ntk);
