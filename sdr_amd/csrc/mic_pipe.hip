@@ -1195,6 +1195,9 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   // by dJ per step: the offsets then cost a compare and a select each
   int rzp = row(lwave, -1), rxp = row(lane < TK ? lane : 0, -1);
   int rzf = zrow_of(1 + PFH), rxf = xrow_of(1 + PFH);
+  // the z offsets' row terms, advanced with the rows (no multiply in the step)
+  const uint32_t dz8 = (uint32_t)(dJ * zrow) * 8u;
+  uint32_t ozp = (uint32_t)(zh_out + rzp * zrow) * 8u, ozf = (uint32_t)(zh_in + rzf * zrow) * 8u;
   for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1208,9 +1211,10 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       __builtin_amdgcn_sched_barrier(0);
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
-      publish_at(zpub_off(rzp), xpub_off(rxp), vz, vx);
-      fill_at(un, zoff(rzf), xoff(rxf));
+      publish_at((z_out & zl_ok & ((unsigned)rzp < (unsigned)NJ)) ? ozp : kOob, xpub_off(rxp), vz, vx);
+      fill_at(un, (z_in & ((unsigned)rzf < (unsigned)NJ)) ? ozf : kOob, xoff(rxf));
       rzp += dJ, rxp += dJ, rzf += dJ, rxf += dJ;
+      ozp += dz8, ozf += dz8;
       __syncthreads();
       __builtin_amdgcn_sched_barrier(0);
     }
