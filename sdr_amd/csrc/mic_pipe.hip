@@ -73,6 +73,22 @@ __device__ inline T wave_shift(T v) {
   return r;
 }
 
+// The same shift with `edge` in the lane that has no source (lane 0 shifting right,
+// lane 63 shifting left): DPP keeps the old value there, so no select is needed.
+template <bool REV, class T>
+__device__ inline T wave_shift_in(T v, T edge) {
+  constexpr int n = sizeof(T) / 4;
+  uint32_t w[n], e[n];
+  __builtin_memcpy(w, &v, sizeof(T));
+  __builtin_memcpy(e, &edge, sizeof(T));
+#pragma unroll
+  for (int q = 0; q < n; ++q)
+    w[q] = (uint32_t)__builtin_amdgcn_update_dpp((int)e[q], (int)w[q], REV ? 0x130 : 0x138, 0xf, 0xf, false);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
 // Values a cell publishes per direction: the build sends (term, modification), the
 // solves send one value.
 template <int OP>
@@ -969,9 +985,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
         const int u = uu % PF;
         const int cb = uu & 1, pb = cb ^ 1;
         const T zin = zbuf[pb][zsrc][lane];
-        const T xe = xlds[pb][wk];
-        const T xs = wave_shift<REV>(prev);
-        const T xin = lane == edge_in ? xe : xs;
+        const T xin = wave_shift_in<REV>(prev, xlds[pb][wk]);  // the edge lane takes the x halo
         const uint32_t f = cf[u];
         T inu = ci[u];
         const T pcu = cp[u];
