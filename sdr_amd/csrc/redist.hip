@@ -16,17 +16,19 @@
 //     count - cap entries are then put in reference (rank) order by a partial
 //     selection, one thread per source (one wave for large clumps);
 //   * rounds: the head particle of every pending source reserves its open receivers
-//     with (round, reference rank) by a 64-bit atomic max; a head that holds every
-//     receiver of its best gravity class sees exactly the counts the sequential loop
-//     would and commits.  The lowest pending head always commits, so every round makes
-//     progress.  Pending sources are appended to the next round's list with one atomic
-//     per block, so a round costs in proportion to what is still pending.
-//   * a source that holds all of its open receivers also commits the particles after
-//     its head, in reference order, up to the rank of the earliest head that lost one
-//     of those receivers to it this round (the reservation keeps that rank too): no
-//     particle of another cell comes between them in the sequential loop and touches
-//     these receivers, so a large clump with uncontested neighbours drains in one
-//     round instead of one round per particle.
+//     with (round, reference rank) by a 64-bit atomic max; a head whose choice depends
+//     only on receivers it holds (or that are already full) sees exactly the counts the
+//     sequential loop would and commits.  The lowest pending head always commits, so
+//     every round makes progress.  Pending sources are appended to the next round's
+//     list with one atomic per block, so a round costs in proportion to what is still
+//     pending.
+//   * a committing source goes on with the particles after its head, in reference
+//     order, while each choice still depends only on held or full receivers and the
+//     rank stays below the earliest head that lost a held receiver to it this round
+//     (the reservation keeps that rank too): no particle of another cell comes between
+//     them in the sequential loop and touches these receivers, so a clump whose
+//     downhill neighbours are uncontested drains in one round instead of one round per
+//     particle.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -37,6 +39,7 @@ namespace {
 
 constexpr int kRdBlock = 256;
 constexpr uint32_t kBig = 64;  // buckets above this size are ordered by a whole wave
+constexpr int kRun = 8;        // bucket ranks a committing source loads at once
 
 inline unsigned rd_grid(int64_t n, int block = kRdBlock) {
   int64_t b = (n + block - 1) / block;
@@ -230,15 +233,13 @@ __global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__
 }
 
 // Reservation: a head reserves every open receiver it could still ever write (rooms only
-// shrink, so later particles of its cell touch a subset), but its own decision reads only
-// the open receivers of the best gravity class: a receiver of a higher class is closed
-// for good, and one of a lower class loses to any open member of the best class.
+// shrink, so later particles of its cell touch a subset); the set is kept for the commit.
 __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* nnext,
     const uint32_t* __restrict__ src_cell, uint32_t* src_head, const uint32_t* __restrict__ src_lim,
     uint16_t* src_rmask, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
-    const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, RedistDirs dn,
-    unsigned long long* resv, unsigned long long* second, uint32_t epoch) {
+    const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, unsigned long long* resv,
+    unsigned long long* second, uint32_t epoch) {
   const uint32_t nl = *nlist;
   if (blockIdx.x == 0 && threadIdx.x == 0) *nnext = 0;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
@@ -253,38 +254,39 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
     const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
     const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
     uint8_t rm = 0;
-    double best = -INFINITY;
     for (int q = 0; q < 6; ++q) {
       if (!in[q] || !mask[nb[q]] || count[nb[q]] >= cap) continue;
       rm |= (uint8_t)(1u << q);
-      best = fmax(best, dn.d[q]);
       // of this head and the one it displaced (or that kept the receiver), the later
       // loses; the earliest loser of each receiver is kept in `second`
       const unsigned long long old = atomicMax(resv + nb[q], val);
       const unsigned long long loser = old > val ? val : old;
       if ((uint32_t)(loser >> 32) == epoch) atomicMax(second + nb[q], loser);
     }
-    uint8_t fm = 0;
-    for (int q = 0; q < 6; ++q)
-      if (((rm >> q) & 1u) && dn.d[q] == best) fm |= (uint8_t)(1u << q);
-    src_rmask[s] = (uint16_t)(fm | ((uint16_t)rm << 8));
+    src_rmask[s] = rm;
     if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
   }
 }
 
-// Commit: a head that holds its best class moves (even if an earlier head holds one of
-// its other receivers: that head's particles never touch the class it holds).  A source
-// that holds every open receiver goes on with the particles after the head while their
-// ranks stay below the earliest competing head's.  Sources still pending go to the
-// next round's list.
+// Commit: a source moves its head, then the particles after it in reference order,
+// while each choice is exact and no particle of another cell can come between:
+//   * a receiver this source holds changes only by its own moves this round, so its
+//     room is exact while the particle's rank stays below the earliest head that lost
+//     one of the held receivers to it (kept by the reservation);
+//   * a receiver held by another source is written this round only by particles that
+//     come before this source's head (its holder's run stops at the earliest loser,
+//     which is at most this head), and counts only grow: read full, it is full for
+//     every particle of this source; read with room, its room is unknown;
+// so a choice is exact when every open receiver at least as downhill as the chosen one
+// is held or read full.  The head's own choice among the receivers of its best gravity
+// class follows the same test.  Sources still pending go to the next round's list.
 __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* next,
     uint32_t* nnext, const uint32_t* __restrict__ src_cell, uint32_t* src_head,
     const uint32_t* __restrict__ src_lim, const uint16_t* __restrict__ src_rmask,
-    const uint32_t* __restrict__ brank, uint32_t* count, const uint8_t* __restrict__ mask, int nx,
-    int ny, int nz, uint32_t cap, RedistDirs dn, const unsigned long long* __restrict__ resv,
-    const unsigned long long* __restrict__ second, uint32_t epoch, int32_t* dest,
-    unsigned long long* moved) {
+    const uint32_t* __restrict__ brank, uint32_t* count, int nx, int ny, uint32_t cap, RedistDirs dn,
+    const unsigned long long* __restrict__ resv, const unsigned long long* __restrict__ second,
+    uint32_t epoch, int32_t* dest, unsigned long long* moved) {
   const uint32_t nl = *nlist;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   uint32_t nmoved = 0;
@@ -298,38 +300,41 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
       if (head < lim) {
         const int64_t c = src_cell[s];
         const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
-        const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
         const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-        const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
-        const uint16_t mk = src_rmask[s];
-        const uint32_t fm = mk & 63u, rm = (mk >> 8) & 63u;
-        bool held = true, all = true;
-        unsigned long long comp = 0;  // earliest competing head on an open receiver
+        // the receivers open at the reservation (a closed one stays closed)
+        const uint32_t rm = src_rmask[s];
+        uint32_t held = 0;
+        unsigned long long comp = 0;  // earliest competing head on a held receiver
+        uint32_t room[6];
         for (int q = 0; q < 6; ++q) {
+          room[q] = 0;
           if (!((rm >> q) & 1u)) continue;
-          if (resv[nb[q]] != val) {
-            all = false;
-            if ((fm >> q) & 1u) held = false;
+          if (resv[nb[q]] == val) {
+            held |= 1u << q;
+            const unsigned long long s2 = second[nb[q]];
+            if ((uint32_t)(s2 >> 32) == epoch && s2 > comp) comp = s2;
           }
-          const unsigned long long s2 = second[nb[q]];
-          if ((uint32_t)(s2 >> 32) == epoch && s2 > comp) comp = s2;
+          // rooms as the sequential loop sees them (solver.rs:486-544), exact for held
+          // receivers and at most what is left for the others
+          const uint32_t cn = count[nb[q]];
+          room[q] = cn < cap ? cap - cn : 0u;
         }
-        if (!held) {
-          pend = true;  // an earlier particle holds a receiver of the class
-        } else {
-          // rooms as the sequential loop sees them (solver.rs:486-544); those of the
-          // receivers this source holds change only by its own moves below
-          uint32_t room[6];
-          for (int q = 0; q < 6; ++q) {
-            const bool open = in[q] && mask[nb[q]] && count[nb[q]] < cap;
-            room[q] = open ? cap - count[nb[q]] : 0u;
+        const uint64_t rank_lim = comp ? (uint64_t)(0xFFFFFFFFu - (uint32_t)comp) : (1ull << 32);
+        uint32_t add[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t p = head;
+        bool exhausted = false, go = true;
+        while (go) {
+          // the ranks of the next kRun bucket entries, loaded together rather than one
+          // dependent load per moved particle
+          uint32_t rk[kRun];
+#pragma unroll
+          for (int t = 0; t < kRun; ++t) {
+            const uint32_t e = p + 1 + (uint32_t)t;
+            rk[t] = e < lim ? brank[e] : 0u;
           }
-          // ranks below this one are committed without waiting (none: only the head)
-          const uint64_t rank_lim = !all ? 0ull : (comp ? (uint64_t)(0xFFFFFFFFu - (uint32_t)comp) : (1ull << 32));
-          uint32_t add[6] = {0, 0, 0, 0, 0, 0};
-          uint32_t p = head;
-          bool exhausted = false;
-          while (true) {
+#pragma unroll
+          for (int t = 0; t < kRun; ++t) {
+            if (!go) continue;
             int bq = -1;
             double bd = -INFINITY;
             uint32_t br = 0;
@@ -338,25 +343,35 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
               if (bq < 0 || dn.d[q] > bd || (dn.d[q] == bd && room[q] > br)) bq = q, bd = dn.d[q], br = room[q];
             }
             if (bq < 0) {
-              exhausted = true;  // rooms only shrink: no later particle can move
-              break;
+              // every open receiver is full for good: no later particle can move
+              exhausted = true;
+              go = false;
+              continue;
+            }
+            bool exact = true;
+            for (int q = 0; q < 6; ++q)
+              if (!((held >> q) & 1u) && room[q] != 0 && dn.d[q] >= bd) exact = false;
+            if (!exact) {
+              go = false;
+              continue;
             }
             room[bq] -= 1;
             add[bq] += 1;
             dest[p] = (int32_t)nb[bq];
             ++p;
-            if (p >= lim || (uint64_t)brank[p] >= rank_lim) break;
+            // p is entry p0 + t + 1 now: rk[t] is its rank when it is below lim
+            if (p >= lim || (uint64_t)rk[t] >= rank_lim) go = false;
           }
-          const uint32_t nm = p - head;
-          nmoved += nm;
-          if (nm) {
-            count[c] -= nm;
-            for (int q = 0; q < 6; ++q)
-              if (add[q]) count[nb[q]] += add[q];
-          }
-          src_head[s] = exhausted ? lim : p;
-          pend = !exhausted && p < lim;
         }
+        const uint32_t nm = p - head;
+        nmoved += nm;
+        if (nm) {
+          count[c] -= nm;
+          for (int q = 0; q < 6; ++q)
+            if (add[q]) count[nb[q]] += add[q];
+        }
+        src_head[s] = exhausted ? lim : p;
+        pend = !exhausted && p < lim;
       }
     }
     const uint32_t q = block_append(nnext, pend);
@@ -404,11 +419,11 @@ void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_
   int64_t blocks = (max_pending + kRdBlock - 1) / kRdBlock;
   if (blocks > 4096) blocks = 4096;
   k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, src_rmask,
-                                                       brank, count, mask, nx, ny, nz, cap, dn, resv,
-                                                       second, epoch);
+                                                       brank, count, mask, nx, ny, nz, cap, resv, second,
+                                                       epoch);
   k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_cell, src_head, src_lim,
-                                                      src_rmask, brank, count, mask, nx, ny, nz, cap, dn,
-                                                      resv, second, epoch, dest, (unsigned long long*)(ctr + 6));
+                                                      src_rmask, brank, count, nx, ny, cap, dn, resv, second,
+                                                      epoch, dest, (unsigned long long*)(ctr + 6));
 }
 
 }  // namespace sdr
