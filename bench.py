@@ -116,8 +116,10 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
     """Seed the workload, run warmup + timed substeps; returns (solver, elapsed, iters,
     particle-substeps, sweep timing, breakdown).
 
-    The timed substeps carry HIP events around the MIC(0) sweeps only (the roofline
-    kernel; events around every region cost ~4% of a substep).  `breakdown_steps` more
+    The timed substeps carry HIP events around one call in 16 of each MIC(0) sweep
+    region (the roofline kernel; an event pair delays the next kernel by a few
+    microseconds, so events around every region cost ~4% of a substep and around every
+    sweep ~1.7%).  `breakdown_steps` more
     substeps follow, untimed, with events around every region: their per-region times
     and PCG iterations are the breakdown."""
     from sdr_amd import _abi as A

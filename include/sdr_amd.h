@@ -214,7 +214,8 @@ int sdr_solver_synchronize(sdr_solver* s);
 
 /* Per-kernel timing of the last substep, filled when enabled (HIP events on the
  * solver's stream): on = 1 times every region, on = 2 only the MIC(0) sweep regions
- * (mic_*; fewer events in the stream), 0 stops.  names/ms arrays of length cap; *count
+ * (mic_*), the 16th, 32nd, ... call of each (an event pair delays the next kernel by a
+ * few microseconds), 0 stops.  names/ms arrays of length cap; *count
  * receives the number written. */
 int sdr_solver_enable_timing(sdr_solver* s, int32_t on);
 int sdr_solver_get_timing(sdr_solver* s, char* names, size_t name_len, double* ms,

@@ -289,7 +289,8 @@ class Solver:
         self.lib.solver_synchronize(self._h)
 
     def enable_timing(self, on: bool = True, sweeps_only: bool = False):
-        """Per-region HIP-event timing (sweeps_only: the MIC(0) sweep regions only)."""
+        """Per-region HIP-event timing (sweeps_only: the MIC(0) sweep regions only, one
+        call in 16 of each)."""
         self.lib.solver_enable_timing(self._h, (2 if sweeps_only else 1) if on else 0)
 
     def timing(self) -> dict:

@@ -76,6 +76,26 @@ def test_multi_substep_run_bit_exact(dev, orc):
     _compare_all(o, d, exact=True)
 
 
+def test_timing_modes(dev, orc):
+    """Timing mode 1 brackets every region with events; mode 2 only the MIC(0) sweep
+    regions, one call in 16 of each (the 16th, the 32nd, ...), on the same substep."""
+    sc, o, d1 = paired("c1", orc, FAST)
+    d2 = scenes.make_solver(sc, **FAST)
+    d2.set_particles(o.particles()[0])
+    d1.enable_timing(True)
+    d1.substep(sc.dt)
+    t1 = d1.timing()
+    d2.enable_timing(True, sweeps_only=True)
+    d2.substep(sc.dt)
+    t2 = d2.timing()
+    assert d1.last_pcg_iters == d2.last_pcg_iters > 16
+    assert any(not k.startswith("mic_") for k in t1)
+    sampled = {k for k in t1 if k.startswith("mic_") and t1[k][1] >= 16}
+    assert set(t2) == sampled and "mic_bwd_dot" in t2
+    for k, (ms, calls) in t2.items():
+        assert calls == t1[k][1] // 16 and ms > 0.0, (k, calls, t1[k])
+
+
 def test_cfl_step_bit_exact(dev, orc):
     """Solver::step's CFL sub-stepping (solver.rs:237-281) picks the same substeps."""
     sc, o, d = paired("c1", orc, EXACT, stride=2)
