@@ -72,7 +72,10 @@ class Scene:
     dt: float = 5e-5               # fixed substep of the workload
 
 
-def _cavity_fn(semi, socket_r, blend, ybot_open=True, noise=None, power=2.0):
+def _cavity_fn(semi, socket_r, blend, ybot_open=True, noise=None, power=2.0, sockets=None):
+    """Signed distance of a (super)ellipsoid body joined by a smooth minimum to socket
+    channels running down from y = -ay/2 (one at x = z = 0, or one per (x, z) centre of
+    `sockets`), plus optional seeded wall noise."""
     ax, ay, az = semi
 
     def f(x, y, z):
@@ -82,8 +85,14 @@ def _cavity_fn(semi, socket_r, blend, ybot_open=True, noise=None, power=2.0):
             body = (np.abs(x / ax) ** power + np.abs(y / ay) ** power +
                     np.abs(z / az) ** power) ** (1.0 / power) - 1.0
         body = body * min(semi)
-        radial = np.sqrt(x * x + z * z) - socket_r
         axial = y - (-0.5 * ay)
+        if sockets is None:
+            radial = np.sqrt(x * x + z * z) - socket_r
+        else:
+            radial = None
+            for (sx, sz) in sockets:
+                rr = np.sqrt((x - sx) ** 2 + (z - sz) ** 2) - socket_r
+                radial = rr if radial is None else np.minimum(radial, rr)
         socket = np.maximum(radial, axial)
         phi = smooth_min(body, socket, blend)
         if noise is not None:
@@ -107,6 +116,9 @@ def cavity_scene(level: str) -> Scene:
     c3: 256^3 grid, a box-filling rounded cavity (superellipsoid, power 12) + socket,
         filled to ~130M particles at 8 per cell.
     c3a: 256^3 grid, the literal 60x72x56-cell ellipsoid (~8M particles).
+    c5: 512x512x384 grid at 0.1 mm, an ellipsoid filling the box with three octaves of
+        seeded wall noise and two socket channels, filled to ~400M particles at 8 per
+        cell (the large single scenario).
     """
     fluid = FluidParams()
     solve = SolveParams(200, 1e-6)
@@ -144,6 +156,24 @@ def cavity_scene(level: str) -> Scene:
         outlet = _socket_outlet(origin[1], 8 * dx, dx)
         return Scene(level, sdf, dx, fluid, needle, outlet, solve, (n, n, n),
                      fill_lo=origin[1] + 4 * dx, fill_y=fill_y, dt=2e-5)
+    if level == "c5":
+        dims = (512, 512, 384)
+        dx = 0.1 * MM
+        half = tuple(n * dx / 2.0 for n in dims)
+        origin = tuple(-h for h in half)
+        semi = tuple(0.49 * n * dx for n in dims)
+        socket_r = 12 * dx
+        off = 0.3 * semi[0]
+        fn = _cavity_fn(semi, socket_r, 6 * dx, noise=(3 * dx, 30 * dx, 11, 3),
+                        sockets=((-off, 0.0), (off, 0.0)))
+        sdf = Sdf.from_fn(origin, dx, tuple(n + 1 for n in dims), fn)
+        # a 0.5 mm bore at 4 ml/s: ~5 m/s, half a cell per 1e-5 s substep
+        needle = Needle(tip=(0.0, -0.25 * half[1], -0.5 * half[2]), axis=(0.0, 0.0, 1.0),
+                        radius=5 * dx, flow_rate=4.0 * ML_PER_S)
+        rr = socket_r + 1.5 * dx
+        outlet = Aabb((-off - rr, origin[1] - 2.0 * dx, -rr), (off + rr, origin[1] + 1.5 * dx, rr))
+        return Scene(level, sdf, dx, fluid, needle, outlet, solve, dims, fill_lo=origin[1] + 4 * dx,
+                     fill_y=1e9, dt=1e-5)
     raise ValueError(level)
 
 

@@ -104,12 +104,18 @@ class Sdf:
 
         Node positions are origin + (i*dx, j*dx, k*dx), as in the reference."""
         nx, ny, nz = dims
-        k, j, i = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
-                              np.arange(nx, dtype=np.float64), indexing="ij")
-        x = origin[0] + i * dx
-        y = origin[1] + j * dx
-        z = origin[2] + k * dx
-        data = np.ascontiguousarray(np.asarray(f(x, y, z), dtype=np.float64).reshape(-1))
+        # evaluated a slab of k planes at a time (f is elementwise), so a 10^8-node field
+        # needs a few slabs of temporaries, not a few copies of the whole field
+        data = np.empty(nx * ny * nz, dtype=np.float64)
+        slab = max(1, (1 << 22) // (nx * ny))
+        for k0 in range(0, nz, slab):
+            k1 = min(nz, k0 + slab)
+            k, j, i = np.meshgrid(np.arange(k0, k1, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                                  np.arange(nx, dtype=np.float64), indexing="ij")
+            x = origin[0] + i * dx
+            y = origin[1] + j * dx
+            z = origin[2] + k * dx
+            data[k0 * nx * ny:k1 * nx * ny] = np.asarray(f(x, y, z), dtype=np.float64).reshape(-1)
         return Sdf(tuple(map(float, origin)), float(dx), nx, ny, nz, data)
 
     def bounds(self) -> Aabb:
