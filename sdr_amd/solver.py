@@ -99,15 +99,15 @@ class Sdf:
     data: np.ndarray = field(repr=False)
 
     @staticmethod
-    def from_fn(origin, dx, dims, f) -> "Sdf":
+    def from_fn(origin, dx, dims, f, slab_nodes: int = 1 << 22) -> "Sdf":
         """`Sdf::from_fn` (src/sdf.rs:90-116) for a numpy-vectorised `f(x, y, z)`.
 
-        Node positions are origin + (i*dx, j*dx, k*dx), as in the reference."""
+        Node positions are origin + (i*dx, j*dx, k*dx), as in the reference.  f is
+        evaluated on slabs of whole k planes of about `slab_nodes` nodes (it is
+        elementwise), so a 10^8-node field needs a slab's temporaries, not the field's."""
         nx, ny, nz = dims
-        # evaluated a slab of k planes at a time (f is elementwise), so a 10^8-node field
-        # needs a few slabs of temporaries, not a few copies of the whole field
         data = np.empty(nx * ny * nz, dtype=np.float64)
-        slab = max(1, (1 << 22) // (nx * ny))
+        slab = max(1, slab_nodes // (nx * ny))
         for k0 in range(0, nz, slab):
             k1 = min(nz, k0 + slab)
             k, j, i = np.meshgrid(np.arange(k0, k1, dtype=np.float64), np.arange(ny, dtype=np.float64),

@@ -46,3 +46,31 @@ def test_workload_scenes_build():
         s = scenes.make_solver(sc, lib=oracle.load())
         assert (s.nx, s.ny, s.nz) == dims
         assert sc.outlet.contains((0.0, s.origin[1], 0.0))
+
+
+def test_sdf_slabs_and_socket_channels():
+    """Sdf.from_fn evaluates slab by slab with the same bits as one evaluation over the
+    whole node grid; the two-socket cavity of the large scenario with one socket at the
+    origin is the single-socket cavity, and each of two channels reaches the floor."""
+    from sdr_amd.solver import Sdf
+    dx = 0.1 * scenes.MM
+    dims = (70, 60, 31)
+    origin = (-3.5 * dx * 10, -3.0 * dx * 10, -15.0 * dx * 10)
+    semi = (3.2e-3, 2.7e-3, 13e-3)
+    noise = (3 * dx, 30 * dx, 11, 3)
+    one = scenes._cavity_fn(semi, 0.6e-3, 6 * dx, noise=noise)
+    nx, ny, nz = dims
+    k, j, i = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nx, dtype=np.float64), indexing="ij")
+    whole = one(origin[0] + i * dx, origin[1] + j * dx, origin[2] + k * dx).reshape(-1)
+    for slab_nodes in (1 << 22, 4200 * 4):  # one slab; slabs of 4 planes, the last of 3
+        sd = Sdf.from_fn(origin, dx, dims, one, slab_nodes=slab_nodes)
+        assert np.array_equal(sd.data, whole)
+    same = Sdf.from_fn(origin, dx, dims, scenes._cavity_fn(semi, 0.6e-3, 6 * dx, noise=noise,
+                                                           sockets=((0.0, 0.0),)))
+    assert np.array_equal(same.data, whole)
+    two = scenes._cavity_fn(semi, 0.6e-3, 6 * dx, sockets=((-1.5e-3, 0.0), (1.5e-3, 0.0)))
+    ybot = origin[1]
+    for sx in (-1.5e-3, 1.5e-3):
+        assert two(np.array([sx]), np.array([ybot]), np.array([0.0]))[0] < 0.0
+    assert two(np.array([0.0]), np.array([ybot]), np.array([0.0]))[0] > 0.0
