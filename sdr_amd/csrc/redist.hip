@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__
 __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* nnext,
     const uint32_t* __restrict__ src_cell, uint32_t* src_head, const uint32_t* __restrict__ src_lim,
-    uint16_t* src_rmask, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
+    uint4* rec, uint8_t* rmx, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
     const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, unsigned long long* resv,
     unsigned long long* second, uint32_t epoch) {
   const uint32_t nl = *nlist;
@@ -246,10 +246,15 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nl;
        x += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t s = list[x];
-    const uint32_t head = src_head[s];
-    if (head >= src_lim[s]) continue;
+    const uint32_t head = src_head[s], lim = src_lim[s];
+    if (head >= lim) {
+      rec[x] = make_uint4(0u, lim, lim, 0u);
+      rmx[x] = 0;
+      continue;
+    }
     const int64_t c = src_cell[s];
-    const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
+    const uint32_t hrank = brank[head];
+    const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - hrank);
     const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
     const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
     const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
@@ -263,8 +268,10 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
       const unsigned long long loser = old > val ? val : old;
       if ((uint32_t)(loser >> 32) == epoch) atomicMax(second + nb[q], loser);
     }
-    src_rmask[s] = rm;
-    if (rm == 0) src_head[s] = src_lim[s];  // no receiver left: no later particle can move
+    // what the commit needs of this entry, in list order (coalesced there)
+    rec[x] = make_uint4((uint32_t)c, rm == 0 ? lim : head, lim, hrank);
+    rmx[x] = rm;
+    if (rm == 0) src_head[s] = lim;  // no receiver left: no later particle can move
   }
 }
 
@@ -282,8 +289,7 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
 // class follows the same test.  Sources still pending go to the next round's list.
 __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* next,
-    uint32_t* nnext, const uint32_t* __restrict__ src_cell, uint32_t* src_head,
-    const uint32_t* __restrict__ src_lim, const uint16_t* __restrict__ src_rmask,
+    uint32_t* nnext, uint32_t* src_head, const uint4* __restrict__ rec, const uint8_t* __restrict__ rmx,
     const uint32_t* __restrict__ brank, uint32_t* count, int nx, int ny, uint32_t cap, RedistDirs dn,
     const unsigned long long* __restrict__ resv, const unsigned long long* __restrict__ second,
     uint32_t epoch, int32_t* dest, unsigned long long* moved) {
@@ -296,13 +302,14 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     uint32_t s = 0;
     if (x < nl) {
       s = list[x];
-      const uint32_t head = src_head[s], lim = src_lim[s];
+      const uint4 e = rec[x];  // cell, head, lim, head's rank (written by the reservation)
+      const uint32_t head = e.y, lim = e.z;
       if (head < lim) {
-        const int64_t c = src_cell[s];
-        const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - brank[head]);
+        const int64_t c = e.x;
+        const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - e.w);
         const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
         // the receivers open at the reservation (a closed one stays closed)
-        const uint32_t rm = src_rmask[s];
+        const uint32_t rm = rmx[x];
         uint32_t held = 0;
         unsigned long long comp = 0;  // earliest competing head on a held receiver
         uint32_t room[6];
@@ -407,7 +414,7 @@ void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n,
 
 void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
                       const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
-                      uint16_t* src_rmask, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+                      uint32_t* rec, uint8_t* rmx, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
                       int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
                       unsigned long long* second, uint32_t epoch, int32_t* dest, uint32_t* ctr,
                       hipStream_t st) {
@@ -418,12 +425,13 @@ void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_
   uint32_t* nout = ctr + 3 - parity;
   int64_t blocks = (max_pending + kRdBlock - 1) / kRdBlock;
   if (blocks > 4096) blocks = 4096;
-  k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, src_rmask,
+  uint4* r4 = reinterpret_cast<uint4*>(rec);
+  k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, r4, rmx,
                                                        brank, count, mask, nx, ny, nz, cap, resv, second,
                                                        epoch);
-  k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_cell, src_head, src_lim,
-                                                      src_rmask, brank, count, nx, ny, cap, dn, resv, second,
-                                                      epoch, dest, (unsigned long long*)(ctr + 6));
+  k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_head, r4, rmx, brank, count, nx,
+                                                      ny, cap, dn, resv, second, epoch, dest,
+                                                      (unsigned long long*)(ctr + 6));
 }
 
 }  // namespace sdr
