@@ -171,9 +171,10 @@ void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n,
                        uint32_t* brank, uint32_t* bslot, uint32_t* trank, uint32_t* tslot,
                        uint32_t* big, uint32_t* ctr, hipStream_t st);
 // One reservation round; parity selects which pending list is read (0: a, 1: b).
-void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
-                      const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
-                      uint32_t* rec, uint8_t* rmx, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+void launch_rd_round2(int64_t max_pending, int parity, bool first, uint32_t* pend_a, uint32_t* pend_b,
+                      const uint32_t* src_cell, const uint32_t* src_head, const uint32_t* src_lim,
+                      uint32_t* rec_a, uint32_t* rec_b, uint8_t* rmx, const uint32_t* brank, uint32_t* count,
+                      const uint8_t* mask,
                       int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
                       unsigned long long* second, uint32_t epoch, int32_t* dest, uint32_t* ctr,
                       hipStream_t st);

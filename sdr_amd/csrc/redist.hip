@@ -232,28 +232,39 @@ __global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__
   }
 }
 
+// The first round's records: entry x of the pending list is source list[x] with its
+// cell, head, limit and head rank.
+__global__ void k_rd_rec_init(const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist,
+                              const uint32_t* __restrict__ src_cell, const uint32_t* __restrict__ src_head,
+                              const uint32_t* __restrict__ src_lim, const uint32_t* __restrict__ brank,
+                              uint4* rec) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= *nlist) return;
+  const uint32_t s = list[x], head = src_head[s], lim = src_lim[s];
+  rec[x] = make_uint4(src_cell[s], head, lim, head < lim ? brank[head] : 0u);
+}
+
 // Reservation: a head reserves every open receiver it could still ever write (rooms only
 // shrink, so later particles of its cell touch a subset); the set is kept for the commit.
+// Entry x of the pending list comes as a record (cell, head, limit, head rank) left by
+// the previous round's commit, read and kept in list order.
 __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
-    const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* nnext,
-    const uint32_t* __restrict__ src_cell, uint32_t* src_head, const uint32_t* __restrict__ src_lim,
-    uint4* rec, uint8_t* rmx, const uint32_t* __restrict__ brank, const uint32_t* __restrict__ count,
-    const uint8_t* __restrict__ mask, int nx, int ny, int nz, uint32_t cap, unsigned long long* resv,
-    unsigned long long* second, uint32_t epoch) {
+    const uint32_t* __restrict__ nlist, uint32_t* nnext, uint4* rec, uint8_t* rmx,
+    const uint32_t* __restrict__ count, const uint8_t* __restrict__ mask, int nx, int ny, int nz,
+    uint32_t cap, unsigned long long* resv, unsigned long long* second, uint32_t epoch) {
   const uint32_t nl = *nlist;
   if (blockIdx.x == 0 && threadIdx.x == 0) *nnext = 0;
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nl;
        x += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t s = list[x];
-    const uint32_t head = src_head[s], lim = src_lim[s];
+    const uint4 e = rec[x];
+    const uint32_t head = e.y, lim = e.z;
     if (head >= lim) {
-      rec[x] = make_uint4(0u, lim, lim, 0u);
       rmx[x] = 0;
       continue;
     }
-    const int64_t c = src_cell[s];
-    const uint32_t hrank = brank[head];
+    const int64_t c = e.x;
+    const uint32_t hrank = e.w;
     const unsigned long long val = ((unsigned long long)epoch << 32) | (0xFFFFFFFFu - hrank);
     const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
     const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
@@ -268,10 +279,9 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
       const unsigned long long loser = old > val ? val : old;
       if ((uint32_t)(loser >> 32) == epoch) atomicMax(second + nb[q], loser);
     }
-    // what the commit needs of this entry, in list order (coalesced there)
-    rec[x] = make_uint4((uint32_t)c, rm == 0 ? lim : head, lim, hrank);
+    // no receiver left: no later particle can move
+    if (rm == 0) rec[x] = make_uint4((uint32_t)c, lim, lim, hrank);
     rmx[x] = rm;
-    if (rm == 0) src_head[s] = lim;  // no receiver left: no later particle can move
   }
 }
 
@@ -289,7 +299,7 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_reserve2(
 // class follows the same test.  Sources still pending go to the next round's list.
 __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ nlist, uint32_t* next,
-    uint32_t* nnext, uint32_t* src_head, const uint4* __restrict__ rec, const uint8_t* __restrict__ rmx,
+    uint32_t* nnext, const uint4* __restrict__ rec, uint4* rec_next, const uint8_t* __restrict__ rmx,
     const uint32_t* __restrict__ brank, uint32_t* count, int nx, int ny, uint32_t cap, RedistDirs dn,
     const unsigned long long* __restrict__ resv, const unsigned long long* __restrict__ second,
     uint32_t epoch, int32_t* dest, unsigned long long* moved) {
@@ -300,9 +310,10 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
     const int64_t x = b0 + threadIdx.x;
     bool pend = false;
     uint32_t s = 0;
+    uint4 enext = make_uint4(0u, 0u, 0u, 0u);  // the entry's record for the next round
     if (x < nl) {
       s = list[x];
-      const uint4 e = rec[x];  // cell, head, lim, head's rank (written by the reservation)
+      const uint4 e = rec[x];  // cell, head, lim, head's rank
       const uint32_t head = e.y, lim = e.z;
       if (head < lim) {
         const int64_t c = e.x;
@@ -328,7 +339,7 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
         }
         const uint64_t rank_lim = comp ? (uint64_t)(0xFFFFFFFFu - (uint32_t)comp) : (1ull << 32);
         uint32_t add[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t p = head;
+        uint32_t p = head, prank = e.w;  // the next particle and its rank
         bool exhausted = false, go = true;
         while (go) {
           // the ranks of the next kRun bucket entries, loaded together rather than one
@@ -367,7 +378,8 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
             dest[p] = (int32_t)nb[bq];
             ++p;
             // p is entry p0 + t + 1 now: rk[t] is its rank when it is below lim
-            if (p >= lim || (uint64_t)rk[t] >= rank_lim) go = false;
+            prank = rk[t];
+            if (p >= lim || (uint64_t)prank >= rank_lim) go = false;
           }
         }
         const uint32_t nm = p - head;
@@ -377,12 +389,12 @@ __global__ void __launch_bounds__(kRdBlock) k_rd_commit2(
           for (int q = 0; q < 6; ++q)
             if (add[q]) count[nb[q]] += add[q];
         }
-        src_head[s] = exhausted ? lim : p;
         pend = !exhausted && p < lim;
+        enext = make_uint4((uint32_t)c, p, lim, prank);
       }
     }
     const uint32_t q = block_append(nnext, pend);
-    if (pend) next[q] = s;
+    if (pend) next[q] = s, rec_next[q] = enext;
   }
   block_add64(moved, nmoved);
 }
@@ -412,9 +424,10 @@ void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n,
   k_rd_sort_big<<<256, 64, 0, st>>>(ctr + 4, big, src_cell, src_off, count, brank, bslot, trank, tslot);
 }
 
-void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_t* pend_b,
-                      const uint32_t* src_cell, uint32_t* src_head, const uint32_t* src_lim,
-                      uint32_t* rec, uint8_t* rmx, const uint32_t* brank, uint32_t* count, const uint8_t* mask,
+void launch_rd_round2(int64_t max_pending, int parity, bool first, uint32_t* pend_a, uint32_t* pend_b,
+                      const uint32_t* src_cell, const uint32_t* src_head, const uint32_t* src_lim,
+                      uint32_t* rec_a, uint32_t* rec_b, uint8_t* rmx, const uint32_t* brank, uint32_t* count,
+                      const uint8_t* mask,
                       int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
                       unsigned long long* second, uint32_t epoch, int32_t* dest, uint32_t* ctr,
                       hipStream_t st) {
@@ -425,12 +438,15 @@ void launch_rd_round2(int64_t max_pending, int parity, uint32_t* pend_a, uint32_
   uint32_t* nout = ctr + 3 - parity;
   int64_t blocks = (max_pending + kRdBlock - 1) / kRdBlock;
   if (blocks > 4096) blocks = 4096;
-  uint4* r4 = reinterpret_cast<uint4*>(rec);
-  k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, nout, src_cell, src_head, src_lim, r4, rmx,
-                                                       brank, count, mask, nx, ny, nz, cap, resv, second,
-                                                       epoch);
-  k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, src_head, r4, rmx, brank, count, nx,
-                                                      ny, cap, dn, resv, second, epoch, dest,
+  // the records pair with the lists: rec_a with pend_a, rec_b with pend_b
+  uint4* rin = reinterpret_cast<uint4*>(parity ? rec_b : rec_a);
+  uint4* rout = reinterpret_cast<uint4*>(parity ? rec_a : rec_b);
+  if (first)
+    k_rd_rec_init<<<rd_grid(max_pending), kRdBlock, 0, st>>>(in, nin, src_cell, src_head, src_lim, brank, rin);
+  k_rd_reserve2<<<(unsigned)blocks, kRdBlock, 0, st>>>(nin, nout, rin, rmx, count, mask, nx, ny, nz, cap, resv,
+                                                       second, epoch);
+  k_rd_commit2<<<(unsigned)blocks, kRdBlock, 0, st>>>(in, nin, out, nout, rin, rout, rmx, brank, count, nx, ny,
+                                                      cap, dn, resv, second, epoch, dest,
                                                       (unsigned long long*)(ctr + 6));
 }
 

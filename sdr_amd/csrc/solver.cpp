@@ -906,7 +906,7 @@ struct Solver : SolverBase {
       for (Buf* b : {&rd_sidx, &rd_boff, &rd_src_cell, &rd_src_off, &rd_src_head, &rd_src_lim, &rd_src_fill,
                      &rd_pend_a, &rd_pend_b, &rd_big})
         b->alloc((size_t)nc * 4);
-      rd_rec.alloc((size_t)nc * 16);  // per pending-list entry: cell, head, lim, head rank
+      rd_rec.alloc((size_t)nc * 32);  // per entry of either pending list: cell, head, lim, head rank
       rd_rmx.alloc((size_t)nc);       // and its open receivers
     }
     if (!rd_ctr.p) rd_ctr.alloc(64);
@@ -941,9 +941,10 @@ struct Solver : SolverBase {
           HIPCHK(hipMemsetAsync(rd_second.p, 0, rd_second.bytes, st));
           rd_epoch = 1;
         }
-        launch_rd_round2(pending, (int)(rounds & 1), rd_pend_a.as<uint32_t>(), rd_pend_b.as<uint32_t>(),
-                         rd_src_cell.as<uint32_t>(), rd_src_head.as<uint32_t>(), rd_src_lim.as<uint32_t>(),
-                         rd_rec.as<uint32_t>(), rd_rmx.as<uint8_t>(), brank, count.as<uint32_t>(),
+        launch_rd_round2(pending, (int)(rounds & 1), rounds == 0, rd_pend_a.as<uint32_t>(),
+                         rd_pend_b.as<uint32_t>(), rd_src_cell.as<uint32_t>(), rd_src_head.as<uint32_t>(),
+                         rd_src_lim.as<uint32_t>(), rd_rec.as<uint32_t>(), rd_rec.as<uint32_t>() + 4 * nc,
+                         rd_rmx.as<uint8_t>(), brank, count.as<uint32_t>(),
                          mask.as<uint8_t>(), nx, ny, nz,
                          capc, dn, rd_resv.as<unsigned long long>(), rd_second.as<unsigned long long>(), rd_epoch,
                          dest.as<int32_t>(), ctr, st);
@@ -1210,7 +1211,7 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
     // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
     // vectors (f32 path), halo words of the pipelined sweeps
-    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 73.0;
+    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 89.0;
     const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;
