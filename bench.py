@@ -327,9 +327,9 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the substep runs "
-              "on one device; see DESIGN.md for how it would shard). Use --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the substep runs on one "
+              "device; scenarios would run one per GPU, and a single large grid would split "
+              "into k-slabs with halo exchange). Use --gpus 1.", file=sys.stderr)
         return 2
 
     import sdr_amd
