@@ -209,6 +209,105 @@ int sdr_solver_get_pressure(sdr_solver* s, double* p);
 int sdr_solver_get_state(const sdr_solver* s, sdr_state* st);
 int sdr_solver_set_state(sdr_solver* s, const sdr_state* st);
 
+/* The reference's pub diagnostics of `Solver`, computed on the device in one call
+ * (no particle or cell download):
+ *   particle_volume         solver.rs:225-228   dx^3 / max(ppc, 1)
+ *   fluid_volume            solver.rs:231-233   particles * particle_volume
+ *   drained_volume          solver.rs:758-760   drained * particle_volume
+ *   cells_fluid_count       solver.rs:763-765   cells classified Fluid
+ *   max_particles_per_cell  solver.rs:770-782   largest per-cell count (cell_of, all cells)
+ *   max_pressure            solver.rs:859-867   max(0, pressure over Fluid cells)
+ *   mean_pressure           solver.rs:869-882   mean pressure over Fluid cells (0 if none)
+ *   max_speed               solver.rs:884-890   max particle |v|
+ *   kinetic_energy          particles.rs:52-58  0.5 * sum |v|^2
+ * Counts and maxima are exact; the two sums are deterministic trees (the reference sums
+ * sequentially), equal to it within rounding. */
+typedef struct sdr_diagnostics {
+  double particle_volume;
+  double fluid_volume;
+  double drained_volume;
+  uint64_t cells_fluid_count;
+  uint64_t max_particles_per_cell;
+  double max_pressure;
+  double mean_pressure;
+  double max_speed;
+  double kinetic_energy;
+  uint64_t particles;
+} sdr_diagnostics;
+int sdr_solver_diagnostics(sdr_solver* s, sdr_diagnostics* out);
+
+/* The reference's private `Solver::cfl_dt` (solver.rs:262-281, with
+ * `MacGrid::max_speed`, grid.rs:202-206) on the current grid: the sub-step `step` would
+ * take next before its floor and the frame remainder apply. */
+int sdr_solver_cfl_dt(sdr_solver* s, double* dt);
+
+/* ---------------------------------------------------------------------------------
+ * Clinical metrics (`MetricsCollector`, metrics.rs:116-446) on the device.  The
+ * collector marks wall cells (cavity cells with a non-cavity or out-of-domain face
+ * neighbour, metrics.rs:119-182) once; each record marks the wall cells holding
+ * particles as washed and reduces the frame's metrics on the device, so a frame costs
+ * one pass over the particles and one over the cells and no download.  Sums over wet
+ * wall cells and fluid cells run in a fixed tree order (the reference sums wet wall
+ * cells sequentially in cell-index order, metrics.rs:189-196): deterministic, equal to
+ * the reference within rounding.  mass_balance uses the integer counts
+ * emitted = resident + drained + culled instead of the reference's resident+drained
+ * over itself (metrics.rs:220,355-362, which is 1 by construction). */
+typedef struct sdr_frame_metrics {  /* `FrameMetrics` (metrics.rs:27-67) */
+  uint64_t frame;
+  double time_s;
+  uint64_t particles;
+  double fluid_volume_ml;
+  double drained_volume_ml;
+  double fill_fraction;
+  double wall_coverage;
+  double mean_pressure_pa;
+  double max_pressure_pa;
+  double max_pressure_mmhg;
+  double wall_pressure_pa;
+  double peak_wall_pressure_pa;
+  double peak_wall_pressure_mmhg;
+  uint64_t membrane_artifacts;
+  double max_speed_m_s;
+  double kinetic_energy;
+  uint64_t pcg_iters;
+  uint64_t wet_wall_cells;  /* wall cells holding particles this frame */
+  uint64_t emitted, drained, culled;  /* integer conservation counters at this frame */
+} sdr_frame_metrics;
+
+typedef struct sdr_summary {  /* `Summary` (metrics.rs:71-107) */
+  uint64_t frames;
+  double final_time_s;
+  double peak_fill_fraction;
+  double final_fill_fraction;
+  double peak_wall_coverage;
+  double final_wall_coverage;
+  double peak_membrane_pressure_pa;
+  double peak_membrane_pressure_mmhg;
+  double mean_membrane_pressure_pa;
+  double mean_membrane_pressure_mmhg;
+  double peak_pressure_pa;
+  double peak_pressure_mmhg;
+  uint64_t membrane_artifacts_total;
+  double total_drained_ml;
+  double injected_ml;
+  double mass_balance;  /* (resident + drained + culled) / emitted at the last frame; 1 if none */
+  uint64_t emitted, resident, drained, culled;
+} sdr_summary;
+
+typedef struct sdr_metrics sdr_metrics;
+/* `MetricsCollector::new(solver, cavity_volume)` (metrics.rs:119-182). */
+int sdr_metrics_new(sdr_solver* s, double cavity_volume, sdr_metrics** out);
+int sdr_metrics_free(sdr_metrics* m);
+/* Number of wall cells (`wall_count`, metrics.rs:127). */
+int sdr_metrics_wall_count(const sdr_metrics* m, uint64_t* n);
+/* `MetricsCollector::record(solver, frame)` (metrics.rs:188-316); s must be the solver
+ * the collector was made for. */
+int sdr_metrics_record(sdr_metrics* m, sdr_solver* s, uint64_t frame, sdr_frame_metrics* out);
+/* `MetricsCollector::summary()` (metrics.rs:324-382) over the recorded frames. */
+int sdr_metrics_summary(const sdr_metrics* m, sdr_summary* out);
+/* Washed flags per cell (0/1, cell order) and wall flags, for checks. */
+int sdr_metrics_get_cells(const sdr_metrics* m, uint8_t* washed, uint8_t* is_wall);
+
 /* Block until every operation queued on the solver's stream has finished. */
 int sdr_solver_synchronize(sdr_solver* s);
 

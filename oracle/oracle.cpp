@@ -1006,6 +1006,9 @@ int orc_solver_get_particles(void* h, double* pos, double* vel, uint64_t n) {
 }
 int orc_solver_set_particles(void* h, const double* pos, const double* vel, uint64_t n) {
   Solver* s = (Solver*)h;
+  // the replaced particles leave the books, the new ones enter them, so that
+  // emitted = resident + drained + culled keeps holding
+  s->emitted = s->emitted - s->pos.size() + n;
   s->pos.resize(n);
   s->vel.resize(n);
   for (uint64_t t = 0; t < n; ++t) {
@@ -1182,6 +1185,243 @@ int orc_extrapolate(double* field, const uint8_t* valid, uint64_t nx, uint64_t n
   std::vector<uint8_t> v(valid, valid + nx * ny * nz);
   Solver::extrapolate(f, v, Dims{nx, ny, nz}, iters);
   memcpy(field, f.data(), f.size() * 8);
+  return 0;
+}
+
+// ---- Solver diagnostics (solver.rs:225-233, 758-891; particles.rs:52-58)
+int orc_solver_cfl_dt(void* h, double* out) {  // solver.rs:262-281
+  *out = ((Solver*)h)->cfl_dt();
+  return 0;
+}
+int orc_solver_diagnostics(void* h, sdr_diagnostics* d) {
+  Solver* s = (Solver*)h;
+  const uint64_t nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz, n = nx * ny * nz;
+  memset(d, 0, sizeof *d);
+  d->particle_volume = s->particle_volume();                              // :225-228
+  d->fluid_volume = (double)s->pos.size() * s->particle_volume();         // :231-233
+  d->drained_volume = (double)s->drained * s->particle_volume();          // :758-760
+  for (uint64_t c = 0; c < n; ++c) d->cells_fluid_count += s->cells[c] == FLUID;  // :763-765
+  std::vector<uint32_t> count(n, 0);                                       // :770-782
+  uint32_t mx = 0;
+  for (const V3& p : s->pos) {
+    uint64_t i, j, k;
+    if (s->cell_of(p, i, j, k)) {
+      uint64_t c = (k * ny + j) * nx + i;
+      count[c] += 1;
+      mx = count[c] > mx ? count[c] : mx;
+    }
+  }
+  d->max_particles_per_cell = mx;
+  double pmax = 0.0, psum = 0.0;                                           // :859-882
+  uint64_t pn = 0;
+  for (uint64_t c = 0; c < n; ++c) {
+    if (s->cells[c] != FLUID) continue;
+    pmax = rmax(pmax, s->pressure[c]);
+    psum = psum + s->pressure[c];
+    pn += 1;
+  }
+  d->max_pressure = pmax;
+  d->mean_pressure = pn > 0 ? psum / (double)pn : 0.0;
+  double vmax = 0.0, ke = 0.0;                                             // :884-890
+  for (const V3& v : s->vel) {
+    vmax = rmax(vmax, length(v));
+    ke = ke + dot(v, v);
+  }
+  d->max_speed = vmax;
+  d->kinetic_energy = 0.5 * ke;                                            // particles.rs:52-58
+  d->particles = s->pos.size();
+  return 0;
+}
+
+// ---- MetricsCollector (metrics.rs:116-446)
+namespace {
+constexpr double kM3ToMl = 1.0e6;              // metrics.rs:23
+constexpr double kPaToMmHg = 1.0 / 133.322;    // metrics.rs:24
+constexpr double kMembraneCapFactor = 3.0;     // metrics.rs:454
+
+// metrics.rs:460-479
+void membrane_stats(const double* v, uint64_t n, double cap, double* mean, double* peak, uint64_t* art) {
+  double sum = 0.0, pk = 0.0;
+  uint64_t m = 0, a = 0;
+  for (uint64_t t = 0; t < n; ++t) {
+    double p = v[t];
+    if (p > cap) {
+      a += 1;
+      continue;
+    }
+    sum += p;
+    pk = rmax(pk, p);
+    m += 1;
+  }
+  if (m == 0) *mean = 0.0, *peak = 0.0;
+  else *mean = sum / (double)m, *peak = pk;
+  *art = a;
+}
+
+struct Metrics {
+  double cavity_volume = 0;
+  uint64_t nx = 0, ny = 0, nz = 0;
+  V3 origin{0, 0, 0};
+  double dx = 0;
+  std::vector<uint8_t> is_wall, washed;
+  uint64_t wall_count = 0;
+  std::vector<sdr_frame_metrics> history;
+  double injected_ml = 0;
+
+  void create(const Solver& s, double cv) {  // metrics.rs:119-182
+    cavity_volume = cv;
+    nx = s.grid.nx, ny = s.grid.ny, nz = s.grid.nz, origin = s.grid.origin, dx = s.grid.dx;
+    const uint64_t n = nx * ny * nz;
+    std::vector<uint8_t> inside(n);
+    for (uint64_t k = 0; k < nz; ++k)
+      for (uint64_t j = 0; j < ny; ++j)
+        for (uint64_t i = 0; i < nx; ++i)
+          inside[(k * ny + j) * nx + i] = s.solid.sample(s.grid.cell_center(i, j, k)) < 0.0;
+    is_wall.assign(n, 0);
+    for (uint64_t k = 0; k < nz; ++k)
+      for (uint64_t j = 0; j < ny; ++j)
+        for (uint64_t i = 0; i < nx; ++i) {
+          uint64_t c = (k * ny + j) * nx + i;
+          if (!inside[c]) continue;
+          auto nb_solid = [&](int64_t ci, int64_t cj, int64_t ck) {
+            if (ci < 0 || cj < 0 || ck < 0 || (uint64_t)ci >= nx || (uint64_t)cj >= ny || (uint64_t)ck >= nz)
+              return true;
+            return !inside[((uint64_t)ck * ny + (uint64_t)cj) * nx + (uint64_t)ci];
+          };
+          int64_t a = (int64_t)i, b = (int64_t)j, e = (int64_t)k;
+          bool touches = nb_solid(a - 1, b, e) || nb_solid(a + 1, b, e) || nb_solid(a, b - 1, e) ||
+                         nb_solid(a, b + 1, e) || nb_solid(a, b, e - 1) || nb_solid(a, b, e + 1);
+          if (touches) is_wall[c] = 1, wall_count += 1;
+        }
+    washed.assign(n, 0);
+  }
+  bool cell_index(V3 p, uint64_t& c) const {  // metrics.rs:434-445
+    V3 l = (p - origin) / dx;
+    if (l.x < 0.0 || l.y < 0.0 || l.z < 0.0) return false;
+    uint64_t i = as_usize(l.x), j = as_usize(l.y), k = as_usize(l.z);
+    if (i < nx && j < ny && k < nz) {
+      c = (k * ny + j) * nx + i;
+      return true;
+    }
+    return false;
+  }
+  sdr_frame_metrics record(Solver& s, uint64_t frame) {  // metrics.rs:188-316
+    // the wet wall set in ascending cell order (the reference's BTreeSet)
+    std::vector<uint8_t> wet(nx * ny * nz, 0);
+    for (const V3& p : s.pos) {
+      uint64_t c;
+      if (cell_index(p, c) && is_wall[c]) washed[c] = 1, wet[c] = 1;
+    }
+    uint64_t washed_count = 0;
+    for (uint8_t w : washed) washed_count += w;
+    double coverage = wall_count > 0 ? (double)washed_count / (double)wall_count : 0.0;
+    double fluid_volume = (double)s.pos.size() * s.particle_volume();
+    double drained_volume = (double)s.drained * s.particle_volume();
+    double fill = cavity_volume > 0.0 ? fluid_volume / cavity_volume : 0.0;
+    sdr_diagnostics dg;
+    orc_solver_diagnostics(&s, &dg);
+    double max_p = dg.max_pressure;
+    injected_ml = (fluid_volume + drained_volume) * kM3ToMl;
+    double v_jet = s.needle.jet_speed();
+    double stagnation = 0.5 * s.fluid.density * v_jet * v_jet;
+    double hydrostatic = s.fluid.density * length(s.fluid.gravity) * ((double)ny * dx);
+    double p_cap = kMembraneCapFactor * stagnation + hydrostatic + 500.0;
+    std::vector<double> wall_p;
+    for (uint64_t c = 0; c < wet.size(); ++c)
+      if (wet[c]) wall_p.push_back(s.pressure[c]);
+    double wall_mean, wall_peak;
+    uint64_t art;
+    membrane_stats(wall_p.data(), wall_p.size(), p_cap, &wall_mean, &wall_peak, &art);
+    sdr_frame_metrics m;
+    memset(&m, 0, sizeof m);
+    m.frame = frame;
+    m.time_s = s.time;
+    m.particles = s.pos.size();
+    m.fluid_volume_ml = fluid_volume * kM3ToMl;
+    m.drained_volume_ml = drained_volume * kM3ToMl;
+    m.fill_fraction = fill;
+    m.wall_coverage = coverage;
+    m.mean_pressure_pa = dg.mean_pressure;
+    m.max_pressure_pa = max_p;
+    m.max_pressure_mmhg = max_p * kPaToMmHg;
+    m.wall_pressure_pa = wall_mean;
+    m.peak_wall_pressure_pa = wall_peak;
+    m.peak_wall_pressure_mmhg = wall_peak * kPaToMmHg;
+    m.membrane_artifacts = art;
+    m.max_speed_m_s = dg.max_speed;
+    m.kinetic_energy = dg.kinetic_energy;
+    m.pcg_iters = s.last_pcg_iters;
+    m.wet_wall_cells = wall_p.size();
+    m.emitted = s.emitted, m.drained = s.drained, m.culled = s.culled;
+    history.push_back(m);
+    return m;
+  }
+  sdr_summary summary() const {  // metrics.rs:324-382
+    sdr_summary r;
+    memset(&r, 0, sizeof r);
+    double peak_fill = 0, peak_cov = 0, peak_p = 0, peak_mem = 0, mean_mem_sum = 0;
+    uint64_t art = 0;
+    for (const auto& m : history) {
+      peak_fill = rmax(peak_fill, m.fill_fraction);
+      peak_cov = rmax(peak_cov, m.wall_coverage);
+      peak_p = rmax(peak_p, m.max_pressure_pa);
+      peak_mem = rmax(peak_mem, m.peak_wall_pressure_pa);
+      mean_mem_sum = mean_mem_sum + m.wall_pressure_pa;
+      art += m.membrane_artifacts;
+    }
+    r.frames = history.size();
+    double mean_mem = history.empty() ? 0.0 : mean_mem_sum / (double)history.size();
+    if (!history.empty()) {
+      const auto& l = history.back();
+      r.final_fill_fraction = l.fill_fraction, r.final_wall_coverage = l.wall_coverage;
+      r.final_time_s = l.time_s;
+      r.total_drained_ml = l.drained_volume_ml;
+      r.injected_ml = l.fluid_volume_ml + l.drained_volume_ml;
+      r.emitted = l.emitted, r.resident = l.particles, r.drained = l.drained, r.culled = l.culled;
+    }
+    r.peak_fill_fraction = peak_fill, r.peak_wall_coverage = peak_cov;
+    r.peak_membrane_pressure_pa = peak_mem, r.peak_membrane_pressure_mmhg = peak_mem * kPaToMmHg;
+    r.mean_membrane_pressure_pa = mean_mem, r.mean_membrane_pressure_mmhg = mean_mem * kPaToMmHg;
+    r.peak_pressure_pa = peak_p, r.peak_pressure_mmhg = peak_p * kPaToMmHg;
+    r.membrane_artifacts_total = art;
+    // conservation from the integer counts: emitted = resident + drained + culled
+    r.mass_balance = r.emitted > 0 ? (double)(r.resident + r.drained + r.culled) / (double)r.emitted : 1.0;
+    return r;
+  }
+};
+}  // namespace
+
+int orc_membrane_stats(const double* v, uint64_t n, double cap, double* mean, double* peak, uint64_t* art) {
+  membrane_stats(v, n, cap, mean, peak, art);
+  return 0;
+}
+int orc_metrics_new(void* solver, double cavity_volume, void** out) {
+  Metrics* m = new Metrics();
+  m->create(*(Solver*)solver, cavity_volume);
+  *out = m;
+  return 0;
+}
+int orc_metrics_free(void* m) {
+  delete (Metrics*)m;
+  return 0;
+}
+int orc_metrics_wall_count(const void* m, uint64_t* n) {
+  *n = ((const Metrics*)m)->wall_count;
+  return 0;
+}
+int orc_metrics_record(void* m, void* solver, uint64_t frame, sdr_frame_metrics* out) {
+  sdr_frame_metrics r = ((Metrics*)m)->record(*(Solver*)solver, frame);
+  if (out) *out = r;
+  return 0;
+}
+int orc_metrics_summary(const void* m, sdr_summary* out) {
+  *out = ((const Metrics*)m)->summary();
+  return 0;
+}
+int orc_metrics_get_cells(const void* m, uint8_t* washed, uint8_t* is_wall) {
+  const Metrics* mm = (const Metrics*)m;
+  if (washed) memcpy(washed, mm->washed.data(), mm->washed.size());
+  if (is_wall) memcpy(is_wall, mm->is_wall.data(), mm->is_wall.size());
   return 0;
 }
 

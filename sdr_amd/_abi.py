@@ -106,6 +106,43 @@ class ScenarioResult(C.Structure):
                 ("pcg_iters", C.c_uint64), ("checksum", C.c_uint64)]
 
 
+class Diagnostics(C.Structure):
+    _fields_ = [("particle_volume", C.c_double), ("fluid_volume", C.c_double),
+                ("drained_volume", C.c_double), ("cells_fluid_count", C.c_uint64),
+                ("max_particles_per_cell", C.c_uint64), ("max_pressure", C.c_double),
+                ("mean_pressure", C.c_double), ("max_speed", C.c_double),
+                ("kinetic_energy", C.c_double), ("particles", C.c_uint64)]
+
+
+class FrameMetrics(C.Structure):
+    _fields_ = [("frame", C.c_uint64), ("time_s", C.c_double), ("particles", C.c_uint64),
+                ("fluid_volume_ml", C.c_double), ("drained_volume_ml", C.c_double),
+                ("fill_fraction", C.c_double), ("wall_coverage", C.c_double),
+                ("mean_pressure_pa", C.c_double), ("max_pressure_pa", C.c_double),
+                ("max_pressure_mmhg", C.c_double), ("wall_pressure_pa", C.c_double),
+                ("peak_wall_pressure_pa", C.c_double), ("peak_wall_pressure_mmhg", C.c_double),
+                ("membrane_artifacts", C.c_uint64), ("max_speed_m_s", C.c_double),
+                ("kinetic_energy", C.c_double), ("pcg_iters", C.c_uint64),
+                ("wet_wall_cells", C.c_uint64), ("emitted", C.c_uint64), ("drained", C.c_uint64),
+                ("culled", C.c_uint64)]
+
+
+class Summary(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("final_time_s", C.c_double),
+                ("peak_fill_fraction", C.c_double), ("final_fill_fraction", C.c_double),
+                ("peak_wall_coverage", C.c_double), ("final_wall_coverage", C.c_double),
+                ("peak_membrane_pressure_pa", C.c_double), ("peak_membrane_pressure_mmhg", C.c_double),
+                ("mean_membrane_pressure_pa", C.c_double), ("mean_membrane_pressure_mmhg", C.c_double),
+                ("peak_pressure_pa", C.c_double), ("peak_pressure_mmhg", C.c_double),
+                ("membrane_artifacts_total", C.c_uint64), ("total_drained_ml", C.c_double),
+                ("injected_ml", C.c_double), ("mass_balance", C.c_double), ("emitted", C.c_uint64),
+                ("resident", C.c_uint64), ("drained", C.c_uint64), ("culled", C.c_uint64)]
+
+
+def struct_dict(st) -> dict:
+    return {name: getattr(st, name) for name, _ in st._fields_}
+
+
 P = C.c_void_p
 # name -> argtypes (restype is always c_int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -136,6 +173,14 @@ SIGNATURES = {
     "project_capturing_with_source": [C.POINTER(Dims), c_double_p, c_double_p, c_double_p, c_u8_p,
                                       C.c_double, C.c_double, C.POINTER(SolveParamsC), c_double_p,
                                       c_double_p, C.POINTER(SolveReportC), C.POINTER(Options)],
+    "solver_diagnostics": [P, C.POINTER(Diagnostics)],
+    "solver_cfl_dt": [P, c_double_p],
+    "metrics_new": [P, C.c_double, C.POINTER(P)],
+    "metrics_free": [P],
+    "metrics_wall_count": [P, c_u64_p],
+    "metrics_record": [P, P, C.c_uint64, C.POINTER(FrameMetrics)],
+    "metrics_summary": [P, C.POINTER(Summary)],
+    "metrics_get_cells": [P, c_u8_p, c_u8_p],
 }
 # Entry points only the device library has.
 DEVICE_ONLY = {

@@ -357,12 +357,17 @@ __device__ inline void for_row_pairs(Shear sh, F&& f) {
   }
 }
 
+// max of the residual norms, propagating NaN (f64::max in the reference ignores NaN, so
+// a NaN residual could pass for converged there); equal to fmax for numbers
+__device__ inline double nmax(double a, double b) {
+  return (a != a || b != b) ? __longlong_as_double(0x7ff8000000000000ll) : fmax(a, b);
+}
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 __device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
   return v;
 }
 
@@ -373,7 +378,7 @@ __device__ inline void block_partial(double v, double* partials) {
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0)
-    partials[blockIdx.x] = OPMAX ? fmax(fmax(fmax(red[0], red[1]), red[2]), red[3])
+    partials[blockIdx.x] = OPMAX ? nmax(nmax(nmax(red[0], red[1]), red[2]), red[3])
                                  : ((red[0] + red[1]) + red[2]) + red[3];
 }
 
@@ -483,25 +488,16 @@ inline int unit_span_len(Shear sh, int nblocks, int U) {
   return (int)(((nunits + waves - 1) / waves + U - 1) / U * U);
 }
 
-// UPD: the product of the previous iteration's search-direction update: x is s, and
-// the operand is s' = z + beta s (pressure.rs:386-388) formed at the cell and at each
-// neighbour (the same arithmetic as k_update_s, so the same bits); s' is stored at the
-// cell of another buffer (snew: x is still read by the neighbours' threads).  That
-// saves update_s's pass over s and z.
-template <class T, int U, bool UPD = false>
+template <class T, int U>
 __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __restrict__ flags,
                                                     const T* __restrict__ x, T* __restrict__ out,
                                                     const PcgScalars* sc, int it, double* partials,
-                                                    int span, const T* __restrict__ z = nullptr,
-                                                    T* __restrict__ snew = nullptr) {
+                                                    int span) {
   if (sc->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
-  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ),
-               rz = make_rsrc(UPD ? z : nullptr, UPD ? n * SZ : 0u),
-               rsn = make_rsrc(UPD ? snew : nullptr, UPD ? n * SZ : 0u);
-  const T beta = UPD ? (T)sc->beta : T(0);
+  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ);
   const int nx = sh.nx, ny = sh.ny, nz = sh.nz;
   const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)sh.NJ;
   int q0, q1;
@@ -510,7 +506,7 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
   for (int q = q0; q < q1; q += U) {
     UnitLane L[U];
     uint16_t f[U];
-    T xc[U], xn[U][6], zc[U], zn[U][6];
+    T xc[U], xn[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       L[u] = unit_slot(sh, q + u, q1);
@@ -528,20 +524,6 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
       xc[u] = bld<T>(rx, t * SZ);
 #pragma unroll
       for (int d = 0; d < 6; ++d) xn[u][d] = bld<T>(rx, on[d] * SZ);
-      if (UPD) {
-        zc[u] = bld<T>(rz, t * SZ);
-#pragma unroll
-        for (int d = 0; d < 6; ++d) zn[u][d] = bld<T>(rz, on[d] * SZ);
-      }
-    }
-    if (UPD) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        xc[u] = zc[u] + beta * xc[u];
-#pragma unroll
-        for (int d = 0; d < 6; ++d) xn[u][d] = zn[u][d] + beta * xn[u][d];
-        bst<T>(rsn, L[u].t * SZ, xc[u]);
-      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -634,11 +616,11 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
   double v;
   if (op == kOpResidual || op == kOpRhsNorm) {
     double m = 0.0;
-    for (int q = threadIdx.x; q < nblocks; q += 256) m = fmax(m, partials[q]);
+    for (int q = threadIdx.x; q < nblocks; q += 256) m = nmax(m, partials[q]);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    v = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+    v = nmax(nmax(nmax(red[0], red[1]), red[2]), red[3]);
   } else if (exact) {
     // Iterator::sum over the chunk partials, in chunk order.
     if (threadIdx.x != 0) return;
@@ -653,6 +635,11 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
     v = ((red[0] + red[1]) + red[2]) + red[3];
   }
   if (threadIdx.x != 0) return;
+  if (v != v) {  // a NaN in the vectors reaches every dot and max: stop, report it
+    sc->nan_seen = 1;
+    sc->done = 1;
+    return;
+  }
   switch (op) {
     case kOpSigma0:
       sc->sigma = v;
@@ -671,7 +658,6 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
       break;
     case kOpResidual:
       sc->residual = v;
-      if (v != v) sc->nan_seen = 1;
       if (v <= sc->tol_eff) sc->done = 1;
       break;
     case kOpRhsNorm:
@@ -689,7 +675,7 @@ __global__ void __launch_bounds__(256) k_maxabs(Shear sh, const T* __restrict__ 
                                                 const PcgScalars* sc) {
   if (sc->done) return;
   double m = 0.0;
-  for_cells(sh, [&](int64_t t, int, int, int) { m = fmax(m, fabs((double)a[t])); });
+  for_cells(sh, [&](int64_t t, int, int, int) { m = nmax(m, fabs((double)a[t])); });
   block_partial<1>(m, partials);
 }
 
@@ -727,13 +713,13 @@ __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T
       p[ta] = pa + alpha * sa;
       const T rv = ra - alpha * qa;
       r[ta] = rv;
-      m = fmax(m, fabs((double)rv));
+      m = nmax(m, fabs((double)rv));
     }
     if (bo) {
       p[tb] = pb + alpha * sb;
       const T rv = rb - alpha * qb;
       r[tb] = rv;
-      m = fmax(m, fabs((double)rv));
+      m = nmax(m, fabs((double)rv));
     }
   });
   block_partial<1>(m, partials);
@@ -794,7 +780,7 @@ __global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__
       bst<T>(rp, t[u] * SZ, pv[u] + alpha * sv[u]);
       const T nr = rv[u] - alpha * qv[u];
       bst<T>(rr, t[u] * SZ, nr);
-      m = fmax(m, fabs((double)nr));  // 0 outside the grid
+      m = nmax(m, fabs((double)nr));  // 0 outside the grid
     }
   }
   block_partial<1>(m, partials);
@@ -950,22 +936,22 @@ __global__ void __launch_bounds__(256) k_residual64(Shear sh, const uint16_t* __
       v = b[t] - ax;
     }
     r[t] = v;
-    m = fmax(m, fabs(v));
+    m = nmax(m, fabs(v));
   }
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+  if (threadIdx.x == 0) partials[blockIdx.x] = nmax(nmax(nmax(red[0], red[1]), red[2]), red[3]);
 }
 
 __global__ void __launch_bounds__(256) k_max_finish(const double* partials, int nblocks, double* out) {
   __shared__ double red[4];
   double m = 0.0;
-  for (int q = threadIdx.x; q < nblocks; q += 256) m = fmax(m, partials[q]);
+  for (int q = threadIdx.x; q < nblocks; q += 256) m = nmax(m, partials[q]);
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) *out = fmax(fmax(fmax(red[0], red[1]), red[2]), red[3]);
+  if (threadIdx.x == 0) *out = nmax(nmax(nmax(red[0], red[1]), red[2]), red[3]);
 }
 
 // rhs of a correction solve: b = r * inv (rounded to the storage type)
@@ -1400,9 +1386,11 @@ void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
     if (comp == 1) k_p2g<T, 1, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 2) k_p2g<T, 2, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
   } else {
-    // SDR_P2G_PERFACE=1 selects the face-by-face gather (same sums, same order)
-    const char* pf = getenv("SDR_P2G_PERFACE");
-    const bool perface = pf && atoi(pf) != 0;
+    // SDR_P2G_PERFACE=1 (read once per process) selects the face-by-face gather
+    static const bool perface = [] {
+      const char* pf = getenv("SDR_P2G_PERFACE");
+      return pf && atoi(pf) != 0;
+    }();
     if (!perface && launch_p2g_tiled<T>(comp, g, p, cstart, cend, out, st)) return;
     if (comp == 0) k_p2g<T, 0, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 1) k_p2g<T, 1, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
@@ -1445,15 +1433,6 @@ void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgS
     return;
   }
   k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
-}
-template <class T>
-bool launch_spmv_update(Shear sh, const uint16_t* flags, const T* s, const T* z, T* snew, T* out,
-                        const PcgScalars* sc, int it, double* partials, hipStream_t st) {
-  if (!units_fit<T>(sh)) return false;
-  const int nb = cell_blocks(sh);
-  k_spmv_units<T, 2, true><<<nb, 256, 0, st>>>(sh, flags, s, out, sc, it, partials, unit_span_len(sh, nb, 2), z,
-                                               snew);
-  return true;
 }
 int dot_blocks(Shear sh, bool exact) {
   if (exact) {
@@ -1670,8 +1649,6 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
                                     T*, uint16_t*, double*, hipStream_t);                          \
   template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
                                double*, hipStream_t);                                              \
-  template bool launch_spmv_update<T>(Shear, const uint16_t*, const T*, const T*, T*, T*,          \
-                                      const PcgScalars*, int, double*, hipStream_t);               \
   template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
                                        const PcgScalars*, hipStream_t);                            \
   template void launch_maxabs_partials<T>(Shear, const T*, double*, int, const PcgScalars*,        \

@@ -81,11 +81,6 @@ void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
                  int it, double* partials, hipStream_t st);
-// As = A s' with s' = z + beta s formed on the fly and stored into snew (the update of
-// the search direction fused into the next product); false if the sizes do not allow it
-template <class T>
-bool launch_spmv_update(Shear sh, const uint16_t* flags, const T* s, const T* z, T* snew, T* out,
-                        const PcgScalars* sc, int it, double* partials, hipStream_t st);
 int cell_blocks(Shear sh);
 template <class T>
 void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,

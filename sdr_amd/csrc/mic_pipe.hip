@@ -597,268 +597,11 @@ __global__ void __launch_bounds__(64 * TI * TK)
   }
 }
 
-// ------------------------------------------------------------------ multi-plane solve sweeps
-// As k_mic_solve, but a thread owns KT consecutive planes of its x-pencil: at step s its
-// plane t (of the wave's KT) is at row J - t, so the z neighbour of plane t > 0 is the
-// thread's own plane t - 1 from the previous step (a register), and only the wave's
-// first plane (last, backward) takes its z neighbour from another wave (LDS) or tile
-// (halo words).  A workgroup is TK such waves, one per SIMD, and tiles are TK * KT
-// planes deep, so a tile needs few waves for many planes and few hops cross tiles.
-// Tiles can be placed by XCD (`xcd`: the k-tiles of XCD x are a contiguous range, with
-// every i-tile, assuming workgroup b runs on XCD b % xcd), so that neighbouring tiles
-// share an XCD.  Halo words are read at agent scope either way: a narrower scope may be
-// served by a stale vector-L1 line.
-template <class T, int TK, int KT>
-struct Solve2Depth {
-  static constexpr int n = (sizeof(T) == 8 && KT >= 8) ? 4 : 8;
-};
-
-// Logical tile (forward dependency order) of workgroup b.
-__device__ inline void tile_of(int b, int nti, int ntk, int xcd, int* itl, int* ktl) {
-  if (xcd > 0) {
-    const int x = b % xcd, q = b / xcd, per = ntk / xcd;
-    *itl = q % nti;
-    *ktl = x * per + q / nti;
-  } else {
-    *itl = b % nti;
-    *ktl = b / nti;
-  }
-}
-
-template <class T, bool REV, int TK, int KT>
-__global__ void __launch_bounds__(64 * TK)
-    k_mic_solve2(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
-                 const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
-                 MicPipeArgs<T> pa) {
-  if (sc->done) return;
-  constexpr int W = Words<T>::n;
-  constexpr uint32_t SZ = sizeof(T);
-  constexpr int PF = Solve2Depth<T, TK, KT>::n;
-  constexpr int P = TK * KT;  // planes per tile
-  static_assert(PF % 2 == 0, "the LDS parity of a step is its ring slot's parity");
-  static_assert(P <= 64, "one fetch lane per plane of the tile");
-  __shared__ T zbuf[2][TK][64];
-  __shared__ T xlds[2][P];
-
-  const int lane = threadIdx.x & 63;
-  const int wk = threadIdx.x >> 6;
-  const int nti = pa.ntile_i, ntk = pa.ntile_k;
-  int itl, ktl;
-  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
-  const int it = REV ? nti - 1 - itl : itl;
-  const int kt = REV ? ntk - 1 - ktl : ktl;
-  const int i = it * 64 + lane;
-  const int k0 = kt * P + wk * KT;  // plane of this wave's first pencil
-  const int NJ = sh.NJ, nx = sh.nx, ny = sh.ny, nz = sh.nz;
-  const int nxp = nti * 64;
-  const uint32_t ncell = (uint32_t)sh.size();
-  const rsrc_t rf = make_rsrc(flags, ncell * 2u), rp = make_rsrc(precon, ncell * SZ),
-               ri = make_rsrc(in, ncell * SZ), ro = make_rsrc(out, ncell * SZ),
-               rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
-  const uint32_t tag = pa.tag;
-  const int dJ = REV ? -1 : 1;
-  // the plane that takes its z neighbour from outside the wave, and the one that gives
-  const int tzin = REV ? KT - 1 : 0, tzout = REV ? 0 : KT - 1;
-  const bool z_lds = REV ? wk + 1 < TK : wk > 0;
-  const bool z_in = !z_lds && (REV ? kt + 1 < ntk : kt > 0);
-  const bool x_tile_in = REV ? it + 1 < nti : it > 0;
-  // One wave fetches the x halo words of every plane of the tile (lane l: plane l) a
-  // step ahead and hands them over through LDS.
-  const bool x_fetch = x_tile_in && wk == (REV ? 0 : TK - 1);
-  const int kl = kt * P + lane;
-  const bool x_lane = x_fetch && lane < P && kl < nz;
-  const bool z_out = (REV ? wk == 0 : wk == TK - 1) && (REV ? kt > 0 : kt + 1 < ntk);
-  const bool x_out = REV ? it > 0 : it + 1 < nti;
-  const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
-  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
-  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
-  const int zsrc = REV ? wk + 1 : wk - 1;
-  const int zrow = W * nxp;
-  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
-  const int xl_in = (xb_in * nz + kl) * NJ * W;
-  int base[KT], xh_out[KT];
-  bool tok[KT];
-#pragma unroll
-  for (int t = 0; t < KT; ++t) {
-    const int k = k0 + t;
-    tok[t] = i < nx && k < nz;
-    base[t] = k * NJ * nx + i;
-    xh_out[t] = x_out && lane == edge_out && k < nz ? (xb_out * nz + k) * NJ * W : -1;
-  }
-
-  const int nsteps = NJ + P - 1;
-  const int nsteps_pad = (nsteps + PF - 1) / PF * PF;
-  // row of plane 0 of this wave at step 0 (plane t is at row J - t)
-  const int J0 = REV ? (NJ - 1) + (P - 1 - wk * KT) : -(wk * KT);
-  const int JX0 = REV ? (NJ - 1) + (P - 1 - lane) : -lane;
-
-  uint32_t cf[PF][KT], co[PF][KT];
-  T cp[PF][KT], ci[PF][KT];
-  uint64_t hz[PF][W], hx[PF][W];
-  auto refill = [&](int u, int J, int JX) {
-#pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      const int R = J - t;
-      const bool v = tok[t] && (unsigned)(R - i) < (unsigned)ny;
-      const uint32_t c = (uint32_t)(base[t] + R * nx);
-      const uint32_t ot = v ? c * SZ : kOob;
-      co[u][t] = ot;
-      cf[u][t] = __builtin_amdgcn_raw_buffer_load_b16(rf, v ? c * 2u : kOob, 0, 0);
-      cp[u][t] = bld<T>(rp, ot);
-      ci[u][t] = bld<T>(ri, ot);
-    }
-    if (z_in) {
-      const int R = J - tzin;
-      const uint32_t o = (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
-#pragma unroll
-      for (int q = 0; q < W; ++q) hz[u][q] = bld64<kAgent>(rhz, o + (uint32_t)(q * nxp) * 8u);
-    }
-    if (x_fetch) {
-      const uint32_t o = x_lane && (unsigned)JX < (unsigned)NJ ? (uint32_t)(xl_in + JX * W) * 8u : kOob;
-#pragma unroll
-      for (int q = 0; q < W; ++q) hx[u][q] = bld64<kAgent>(rhx, o + (uint32_t)q * 8u);
-    }
-  };
-  auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
-    long spins = 0;
-    while (!tags_ok<W>(w, tag)) {
-      if (++spins > pa.spin_limit ||
-          __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-      for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
-    }
-  };
-  // A stale word: wait for the window's newest word, then reload the window (rows are
-  // published in order), so later steps find fresh words instead of each paying one
-  // more round trip.
-  auto resync = [&](uint64_t(&ring)[PF][W], int u, int Jh, rsrc_t r, int hbase, int rowsz,
-                    uint32_t stride) {
-    int Jf = Jh + (PF - 1) * dJ;
-    Jf = Jf < 0 ? 0 : (Jf >= NJ ? NJ - 1 : Jf);
-    uint64_t w[W];
-#pragma unroll
-    for (int q = 0; q < W; ++q) w[q] = 0;  // tag 0 is never a sweep's tag
-    settle(w, r, (uint32_t)(hbase + Jf * rowsz) * 8u, stride);
-#pragma unroll
-    for (int v = 0; v < PF; ++v) {
-      const int R = Jh + v * dJ;
-      const uint32_t o = (unsigned)R < (unsigned)NJ ? (uint32_t)(hbase + R * rowsz) * 8u : kOob;
-#pragma unroll
-      for (int q = 0; q < W; ++q) ring[(u + v) % PF][q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
-    }
-    settle(ring[u], r, (uint32_t)(hbase + Jh * rowsz) * 8u, stride);
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-  };
-
-  zbuf[1][wk][lane] = T(0);  // read by step 0
-  if (threadIdx.x < P) xlds[1][threadIdx.x] = T(0);
-#pragma unroll
-  for (int u = 0; u < PF; ++u) {
-#pragma unroll
-    for (int q = 0; q < W; ++q) hz[u][q] = hx[u][q] = 0;
-    refill(u, J0 + u * dJ, JX0 + u * dJ);
-  }
-  __syncthreads();
-
-  T prev[KT];
-#pragma unroll
-  for (int t = 0; t < KT; ++t) prev[t] = T(0);
-  int J = J0, JX = JX0;
-  for (int s0 = 0; s0 < nsteps_pad; s0 += PF) {
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const int cb = u & 1, pb = cb ^ 1;
-      // ---- z neighbour of the wave's boundary plane
-      T zb = z_lds ? zbuf[pb][zsrc][lane] : T(0);
-      const int rz = J - tzin;
-      if (z_in && (unsigned)rz < (unsigned)NJ) {
-        if (!tags_ok<W>(hz[u], tag)) resync(hz, u, rz, rhz, zh_in, zrow, (uint32_t)nxp * 8u);
-        zb = words_value<T>(hz[u]);
-      }
-      T pubn[KT];
-#pragma unroll
-      for (int t = 0; t < KT; ++t) {
-        const T xs = wave_shift<REV>(prev[t]);
-        const T xe = x_tile_in ? xlds[pb][wk * KT + t] : T(0);
-        const T xin = lane == edge_in ? xe : xs;
-        const T zin = t == tzin ? zb : prev[REV ? (t + 1) % KT : (t + KT - 1) % KT];
-        const T yin = prev[t];
-        const uint32_t f = cf[u][t];
-        const T pcu = cp[u][t], inu = ci[u][t];
-        const bool live = (f & kFlFluid) != 0;
-        T res, pub;
-        if (!REV) {
-          // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
-          const T q = (((inu - xin) - yin) - zin) * pcu;
-          res = live ? q : T(0);
-          pub = live ? (-pcu) * q : T(0);
-        } else {
-          const T npc = -pcu;
-          T x = inu;
-          x = x - ((f & kFlPx) ? npc * xin : T(0));
-          x = x - ((f & kFlPy) ? npc * yin : T(0));
-          x = x - ((f & kFlPz) ? npc * zin : T(0));
-          const T z = x * pcu;
-          res = live ? z : T(0);
-          pub = res;
-        }
-        bst<T>(ro, co[u][t], res);
-        pubn[t] = pub;
-      }
-      zbuf[cb][wk][lane] = pubn[tzout];
-      // ---- halo publication
-      if (z_out) {
-        const int R = J - tzout;
-        if ((unsigned)R < (unsigned)NJ) {
-          uint32_t wv[W];
-          __builtin_memcpy(wv, &pubn[tzout], sizeof(T));
-          const uint32_t o = (uint32_t)(zh_out + R * zrow) * 8u;
-#pragma unroll
-          for (int q = 0; q < W; ++q)
-            bst64<kWriteThrough>(rhz, o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
-        }
-      }
-      if (x_out) {
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-          const int R = J - t;
-          uint32_t wv[W];
-          __builtin_memcpy(wv, &pubn[t], sizeof(T));
-          const uint32_t o = xh_out[t] >= 0 && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out[t] + R * W) * 8u : kOob;
-#pragma unroll
-          for (int q = 0; q < W; ++q) bst64<kWriteThrough>(rhx, o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
-        }
-      }
-      if (x_fetch) {
-        T xv = T(0);
-        if (x_lane && (unsigned)JX < (unsigned)NJ) {
-          if (!tags_ok<W>(hx[u], tag)) resync(hx, u, JX, rhx, xl_in, W, 8u);
-          xv = words_value<T>(hx[u]);
-        }
-        if (lane < P) xlds[cb][lane] = xv;
-      }
-#pragma unroll
-      for (int t = 0; t < KT; ++t) prev[t] = pubn[t];
-      refill(u, J + PF * dJ, JX + PF * dJ);
-      J += dJ;
-      JX += dJ;
-      __syncthreads();
-    }
-  }
-}
-
-template <class T, int TK, int KT>
-void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
-                   const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
-  const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd)
-    k_mic_solve2<T, false, TK, KT><<<grid, 64 * TK, 0, st>>>(sh, flags, precon, in, out, sc, pa);
-  else
-    k_mic_solve2<T, true, TK, KT><<<grid, 64 * TK, 0, st>>>(sh, flags, precon, in, out, sc, pa);
+// Logical tile (forward dependency order) of workgroup b: i fastest.
+__device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
+  (void)ntk;
+  *itl = b % nti;
+  *ktl = b / nti;
 }
 
 // ------------------------------------------------------------------ lean solve sweeps with an I/O wave
@@ -876,27 +619,21 @@ void launch_solve2(int op, Shear sh, const uint16_t* flags, const T* precon, con
 //     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) are
 //     loaded from a far offset (not fetched: half the sheared slots), read as flags 0,
 //     and compute and store 0; that select is the only bounds test in the step.
-// AXPY (forward solve): the wave first applies p += alpha s and r -= alpha As to its
-// cell (pressure.rs:371-375, r updated in place) and solves with the new r; the lanes'
-// max|r| (pressure.rs:377) leaves one partial per workgroup.  The update rides on a
-// latency-bound sweep instead of costing a pass of its own.
 // DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
-template <class T, bool REV, int TK, bool DOT, int PFH, bool AXPY, int PFC, bool IO2>
-__global__ void __launch_bounds__(64 * (TK + 1 + IO2))
+template <class T, bool REV, int TK, bool DOT, int PFH>
+__global__ void __launch_bounds__(64 * (TK + 1))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
                  MicPipeArgs<T> pa) {
   if (sc->done) return;
   constexpr int W = Words<T>::n;
   constexpr uint32_t SZ = sizeof(T);
-  // the cell ring, PFC steps deep: 3 loads and a store per step (4 and a store with
-  // DOT), 6 loads and 3 stores with AXPY; a step waits for the loads of PFC steps
-  // before, so a deeper ring hides more memory latency, up to the wave's 63
-  // outstanding memory operations
-  constexpr int PF = AXPY ? 4 : PFC;
-  static_assert(!(AXPY && REV) && !(AXPY && DOT), "AXPY is a forward-sweep option");
+  // the cell ring, PF steps deep: 3 loads and a store per step (4 and a store with
+  // DOT); a step waits for the loads of PF steps before (deeper rings, 12 and 16 steps,
+  // measured no faster)
+  constexpr int PF = 8;
   // the step loop is unrolled by U: ring slots are compile-time (u % PF for the cells,
   // (u + 1) % PFH for the halo words) and the LDS parity of step s0 + u is u & 1
   constexpr int U = PF > PFH ? PF : PFH;
@@ -913,7 +650,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
   const bool io = wave >= TK;
   const int nti = pa.ntile_i, ntk = pa.ntile_k;
   int itl, ktl;
-  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
+  tile_of(blockIdx.x, nti, ntk, &itl, &ktl);
   const int it = REV ? nti - 1 - itl : itl;
   const int kt = REV ? ntk - 1 - ktl : ktl;
   const int i = it * 64 + lane;
@@ -940,11 +677,7 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
                  rp = make_rsrc(pbase, recs * SZ),
                  ri = make_rsrc(in + (int64_t)plane * NJ * nx, recs * SZ),
                  ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ),
-                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u),
-                 rap = make_rsrc(AXPY ? pa.ax_p + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u),
-                 ras = make_rsrc(AXPY ? pa.ax_s + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u),
-                 raq = make_rsrc(AXPY ? pa.ax_as + (int64_t)plane * NJ * nx : nullptr, AXPY ? recs * SZ : 0u);
-    const T alpha = AXPY ? (T)sc->alpha : T(0);
+                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u);
     const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
     // element offset of (row J, lane i); lanes past the grid start out of range for good
     // (2^28 * size stays below 2^32 and above every plane's record count)
@@ -957,11 +690,10 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
     uint32_t ofn = (lbase + (uint32_t)((J0 + PF * dJ) * nx)) * SZ;
     const uint32_t d2 = (uint32_t)(dJ * nx) * 2u, dS = (uint32_t)(dJ * nx) * SZ;
     // j of the row PF steps ahead: slots outside the grid (j < 0 or j >= ny, half the
-    // sheared slots) are not loaded (with AXPY not stored either, so they keep their p
-    // and r and add nothing to max|r|)
+    // sheared slots) are not loaded
     int jn = J0 + PF * dJ - i;
     uint32_t cf[PF];
-    T cp[PF], ci[PF], cr[PF], cap[PF], cas[PF], caq[PF];
+    T cp[PF], ci[PF], cr[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const bool in = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny;
@@ -970,12 +702,11 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
       if (DOT) cr[u] = bld<T>(rr, e * SZ);
-      if (AXPY) cap[u] = bld<T>(rap, e * SZ), cas[u] = bld<T>(ras, e * SZ), caq[u] = bld<T>(raq, e * SZ);
       // issue slot by slot, in the order of the step loop: the wait for a slot's loads
       // at the top of the loop then also holds for the ones issued here
       __builtin_amdgcn_sched_barrier(0);
     }
-    double dacc = 0.0;  // dot(z, r) (DOT) or max|r| (AXPY)
+    double dacc = 0.0;  // dot(z, r) (DOT)
     zbuf[1][wk][lane] = T(0);
     T prev = T(0);
     __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
@@ -987,15 +718,8 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
         const T zin = zbuf[pb][zsrc][lane];
         const T xin = wave_shift_in<REV>(prev, xlds[pb][wk]);  // the edge lane takes the x halo
         const uint32_t f = cf[u];
-        T inu = ci[u];
+        const T inu = ci[u];
         const T pcu = cp[u];
-        T np = T(0);
-        if (AXPY) {
-          // p += alpha s; r -= alpha As (pressure.rs:371-375), then max|r| (:377)
-          np = cap[u] + alpha * cas[u];
-          inu = inu - alpha * caq[u];
-          dacc = fmax(dacc, fabs((double)inu));
-        }
         const bool live = (f & kFlFluid) != 0;
         T res, pub;
         if (!REV) {
@@ -1018,11 +742,6 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
         zbuf[cb][wk][lane] = pub;
         __builtin_amdgcn_sched_barrier(0);
         prev = pub;
-        if (AXPY) {
-          const uint32_t ec = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny ? oc : kFarElem * SZ;
-          bst<T>(rap, ec, np);
-          bst<T>(ri, ec, inu);
-        }
         bst<T>(ro, oc, res);
         if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
         const bool inn = (unsigned)jn < (unsigned)sh.ny;
@@ -1031,7 +750,6 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
         cp[u] = bld<T>(rp, enS);
         ci[u] = bld<T>(ri, enS);
         if (DOT) cr[u] = bld<T>(rr, enS);
-        if (AXPY) cap[u] = bld<T>(rap, enS), cas[u] = bld<T>(ras, enS), caq[u] = bld<T>(raq, enS);
         oc += dS;
         of2 += d2;
         ofn += dS;
@@ -1046,25 +764,20 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o);
       if (lane == 0) dsum[wk] = dacc;
     }
-    if (AXPY) {
-      for (int o = 32; o > 0; o >>= 1) dacc = fmax(dacc, __shfl_xor(dacc, o));
-      if (lane == 0) dsum[wk] = dacc;
-    }
     __syncthreads();  // matched by the I/O wave
-    if ((DOT || AXPY) && wk == 0 && lane == 0) {
+    if (DOT && wk == 0 && lane == 0) {
       double t = 0.0;
-      for (int w = 0; w < TK; ++w) t = DOT ? t + dsum[w] : fmax(t, dsum[w]);
+      for (int w = 0; w < TK; ++w) t = t + dsum[w];
       pa.dot_partials[blockIdx.x] = t;
     }
     return;
   }
 
   // -------------------------------------------------------------- I/O wave(s)
-  // ZX = 2: one wave serves both halos; with IO2, wave TK serves the z halo (ZX = 0)
-  // and wave TK + 1 the x halo (ZX = 1), each with half the instructions per step.
-  auto io_body = [&](auto zx) {
-  constexpr int ZX = decltype(zx)::value;
-  constexpr bool DZ = ZX != 1, DX = ZX != 0;
+  // one wave serves both halos (DZ: the z halo, DX: the x halo; splitting them over two
+  // waves measured slower)
+  {
+  constexpr bool DZ = true, DX = true;
   // the I/O wave's step is the longest of the workgroup's: let its instructions issue
   // first on the SIMD it shares with a compute wave
   __builtin_amdgcn_s_setprio(3);
@@ -1233,347 +946,21 @@ __global__ void __launch_bounds__(64 * (TK + 1 + IO2))
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  };
-  if (!IO2)
-    io_body(std::integral_constant<int, 2>{});
-  else if (wave == TK)
-    io_body(std::integral_constant<int, 0>{});
-  else
-    io_body(std::integral_constant<int, 1>{});
-  __syncthreads();  // matches the compute waves' final barrier
-}
-
-// ------------------------------------------------------------------ multi-plane sweeps with an I/O wave
-// k_mic_sweep3 with KT planes per compute wave: plane t of a wave is at row J - t, so
-// the z neighbour of its plane t > 0 (t < KT - 1 backward) is the wave's own plane
-// t - 1 (t + 1) from the previous step, a register; only the wave's boundary plane
-// takes it from LDS (the wave before) or the halo (the tile before).  A tile is TK * KT
-// planes deep for the same TK + 1 waves, so the tiles, and the hops between them, are
-// KT times fewer; the KT cells of a step are independent of each other.  The cell ring
-// is PF steps deep (3 loads and a store per plane and step all count against the
-// wave's 63 outstanding memory operations, so PF shrinks as KT grows).
-template <class T, bool REV, int TK, int KT, int PF, bool DOT, int PFH>
-__global__ void __launch_bounds__(64 * (TK + 1))
-    k_mic_sweep4(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
-                 const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
-                 MicPipeArgs<T> pa) {
-  if (sc->done) return;
-  constexpr int W = Words<T>::n;
-  constexpr uint32_t SZ = sizeof(T);
-  constexpr int P = TK * KT;  // planes per tile
-  static_assert(P <= 64, "one I/O lane per plane of the tile");
-  // the step loop is unrolled by U: ring slots are compile-time (u % PF, (u+1) % PFH) and
-  // the LDS parity of step s0 + u is u & 1
-  constexpr int U = PF > PFH ? PF : PFH;
-  static_assert(U % PF == 0 && U % PFH == 0 && U % 2 == 0, "ring depths must divide the unroll");
-  // zbuf[.][w] for w < TK: the boundary plane of compute wave w (its last plane forward,
-  // first backward) at a step; zbuf[.][TK]: the fetched z halo row of the tile's first
-  // (last) plane.  edgev: every plane's edge lane (published as x halo by the I/O wave).
-  __shared__ T zbuf[2][TK + 1][64];
-  __shared__ __align__(16) T xlds[2][P];
-  __shared__ __align__(16) T edgev[2][P];
-  __shared__ double dsum[TK];
-
-  const int lane = threadIdx.x & 63;
-  // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool io = wave == TK;
-  const int nti = pa.ntile_i, ntk = pa.ntile_k;
-  int itl, ktl;
-  tile_of(blockIdx.x, nti, ntk, pa.xcd, &itl, &ktl);
-  const int it = REV ? nti - 1 - itl : itl;
-  const int kt = REV ? ntk - 1 - ktl : ktl;
-  const int i = it * 64 + lane;
-  const int NJ = sh.NJ, nx = sh.nx, nz = sh.nz;
-  const int nxp = nti * 64;
-  const int dJ = REV ? -1 : 1;
-  const uint32_t tag = pa.tag;
-  // row of tile plane p at step s
-  auto row = [&](int p, int s) { return REV ? (NJ - 1) - s + (P - 1 - p) : s - p; };
-  const int nsteps = NJ + P - 1;  // compute steps; the I/O wave runs one more to publish
-  const int nsteps_pad = (nsteps + 1 + U - 1) / U * U;
-  const int bwave = REV ? TK - 1 : 0;       // boundary wave: its z neighbour is in the tile before
-  const int lwave = REV ? 0 : TK - 1;       // last wave: the tile after takes its z neighbour from it
-  const int bplane = REV ? P - 1 : 0, lplane = REV ? 0 : P - 1;
-  const int tzin = REV ? KT - 1 : 0, tzout = REV ? 0 : KT - 1;  // a wave's boundary planes
-  const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
-
-  if (!io) {
-    // ------------------------------------------------------------ compute wave
-    const int wk = wave;
-    // One descriptor per array for the wave's KT consecutive planes (record count: the
-    // planes inside the grid).  Plane t's cell (row J, lane i) is element
-    // (t * NJ + J) * nx + i; a row outside [0, NJ) gets the out-of-range offset kFar
-    // instead, as do lanes with i >= nx, so no bounds test is left in the step.  Both
-    // together give 2^28 elements, whose byte offset (times 8 at most) stays below 2^32,
-    // and kFar exceeds every record count (KT * NJ * nx elements, checked on the host).
-    const int k0 = kt * P + wk * KT;
-    const int np_ = nz - k0 < 0 ? 0 : (nz - k0 > KT ? KT : nz - k0);
-    const int64_t pl = (int64_t)(np_ > 0 ? k0 : 0) * NJ * nx;
-    const uint32_t recs = (uint32_t)np_ * (uint32_t)(NJ * nx);
-    const rsrc_t rf = make_rsrc(flags + pl, recs * 2u), rp = make_rsrc(precon + pl, recs * SZ),
-                 ri = make_rsrc(in + pl, recs * SZ), ro = make_rsrc(out + pl, recs * SZ),
-                 rr = make_rsrc(DOT ? pa.dot_r + pl : nullptr, DOT ? recs * SZ : 0u);
-    constexpr uint32_t kFar = 1u << 27;
-    const uint32_t li = i < nx ? (uint32_t)i : kFar;
-    const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
-    const int J0 = row(wk * KT, 0);
-    // scalar part of plane t's element offset at plane-0 row J
-    auto soff = [&](int t, int J) -> uint32_t {
-      const int R = J - t;
-      return (unsigned)R < (unsigned)NJ ? (uint32_t)((t * NJ + R) * nx) : kFar;
-    };
-    uint32_t cf[PF][KT];
-    T cp[PF][KT], ci[PF][KT], cr[PF][KT];
-    auto load = [&](int u, int J) {
-#pragma unroll
-      for (int t = 0; t < KT; ++t) {
-        const uint32_t e = li + soff(t, J);
-        cf[u][t] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
-        cp[u][t] = bld<T>(rp, e * SZ);
-        ci[u][t] = bld<T>(ri, e * SZ);
-        if (DOT) cr[u][t] = bld<T>(rr, e * SZ);
-      }
-    };
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      load(u, J0 + u * dJ);
-      __builtin_amdgcn_sched_barrier(0);  // slot by slot, in the order of the step loop
-    }
-    int J = J0;
-    double dacc = 0.0;
-    zbuf[1][wk][lane] = T(0);
-    T prev[KT];
-#pragma unroll
-    for (int t = 0; t < KT; ++t) prev[t] = T(0);
-    __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
-    for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
-#pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        const int u = uu % PF;
-        const int cb = uu & 1, pb = cb ^ 1;
-        const T zb = zbuf[pb][zsrc][lane];
-        T pubn[KT];
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-          const T xe = xlds[pb][wk * KT + t];
-          const T xs = wave_shift<REV>(prev[t]);
-          const T xin = lane == edge_in ? xe : xs;
-          const T zin = t == tzin ? zb : prev[REV ? (t + 1) % KT : (t + KT - 1) % KT];
-          const T yin = prev[t];
-          const uint32_t f = cf[u][t];
-          const T pcu = cp[u][t], inu = ci[u][t];
-          const bool live = (f & kFlFluid) != 0;
-          T res, pub;
-          if (!REV) {
-            // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
-            const T q = (((inu - xin) - yin) - zin) * pcu;
-            res = live ? q : T(0);
-            pub = live ? (-pcu) * q : T(0);
-          } else {
-            const T npc = -pcu;
-            T x = inu;
-            x = x - ((f & kFlPx) ? npc * xin : T(0));
-            x = x - ((f & kFlPy) ? npc * yin : T(0));
-            x = x - ((f & kFlPz) ? npc * zin : T(0));
-            const T z = x * pcu;
-            res = live ? z : T(0);
-            pub = res;
-          }
-          bst<T>(ro, (li + soff(t, J)) * SZ, res);
-          if (DOT) dacc += (double)res * (double)cr[u][t];  // 0 outside the grid and off-fluid
-          pubn[t] = pub;
-        }
-        zbuf[cb][wk][lane] = pubn[tzout];
-        if (lane == edge_out) {
-#pragma unroll
-          for (int t = 0; t < KT; ++t) edgev[cb][wk * KT + t] = pubn[t];
-        }
-#pragma unroll
-        for (int t = 0; t < KT; ++t) prev[t] = pubn[t];
-        load(u, J + PF * dJ);
-        J += dJ;
-        __syncthreads();
-        __builtin_amdgcn_sched_barrier(0);  // each step's instructions stay in the step
-      }
-    }
-    if (DOT) {
-      for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o);
-      if (lane == 0) dsum[wk] = dacc;
-    }
-    __syncthreads();  // matched by the I/O wave
-    if (DOT && wk == 0 && lane == 0) {
-      double t = 0.0;
-      for (int w = 0; w < TK; ++w) t += dsum[w];
-      pa.dot_partials[blockIdx.x] = t;
-    }
-    return;
-  }
-
-  // -------------------------------------------------------------- I/O wave
-  const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
-  const bool z_in = REV ? kt + 1 < ntk : kt > 0;
-  const bool z_out = REV ? kt > 0 : kt + 1 < ntk;
-  const bool x_in = REV ? it + 1 < nti : it > 0;
-  const bool x_out = REV ? it > 0 : it + 1 < nti;
-  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
-  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
-  const int zrow = W * nxp;
-  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
-  // lane l < P serves plane kt * P + l for the x halo
-  const int kl = kt * P + lane;
-  const bool xl = lane < P && kl < nz;
-  const int xh_in = (xb_in * nz + (xl ? kl : 0)) * NJ * W;
-  const int xh_out = (xb_out * nz + (xl ? kl : 0)) * NJ * W;
-  const int klast = kt * P + lplane;
-  const bool zl_ok = klast < nz;
-  // Inputs of target step t: the z halo row of the boundary wave at step t, and the x
-  // neighbour row of plane `lane` at step t (its row at step t, minus dJ).
-  auto zrow_of = [&](int t) { return row(bplane, t); };
-  auto xrow_of = [&](int t) { return row(lane, t) - dJ; };
-  auto zoff = [&](int R) -> uint32_t {
-    return z_in && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_in + R * zrow) * 8u : kOob;
-  };
-  auto xoff = [&](int R) -> uint32_t {
-    return x_in && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_in + R * W) * 8u : kOob;
-  };
-  uint64_t hz[PFH][W], hx[PFH][W];
-  auto fill = [&](int u, int t) {
-    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-      hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
-      hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
-    }
-  };
-  // A word is ready when it carries this sweep's tag, or when it is not needed (its
-  // offset is out of range: the load returned 0, and the value used is 0).
-  auto settle = [&](uint64_t(&w)[W], rsrc_t r, uint32_t o, uint32_t stride) {
-    if (o == kOob) return;
-    long spins = 0;
-    while (!tags_ok<W>(w, tag)) {
-      if (++spins > pa.spin_limit ||
-          __hip_atomic_load(pa.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        __hip_atomic_store(pa.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-      for (int q = 0; q < W; ++q) w[q] = bld64<kAgent>(r, o + (uint32_t)q * stride);
-    }
-  };
-  auto ready = [&](const uint64_t(&w)[W], uint32_t o) -> bool { return (o == kOob) | tags_ok<W>(w, tag); };
-  // Take the inputs of target step t from ring slot u into LDS buffer b.  A stale word
-  // means the producer is behind the prefetch window: wait for the window's newest word
-  // (rows are published in order), then reload the whole window.
-  auto take = [&](int u, int t, int b) {
-    const uint32_t oz = zoff(zrow_of(t)), ox = xoff(xrow_of(t));
-    const bool ok = ready(hz[u], oz) & ready(hx[u], ox);
-    if (__ballot(!ok) != 0ull) {
-      const uint32_t nz_ = zoff(zrow_of(t + PFH - 1)), nx_ = xoff(xrow_of(t + PFH - 1));
-      uint64_t wz[W], wx[W];
-#pragma unroll
-      for (int q = 0; q < W; ++q) wz[q] = wx[q] = 0;  // tag 0 is never a sweep's tag
-      settle(wz, rhz, nz_, (uint32_t)nxp * 8u);
-      settle(wx, rhx, nx_, 8u);
-#pragma unroll
-      for (int v = 0; v < PFH; ++v) fill((u + v) % PFH, t + v);
-      settle(hz[u], rhz, oz, (uint32_t)nxp * 8u);
-      settle(hx[u], rhx, ox, 8u);
-      __builtin_amdgcn_s_waitcnt(0x0F70);
-    }
-    zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
-    if (lane < P) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
-  };
-  // Publish the values compute waves wrote at step t into LDS buffer b (the stores are
-  // issued on every step, out of range when there is nothing to publish, as in
-  // k_mic_sweep3).
-  auto publish = [&](int t, int b) {
-    {
-      const int R = row(lplane, t);
-      const T v = zbuf[b][lwave][lane];
-      uint32_t wv[W];
-      __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = z_out && zl_ok && (unsigned)R < (unsigned)NJ ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
-#pragma unroll
-      for (int q = 0; q < W; ++q)
-        bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
-    }
-    {
-      const int R = row(lane < P ? lane : 0, t);
-      const T v = edgev[b][lane < P ? lane : 0];
-      uint32_t wv[W];
-      __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = x_out && xl && (unsigned)R < (unsigned)NJ ? (uint32_t)(xh_out + R * W) * 8u : kOob;
-#pragma unroll
-      for (int q = 0; q < W; ++q)
-        bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
-    }
-  };
-  zbuf[0][TK][lane] = T(0);
-  if (lane < P) xlds[0][lane] = T(0);
-  // steps -PFH-1 .. -1 in the order of the step loop (see k_mic_sweep3)
-  fill(0, 0);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int u = 1; u < PFH; ++u) {
-    publish(-1, 0);
-    fill(u, u);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  publish(-1, 0);
-  take(0, 0, 1);  // inputs of step 0
-  fill(0, PFH);
-  __syncthreads();
-  __builtin_amdgcn_sched_barrier(0);
-  for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      const int cb = u & 1, pb = cb ^ 1;
-      // values of step s - 1 (in buffer pb), inputs of step s + 1 (into buffer cb)
-      publish(s - 1, pb);
-      const int un = (u + 1) % PFH;
-      take(un, s + 1, cb);
-      fill(un, s + 1 + PFH);
-      __syncthreads();
-      __builtin_amdgcn_sched_barrier(0);
-    }
   }
   __syncthreads();  // matches the compute waves' final barrier
 }
 
-template <class T, int TK, int KT, int PF, int PFH>
-void launch_sweep4(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
-                   const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
-  const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd)
-    k_mic_sweep4<T, false, TK, KT, PF, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
-                                                                                 sc, pa);
-  else if (pa.dot_r)
-    k_mic_sweep4<T, true, TK, KT, PF, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
-                                                                               sc, pa);
-  else
-    k_mic_sweep4<T, true, TK, KT, PF, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out,
-                                                                                sc, pa);
-}
 
-template <class T, int TK, int PFH, int PFC = 8, bool IO2 = false>
+template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd && pa.ax_p)
-    k_mic_sweep3<T, false, TK, false, PFH, true, 4, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
-                                                                              pa);
-  else if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
-                                                                               pa);
+  if (op == kMicFwd)
+    k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
-                                                                             pa);
+    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
-    k_mic_sweep3<T, true, TK, false, PFH, false, PFC, IO2><<<grid, 64 * (TK + 1 + IO2), 0, st>>>(sh, flags, precon, in, out, sc,
-                                                                              pa);
+    k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }
 
 template <class T, int OP, int TI, int TK>
@@ -1598,52 +985,33 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 
 }  // namespace
 
-// Tile shapes.  Variants 0-5: k_mic_solve tiles of 64*TI values of i by TK planes.
-// Variants 6-16: k_mic_solve2 tiles of 64 values of i by TK waves of KT planes, with the
-// XCD placement (x) where the tile grid allows it; 17-23: k_mic_sweep3 tiles of 64
-// values of i by TK planes (plus the I/O wave), halo prefetch depth 8 (kind 2), 4
-// (kind 4) or 2 (kind 6), cell ring depth 8; 24-29: k_mic_sweep4 tiles of TK waves of
-// KT planes (plus the I/O wave), halo depth 4, cell ring depth 2 (kind 8; 4 for KT 2)
-// or 4 (kind 9); 30-34: k_mic_sweep3 with deeper cell rings (pf); 35-38: k_mic_sweep3
-// with two I/O waves (z halo and x halo); 39+: more k_mic_sweep3 tile depths.
+// Tile shapes.  Variants 0 and 1: k_mic_pipe (factor) / k_mic_solve tiles of 64 values
+// of i by 16 and 8 planes, one plane per wave (the factor's shapes: 16 planes for f32,
+// 8 for f64, whose 8-deep f64 ring needs the registers).  Variant 2 (the default solve):
+// k_mic_sweep3 tiles of 64 values of i by 7 planes plus the I/O wave, halo prefetch
+// depth 4.  Other tile shapes (multi-plane waves, 3-15 planes per tile, deeper rings,
+// two I/O waves, XCD-placed tiles) were measured slower on 256^3 and 512x512x384 and
+// are not built.
 namespace {
 struct VariantRow {
-  int ti, tk, kt, kind, xcd, pf, io2;  // pf: k_mic_sweep3's cell ring depth (0: 8); io2: two I/O waves
+  int ti, tk, kind;
 };
-const VariantRow kVariants[] = {
-    {1, 16, 1, 0, 0}, {1, 8, 1, 0, 0}, {2, 8, 1, 0, 0}, {4, 4, 1, 0, 0}, {1, 4, 1, 0, 0},
-    {2, 4, 1, 0, 0},  {1, 4, 4, 1, 0}, {1, 4, 2, 1, 0}, {1, 4, 8, 1, 0}, {1, 8, 2, 1, 0},
-    {1, 4, 4, 1, 8},  {1, 4, 2, 1, 8}, {1, 4, 8, 1, 8}, {1, 2, 8, 1, 8}, {1, 8, 4, 1, 8},
-    {1, 2, 4, 1, 8},  {1, 8, 2, 1, 8},  {1, 15, 1, 2, 0}, {1, 7, 1, 2, 0}, {1, 3, 1, 2, 0},
-    {1, 7, 1, 4, 0},  {1, 7, 1, 6, 0},  {1, 15, 1, 4, 0}, {1, 3, 1, 4, 0},
-    {1, 7, 4, 8, 0},  {1, 7, 2, 8, 0},  {1, 3, 4, 8, 0},  {1, 7, 4, 9, 0}, {1, 3, 8, 8, 0},
-    {1, 15, 2, 8, 0},
-    {1, 7, 1, 4, 0, 12}, {1, 7, 1, 4, 0, 16}, {1, 3, 1, 4, 0, 16}, {1, 15, 1, 4, 0, 12}, {1, 7, 1, 6, 0, 12},
-    {1, 7, 1, 4, 0, 0, 1}, {1, 11, 1, 4, 0, 0, 1}, {1, 14, 1, 4, 0, 0, 1}, {1, 7, 1, 2, 0, 0, 1},
-    {1, 9, 1, 4, 0}, {1, 11, 1, 4, 0}, {1, 5, 1, 4, 0}, {1, 11, 1, 6, 0},
-};
+const VariantRow kVariants[] = {{1, 16, 0}, {1, 8, 0}, {1, 7, 4}};
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 }  // namespace
 
 int mic_pipe_variants() { return kNumVariants; }
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
-  int v = variant < 0 || variant >= kNumVariants ? 0 : variant;
-  // k_mic_sweep4 addresses a wave's planes below 2^27 elements (kFar)
-  if (kVariants[v].kind >= 8 && (int64_t)kVariants[v].kt * sh.NJ * sh.nx >= (1 << 27)) v = 20;
+  (void)ncu;
+  const int v = variant < 0 || variant >= kNumVariants ? 0 : variant;
   const VariantRow& r = kVariants[v];
   MicPipeShape s;
   s.ti = r.ti;
   s.tk = r.tk;
-  s.kt = r.kt;
   s.kind = r.kind;
-  s.pf = r.pf > 0 ? r.pf : 8;
-  s.io2 = r.io2;
-  const int planes = r.tk * r.kt;
   s.ntile_i = (sh.nx + 64 * s.ti - 1) / (64 * s.ti);
-  s.ntile_k = (sh.nz + planes - 1) / planes;
-  // XCD placement needs whole k-tile ranges per XCD and every tile resident at once
-  s.xcd = (r.xcd > 0 && s.ntile_k % r.xcd == 0 && s.ntile_i * s.ntile_k <= ncu) ? r.xcd : 0;
+  s.ntile_k = (sh.nz + s.tk - 1) / s.tk;
   return s;
 }
 
@@ -1653,70 +1021,14 @@ void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* pr
   MicPipeShape s = mic_pipe_shape(sh, variant, pa.ncu);
   pa.ntile_i = s.ntile_i;
   pa.ntile_k = s.ntile_k;
-  pa.xcd = s.xcd;
-  if (s.kind == 2 || s.kind == 4 || s.kind == 6) {
-    const int pfh = s.kind == 2 ? 8 : (s.kind == 4 ? 4 : 2);
-    if (s.io2) switch (s.tk * 10 + pfh) {
-      case 74: launch_sweep3<T, 7, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 114: launch_sweep3<T, 11, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 144: launch_sweep3<T, 14, 4, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 78: launch_sweep3<T, 7, 8, 8, true>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      default: break;
-    }
-    if (!s.io2) switch (s.tk * 1000 + s.pf * 10 + pfh) {
-      case 7124: launch_sweep3<T, 7, 4, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 7164: launch_sweep3<T, 7, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 3164: launch_sweep3<T, 3, 4, 16>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 15124: launch_sweep3<T, 15, 4, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 7122: launch_sweep3<T, 7, 2, 12>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      default: break;
-    }
-    if (s.pf == 8) switch (s.tk * 10 + pfh) {
-      case 158: launch_sweep3<T, 15, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 78: launch_sweep3<T, 7, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 38: launch_sweep3<T, 3, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 154: launch_sweep3<T, 15, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 74: launch_sweep3<T, 7, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 34: launch_sweep3<T, 3, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 72: launch_sweep3<T, 7, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 94: launch_sweep3<T, 9, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 114: launch_sweep3<T, 11, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 54: launch_sweep3<T, 5, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 112: launch_sweep3<T, 11, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      default: break;
-    }
+  if (s.kind == 4 && op != kMicBuild) {
+    launch_sweep3<T, 7, 4>(op, sh, flags, precon, in, out, sc, pa, st);
+    return;
   }
-  if (s.kind == 8 || s.kind == 9) {  // k_mic_sweep4, halo depth 4; cell ring 2 (8) or 4 (9)
-    switch (s.tk * 100 + s.kt * 10 + (s.kind - 8)) {
-      case 740: launch_sweep4<T, 7, 4, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 741: launch_sweep4<T, 7, 4, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 720: launch_sweep4<T, 7, 2, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 340: launch_sweep4<T, 3, 4, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 380: launch_sweep4<T, 3, 8, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 1520: launch_sweep4<T, 15, 2, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      default: break;
-    }
-  }
-  if (s.kind == 1) {
-    switch (s.tk * 100 + s.kt) {
-      case 404: launch_solve2<T, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 402: launch_solve2<T, 4, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 408: launch_solve2<T, 4, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 802: launch_solve2<T, 8, 2>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 208: launch_solve2<T, 2, 8>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 804: launch_solve2<T, 8, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      case 204: launch_solve2<T, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); return;
-      default: break;
-    }
-  }
-  switch (variant) {
-    case 1: dispatch_op<T, 1, 8>(op, sh, flags, precon, in, out, sc, pa, st); break;
-    case 2: dispatch_op<T, 2, 8>(op, sh, flags, precon, in, out, sc, pa, st); break;
-    case 3: dispatch_op<T, 4, 4>(op, sh, flags, precon, in, out, sc, pa, st); break;
-    case 4: dispatch_op<T, 1, 4>(op, sh, flags, precon, in, out, sc, pa, st); break;
-    case 5: dispatch_op<T, 2, 4>(op, sh, flags, precon, in, out, sc, pa, st); break;
-    default: dispatch_op<T, 1, 16>(op, sh, flags, precon, in, out, sc, pa, st); break;
-  }
+  if (s.tk == 8)
+    dispatch_op<T, 1, 8>(op, sh, flags, precon, in, out, sc, pa, st);
+  else
+    dispatch_op<T, 1, 16>(op, sh, flags, precon, in, out, sc, pa, st);
 }
 
 template void launch_mic_pipe<double>(int, int, Shear, const uint16_t*, double*, const double*,

@@ -23,32 +23,24 @@ struct MicPipeArgs {
   int ntile_i, ntile_k;
   uint32_t tag;
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
-  int ncu;       // compute units of the device (bounds the tiles placed by XCD)
-  int xcd;       // XCD placement of the tiles (0: none); set by launch_mic_pipe
-  // Backward solves of kinds 2, 4, 6, 8, 9 only: with dot_r set, the sweep also leaves one partial of
-  // dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in dot_partials.
+  int ncu;       // compute units of the device
+  // Backward solves of the I/O-wave kernel (kind 4) only: with dot_r set, the sweep also
+  // leaves one partial of dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in
+  // dot_partials.
   const T* dot_r;
   double* dot_partials;
-  // Forward solves of kinds 2, 4, 6 only: with ax_p set, `in` is r and the sweep first applies
-  // p += alpha ax_s, r -= alpha ax_as (alpha from the PCG scalars), leaving one max|r|
-  // partial per workgroup in dot_partials.
-  T* ax_p;
-  const T* ax_s;
-  const T* ax_as;
 };
 
-// kind 0: k_mic_solve tiles of 64*ti values of i by tk planes; kind 1: k_mic_solve2
-// tiles of 64 values of i by tk waves of kt planes; kinds 2, 4, 6: k_mic_sweep3 tiles
-// of 64 values of i by tk planes with an I/O wave (halo prefetch depth 8, 4, 2); kinds
-// 8, 9: k_mic_sweep4 tiles of tk waves of kt planes with an I/O wave.  The factor (build) always runs the kind-0
-// kernel.
+// kind 0: k_mic_pipe (factor) / k_mic_solve tiles of 64*ti values of i by tk planes;
+// kind 4: k_mic_sweep3 tiles of 64 values of i by tk planes with an I/O wave (halo
+// prefetch depth 4).  The factor (build) always runs a kind-0 kernel.
 struct MicPipeShape {
-  int ti, tk, kt, kind, ntile_i, ntile_k, xcd;
-  int pf;   // cell ring depth of the k_mic_sweep3 kinds
-  int io2;  // k_mic_sweep3 with two I/O waves
+  int ti, tk, kind, ntile_i, ntile_k;
 };
 
 MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu);
+// the default solve shape (k_mic_sweep3, 7 planes per tile)
+constexpr int kMicDefaultVariant = 2;
 inline int mic_pipe_dot_partials(const MicPipeShape& s) { return s.ntile_i * s.ntile_k; }
 int mic_pipe_variants();
 

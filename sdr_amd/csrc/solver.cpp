@@ -25,6 +25,7 @@
 
 #include "../../include/sdr_amd.h"
 #include "kernels.hpp"
+#include "metrics.hpp"
 #include "mic_pipe.hpp"
 #include "sort.hpp"
 
@@ -201,15 +202,14 @@ struct Projector {
   bool exact_dot = true;
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
-  Buf b, z, s, s2, p, as, q, precon, flags, partials, scal, units;
+  Buf b, z, s, p, as, q, precon, flags, partials, scal, units;
   Buf halo_x, halo_z, pipe_err, sink;
   Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
   PinnedBuf host_scal;
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
-  bool axpy_fuse = false;  // the forward sweep applies the PCG axpys (sweep kinds 2, 4, 6)
-  bool spmv_upd = false;   // s = z + beta s formed inside the next product (measured slower)
   int ncu = 0, dev = 0;  // compute units of the solver's device
+  bool trace_refine = false;  // SDR_TRACE_REFINE=1: one stderr line per refinement pass
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
   void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
@@ -234,12 +234,12 @@ struct Projector {
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
     scal.alloc(sizeof(PcgScalars));
     host_scal.ensure(sizeof(PcgScalars) + 16);
-    // both precisions: 7-plane tiles with an I/O wave, halo prefetch depth 4 (k_mic_sweep3)
-    mic_variant = 20;
+    // both precisions: 7-plane tiles with an I/O wave, halo prefetch depth 4 (k_mic_sweep3);
+    // SDR_MIC_VARIANT (read once per solver) selects another tested shape
+    mic_variant = kMicDefaultVariant;
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
-    if (const char* e = getenv("SDR_MIC_AXPY")) axpy_fuse = atoi(e) != 0;
-    if (const char* e = getenv("SDR_SPMV_UPDATE")) spmv_upd = atoi(e) != 0;
-    if (spmv_upd) s2.alloc(n * sizeof(T));  // the second search-direction buffer
+    if (mic_variant >= mic_pipe_variants()) mic_variant = kMicDefaultVariant;
+    if (const char* e = getenv("SDR_TRACE_REFINE")) trace_refine = atoi(e) != 0;
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -268,18 +268,10 @@ struct Projector {
 
   PcgScalars* sc() const { return scal.as<PcgScalars>(); }
 
-  // The fast reduction mode fuses dot(z, r) into the backward solve (kind-2 sweeps only).
-  bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind >= 2; }
+  // The fast reduction mode fuses dot(z, r) into the backward solve (I/O-wave sweeps only).
+  bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 4; }
 
-  // the forward solve also applies the PCG axpys (kinds 2, 4, 6)
-  bool fused_axpy() const {
-    if (!axpy_fuse || !fused()) return false;
-    const int k = mic_pipe_shape(sh, mic_variant, ncu).kind;
-    return k == 2 || k == 4 || k == 6;
-  }
-
-  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr, T* ax_p = nullptr,
-                 const T* ax_s = nullptr, const T* ax_as = nullptr) {
+  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr) {
     if (mic_variant < 0) {
       const int planes = sh.NJ + sh.nz - 1;
       for (int sgm = 0; sgm < planes; ++sgm)
@@ -297,7 +289,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, 0, dot_r, partials.as<double>(), ax_p, ax_s, ax_as};
+                      (uint32_t)halo_z.bytes, ncu, dot_r, partials.as<double>()};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
@@ -358,39 +350,18 @@ struct Projector {
     const uint64_t check_every = 8;
     if (fused()) {
       const int ndot = mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu));
-      // the search direction lives in s or s2: with the update fused into the product,
-      // each iteration reads one and writes the other
-      T* scur = s.as<T>();
-      T* snext = s2.as<T>();
-      bool upd = spmv_upd;
       for (uint64_t it = 0; it < max_iters; ++it) {
         {
           Scope sc1(*timer, "spmv_dot");
-          bool done_ = false;
-          if (it > 0 && upd) {
-            done_ = launch_spmv_update<T>(sh, flags.as<uint16_t>(), scur, z.as<T>(), snext, as.as<T>(), sc(),
-                                          (int)it, partials.as<double>(), st);
-            if (done_) {
-              std::swap(scur, snext);
-            } else {  // sizes past the fused kernel's offsets: update, then the product
-              upd = false;
-              launch_update_s<T>(sh, scur, z.as<T>(), sc(), st);
-            }
-          }
-          if (!done_)
-            launch_spmv<T>(sh, flags.as<uint16_t>(), scur, as.as<T>(), sc(), (int)it, partials.as<double>(), st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(), st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
         }
-        if (fused_axpy()) {
-          Scope sc1(*timer, "mic_fwd_axpy");
-          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, p.as<T>(), scur, as.as<T>());
-          launch_dot_finish(partials.as<double>(), ndot, false, kOpResidual, sc(), tol, st);
-        } else {
-          {
-            Scope sc1(*timer, "axpy_norm");
-            launch_axpy_pr<T>(sh, p.as<T>(), r, scur, as.as<T>(), sc(), partials.as<double>(), mb, st);
-            launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
-          }
+        {
+          Scope sc1(*timer, "axpy_norm");
+          launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+          launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
+        }
+        {
           Scope sc1(*timer, "mic_fwd");
           mic_sweep(kMicFwd, r, q.as<T>());
         }
@@ -399,9 +370,9 @@ struct Projector {
           mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
           launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
         }
-        if (!upd) {
+        {
           Scope sc1(*timer, "update_s");
-          launch_update_s<T>(sh, scur, z.as<T>(), sc(), st);
+          launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
         }
         if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
           if (read_scalars().done) break;
@@ -489,7 +460,7 @@ struct Projector {
       }
       const double prev = rnorm;
       rnorm = hn[0];
-      if (getenv("SDR_TRACE_REFINE"))
+      if (trace_refine)
         fprintf(stderr, "[refine] pass %d: %d iterations (f32 recursive residual %.3e of %.3e), "
                 "f64 residual %.3e of max|b| %.3e -> %.3e\n", outer, r.iters, r.residual, inner,
                 rnorm, bnorm, rnorm / bnorm);
@@ -832,6 +803,8 @@ struct Solver : SolverBase {
                                       opts.solve_mode == SDR_SOLVE_RELATIVE, opts.refine_tol);
     last_pcg_iters = r.iters;
     last_residual = r.residual;
+    // the reference's NaN-blind inf_norm would report such a solve as converged
+    if (nanf) throw Error(SDR_E_NUMERIC, "pressure solve: NaN in the residual (non-finite faces or source?)");
   }
   // extrapolate_velocities (solver.rs:608-647) with 4 passes of extrapolate (960-1009)
   void extrapolate() {
@@ -1076,6 +1049,53 @@ struct Solver : SolverBase {
     last_substeps = taken;
   }
 
+  // ---------------------------------------------------------------- reductions for readbacks
+  Buf st_cpart, st_ppart, st_out, diag_count;
+  // One pass over the particles and one over the cells (metrics.hip); with is_wall set
+  // also the metrics' wet-wall marking, with counts the per-cell particle histogram.
+  FrameStats frame_stats(const uint8_t* is_wall, uint8_t* washed, uint8_t* wet, double cap, bool counts) {
+    if (!st_cpart.p) {
+      st_cpart.alloc(kStatBlocks * sizeof(CellStats));
+      st_ppart.alloc(kStatBlocks * sizeof(PartStats));
+      st_out.alloc(sizeof(FrameStats));
+    }
+    const uint32_t* cnt = nullptr;
+    if (counts) {
+      if (!diag_count.p) diag_count.alloc((size_t)ncells() * 4);
+      HIPCHK(hipMemsetAsync(diag_count.p, 0, (size_t)ncells() * 4, st));
+      if (n > 0) launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), diag_count.as<uint32_t>(), st);
+      cnt = diag_count.as<uint32_t>();
+    }
+    launch_frame_stats<T>(parts(cur), n, grid(), cells.as<uint8_t>(), pressure.as<T>(), is_wall, washed, wet, cap,
+                          cnt, st_cpart.as<CellStats>(), st_ppart.as<PartStats>(), st_out.as<FrameStats>(), st);
+    pin.ensure(sizeof(FrameStats) + 64);
+    HIPCHK(hipMemcpyAsync(pin.p, st_out.p, sizeof(FrameStats), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    FrameStats r;
+    memcpy(&r, pin.p, sizeof r);
+    return r;
+  }
+  double particle_volume() const {  // solver.rs:225-228
+    const double cell = dx * dx * dx;
+    const uint64_t p = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
+    return cell / (double)p;
+  }
+  sdr_diagnostics diagnostics() {  // solver.rs:225-233, 758-891; particles.rs:52-58
+    const FrameStats f = frame_stats(nullptr, nullptr, nullptr, 0.0, true);
+    sdr_diagnostics d{};
+    d.particle_volume = particle_volume();
+    d.fluid_volume = (double)n * d.particle_volume;
+    d.drained_volume = (double)drained * d.particle_volume;
+    d.cells_fluid_count = f.nfluid;
+    d.max_particles_per_cell = f.maxcount;
+    d.max_pressure = f.pmax;
+    d.mean_pressure = f.nfluid > 0 ? f.psum / (double)f.nfluid : 0.0;
+    d.max_speed = f.vmax;
+    d.kinetic_energy = 0.5 * f.ke;
+    d.particles = (uint64_t)n;
+    return d;
+  }
+
   // ---------------------------------------------------------------- host I/O
   void get_particles(double* pos, double* vel, uint64_t m) {
     if ((int64_t)m != n) throw Error(SDR_E_INVALID, "particle count mismatch");
@@ -1089,6 +1109,9 @@ struct Solver : SolverBase {
   }
   void set_particles(const double* pos, const double* vel, uint64_t m) {
     reserve((int64_t)m);
+    // the replaced particles leave the books, the new ones enter them, so that
+    // emitted = resident + drained + culled keeps holding
+    emitted = emitted - (uint64_t)n + m;
     n = (int64_t)m;
     if (m > 0) {
       aos_pos.ensure(m * 3 * sizeof(double));
@@ -1147,6 +1170,111 @@ struct Solver : SolverBase {
     emitted += (uint64_t)add;
     sorted_valid = false;
     return (uint64_t)add;
+  }
+};
+
+// ------------------------------------------------------------------ clinical metrics
+// `MetricsCollector` (metrics.rs:116-446) over a solver's device state: wall cells are
+// marked once, each record marks the wet wall cells and reduces the frame on the device.
+struct MetricsD {
+  const void* owner = nullptr;  // the solver this collector reads
+  double cavity_volume = 0;
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0;
+  hipStream_t st = nullptr;
+  Buf is_wall, washed, wet, wc;
+  uint64_t wall_count = 0;
+  std::vector<sdr_frame_metrics> history;
+  double injected_ml = 0;
+
+  template <class T>
+  void create(Solver<T>& s, double cv) {  // metrics.rs:119-182
+    HIPCHK(hipSetDevice(s.opts.device));
+    owner = &s;
+    cavity_volume = cv;
+    nx = s.nx, ny = s.ny, nz = s.nz, dx = s.dx, st = s.st;
+    const size_t nc = (size_t)s.ncells();
+    is_wall.alloc(nc), washed.alloc(nc), wet.alloc(nc), wc.alloc(8);
+    HIPCHK(hipMemsetAsync(washed.p, 0, nc, st));
+    HIPCHK(hipMemsetAsync(wet.p, 0, nc, st));
+    HIPCHK(hipMemsetAsync(wc.p, 0, 8, st));
+    // the collector's "inside" test is the solver's cavity mask (both sample the SDF at
+    // the cell centre, metrics.rs:125-134 and solver.rs:194-199)
+    launch_wall_cells(s.mask.template as<uint8_t>(), nx, ny, nz, is_wall.as<uint8_t>(),
+                      wc.as<unsigned long long>(), st);
+    HIPCHK(hipMemcpyAsync(&wall_count, wc.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  template <class T>
+  sdr_frame_metrics record(Solver<T>& s, uint64_t frame) {  // metrics.rs:188-316
+    if (&s != owner) throw Error(SDR_E_INVALID, "metrics collector used with another solver");
+    const double v_jet = [&] {  // solver.rs:107-114
+      const double area = M_PI * s.needle.radius * s.needle.radius;
+      return area > 0.0 ? s.needle.flow_rate / area : 0.0;
+    }();
+    const double stagnation = 0.5 * s.fluid.density * v_jet * v_jet;
+    const double hydrostatic = s.fluid.density * hlen(h3(s.fluid.gravity)) * ((double)ny * dx);
+    const double p_cap = 3.0 * stagnation + hydrostatic + 500.0;  // metrics.rs:236, 454
+    const FrameStats f = s.frame_stats(is_wall.as<uint8_t>(), washed.as<uint8_t>(), wet.as<uint8_t>(), p_cap, false);
+    const double pv = s.particle_volume();
+    const double fluid_volume = (double)s.n * pv, drained_volume = (double)s.drained * pv;
+    const double m3_to_ml = 1.0e6, pa_to_mmhg = 1.0 / 133.322;  // metrics.rs:23-24
+    injected_ml = (fluid_volume + drained_volume) * m3_to_ml;
+    sdr_frame_metrics m{};
+    m.frame = frame;
+    m.time_s = s.time;
+    m.particles = (uint64_t)s.n;
+    m.fluid_volume_ml = fluid_volume * m3_to_ml;
+    m.drained_volume_ml = drained_volume * m3_to_ml;
+    m.fill_fraction = cavity_volume > 0.0 ? fluid_volume / cavity_volume : 0.0;
+    m.wall_coverage = wall_count > 0 ? (double)f.nwashed / (double)wall_count : 0.0;
+    m.mean_pressure_pa = f.nfluid > 0 ? f.psum / (double)f.nfluid : 0.0;
+    m.max_pressure_pa = f.pmax;
+    m.max_pressure_mmhg = f.pmax * pa_to_mmhg;
+    // membrane_stats (metrics.rs:460-479): (0, 0, n) when every wet cell is an artifact
+    m.wall_pressure_pa = f.nacc > 0 ? f.wsum / (double)f.nacc : 0.0;
+    m.peak_wall_pressure_pa = f.nacc > 0 ? f.wpeak : 0.0;
+    m.peak_wall_pressure_mmhg = m.peak_wall_pressure_pa * pa_to_mmhg;
+    m.membrane_artifacts = f.nart;
+    m.max_speed_m_s = f.vmax;
+    m.kinetic_energy = 0.5 * f.ke;
+    m.pcg_iters = s.last_pcg_iters;
+    m.wet_wall_cells = f.nwet;
+    m.emitted = s.emitted, m.drained = s.drained, m.culled = s.culled;
+    history.push_back(m);
+    return m;
+  }
+
+  sdr_summary summary() const {  // metrics.rs:324-382
+    sdr_summary r{};
+    double peak_fill = 0, peak_cov = 0, peak_p = 0, peak_mem = 0, mean_mem_sum = 0;
+    for (const auto& m : history) {
+      peak_fill = std::fmax(peak_fill, m.fill_fraction);
+      peak_cov = std::fmax(peak_cov, m.wall_coverage);
+      peak_p = std::fmax(peak_p, m.max_pressure_pa);
+      peak_mem = std::fmax(peak_mem, m.peak_wall_pressure_pa);
+      mean_mem_sum = mean_mem_sum + m.wall_pressure_pa;
+      r.membrane_artifacts_total += m.membrane_artifacts;
+    }
+    const double pa_to_mmhg = 1.0 / 133.322;
+    r.frames = history.size();
+    const double mean_mem = history.empty() ? 0.0 : mean_mem_sum / (double)history.size();
+    if (!history.empty()) {
+      const auto& l = history.back();
+      r.final_fill_fraction = l.fill_fraction, r.final_wall_coverage = l.wall_coverage;
+      r.final_time_s = l.time_s;
+      r.total_drained_ml = l.drained_volume_ml;
+      r.injected_ml = l.fluid_volume_ml + l.drained_volume_ml;
+      r.emitted = l.emitted, r.resident = l.particles, r.drained = l.drained, r.culled = l.culled;
+    }
+    r.peak_fill_fraction = peak_fill, r.peak_wall_coverage = peak_cov;
+    r.peak_membrane_pressure_pa = peak_mem, r.peak_membrane_pressure_mmhg = peak_mem * pa_to_mmhg;
+    r.mean_membrane_pressure_pa = mean_mem, r.mean_membrane_pressure_mmhg = mean_mem * pa_to_mmhg;
+    r.peak_pressure_pa = peak_p, r.peak_pressure_mmhg = peak_p * pa_to_mmhg;
+    // conservation from the integer counts (emitted = resident + drained + culled)
+    r.mass_balance = r.emitted > 0 ? (double)(r.resident + r.drained + r.culled) / (double)r.emitted : 1.0;
+    return r;
   }
 };
 
@@ -1328,6 +1456,54 @@ int sdr_solver_set_state(sdr_solver* s, const sdr_state* st) {
               S.last_residual = st->last_residual);
 }
 int sdr_solver_synchronize(sdr_solver* s) { return WITH(s, HIPCHK(hipStreamSynchronize(S.st))); }
+int sdr_solver_diagnostics(sdr_solver* s, sdr_diagnostics* out) {
+  return WITH(s, if (!out) throw Error(SDR_E_INVALID, "null output"); *out = S.diagnostics());
+}
+int sdr_solver_cfl_dt(sdr_solver* s, double* dt) {
+  return WITH(s, if (!dt) throw Error(SDR_E_INVALID, "null output"); *dt = S.cfl_dt());
+}
+
+struct sdr_metrics {
+  sdr::MetricsD m;
+};
+int sdr_metrics_new(sdr_solver* s, double cavity_volume, sdr_metrics** out) {
+  return WITH(s, if (!out) throw Error(SDR_E_INVALID, "null output");
+              std::unique_ptr<sdr_metrics> m(new sdr_metrics()); m->m.create(S, cavity_volume);
+              *out = m.release());
+}
+int sdr_metrics_free(sdr_metrics* m) {
+  delete m;
+  return SDR_OK;
+}
+int sdr_metrics_wall_count(const sdr_metrics* m, uint64_t* n) {
+  return guard([&] {
+    if (!m || !n) throw Error(SDR_E_INVALID, "null argument");
+    *n = m->m.wall_count;
+  });
+}
+int sdr_metrics_record(sdr_metrics* m, sdr_solver* s, uint64_t frame, sdr_frame_metrics* out) {
+  if (!m) {
+    sdr::g_err = "null metrics collector";
+    return SDR_E_INVALID;
+  }
+  return WITH(s, sdr_frame_metrics r = m->m.record(S, frame); if (out) *out = r);
+}
+int sdr_metrics_summary(const sdr_metrics* m, sdr_summary* out) {
+  return guard([&] {
+    if (!m || !out) throw Error(SDR_E_INVALID, "null argument");
+    *out = m->m.summary();
+  });
+}
+int sdr_metrics_get_cells(const sdr_metrics* m, uint8_t* washed, uint8_t* is_wall) {
+  return guard([&] {
+    if (!m) throw Error(SDR_E_INVALID, "null argument");
+    const size_t nc = (size_t)m->m.nx * m->m.ny * m->m.nz;
+    if (washed) HIPCHK(hipMemcpyAsync(washed, m->m.washed.p, nc, hipMemcpyDeviceToHost, m->m.st));
+    if (is_wall) HIPCHK(hipMemcpyAsync(is_wall, m->m.is_wall.p, nc, hipMemcpyDeviceToHost, m->m.st));
+    HIPCHK(hipStreamSynchronize(m->m.st));
+  });
+}
+
 int sdr_solver_enable_timing(sdr_solver* s, int32_t on) {
   return WITH(s, S.timer.collect(); S.timer.on = on != 0; S.timer.sweeps_only = on == 2;
               if (on) S.timer.acc.clear(), S.timer.seen.clear());
@@ -1386,10 +1562,12 @@ int sdr_project_capturing_with_source(const sdr_dims* gd, double* u, double* v, 
     }
     sdr::GridD<double> g{nx, ny, nz, gd->dx, gd->origin.x, gd->origin.y, gd->origin.z};
     sdr_solve_params prm = params ? *params : sdr_solve_params{200, 1e-6};
+    int nanf = 0;
     sdr_solve_report r = pj.project(g, sdr::FacesD<double>{du.as<double>(), dv.as<double>(), dw.as<double>()},
                                     dc.as<uint8_t>(), nullptr, source ? ds.as<double>() : nullptr, rho, dt,
-                                    prm.max_iters, prm.tol, false, 1.0, 0.0, dp.as<double>(), nullptr,
+                                    prm.max_iters, prm.tol, false, 1.0, 0.0, dp.as<double>(), &nanf,
                                     o.solve_mode == SDR_SOLVE_RELATIVE, o.refine_tol);
+    if (nanf) throw Error(SDR_E_NUMERIC, "pressure solve: NaN in the residual (non-finite faces or source?)");
     HIPCHK(hipMemcpyAsync(u, du.p, nu * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(v, dv.p, nv * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, st));

@@ -286,6 +286,45 @@ class Solver:
         self.lib.solver_get_pressure(self._h, A.dptr(p))
         return p
 
+    def diagnostics(self) -> dict:
+        """The reference's pub diagnostics (`particle_volume`, `fluid_volume`,
+        `drained_volume`, `cells_fluid_count`, `max_particles_per_cell`, `max_pressure`,
+        `mean_pressure`, `max_speed`: src/solver.rs:225-233, 758-891) and
+        `ParticleSet::kinetic_energy` (src/particles.rs:52-58), in one call."""
+        d = A.Diagnostics()
+        self.lib.solver_diagnostics(self._h, C.byref(d))
+        return A.struct_dict(d)
+
+    def particle_volume(self) -> float:
+        return self.diagnostics()["particle_volume"]
+
+    def fluid_volume(self) -> float:
+        return self.diagnostics()["fluid_volume"]
+
+    def drained_volume(self) -> float:
+        return self.diagnostics()["drained_volume"]
+
+    def cells_fluid_count(self) -> int:
+        return int(self.diagnostics()["cells_fluid_count"])
+
+    def max_particles_per_cell(self) -> int:
+        return int(self.diagnostics()["max_particles_per_cell"])
+
+    def max_pressure(self) -> float:
+        return self.diagnostics()["max_pressure"]
+
+    def mean_pressure(self) -> float:
+        return self.diagnostics()["mean_pressure"]
+
+    def max_speed(self) -> float:
+        return self.diagnostics()["max_speed"]
+
+    def cfl_dt(self) -> float:
+        """The private `Solver::cfl_dt` (src/solver.rs:262-281) on the current grid."""
+        out = C.c_double()
+        self.lib.solver_cfl_dt(self._h, C.byref(out))
+        return out.value
+
     def cell_center(self, i, j, k):
         o, d = self.origin, self.dx
         return (o[0] + (i + 0.5) * d, o[1] + (j + 0.5) * d, o[2] + (k + 0.5) * d)
