@@ -581,6 +581,14 @@ struct Solver : SolverBase {
     return BoxD<T>{has_outlet ? 1 : 0, (T)outlet.min.x, (T)outlet.min.y, (T)outlet.min.z,
                    (T)outlet.max.x, (T)outlet.max.y, (T)outlet.max.z};
   }
+  // The static classification (cavity mask, outlet cells) is computed once, in f64 in
+  // both precisions: the precision form then classifies exactly the cells the plain
+  // path does, and only its particles' cell_of runs in f32.
+  GridD<double> grid64() const { return GridD<double>{nx, ny, nz, dx, origin.x, origin.y, origin.z}; }
+  BoxD<double> outlet_box64() const {
+    return BoxD<double>{has_outlet ? 1 : 0, outlet.min.x, outlet.min.y, outlet.min.z,
+                        outlet.max.x, outlet.max.y, outlet.max.z};
+  }
   Parts<T> parts(int which) const {
     return Parts<T>{part[which][0].as<T>(), part[which][1].as<T>(), part[which][2].as<T>(),
                     part[which][3].as<T>(), part[which][4].as<T>(), part[which][5].as<T>(),
@@ -643,8 +651,20 @@ struct Solver : SolverBase {
     for (Buf* b : {&fu, &fv, &fw, &su, &sv, &sw, &pressure})
       HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, st));
     HIPCHK(hipMemsetAsync(cells.p, kAir, (size_t)nc, st));
-    launch_cavity_mask<T>(grid(), sdf(), mask.as<uint8_t>(), st);
-    launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
+    {
+      Buf sdf64;  // the f64 node values for the mask (the device keeps them as T)
+      const double* d64 = (const double*)sdf_data.p;
+      if (sizeof(T) != sizeof(double)) {
+        sdf64.alloc(nsdf * sizeof(double));
+        HIPCHK(hipMemcpy(sdf64.p, s->data, nsdf * sizeof(double), hipMemcpyHostToDevice));
+        d64 = sdf64.as<double>();
+      }
+      SdfD<double> sd{(int)sdf_desc.nx, (int)sdf_desc.ny, (int)sdf_desc.nz, sdf_desc.dx, sdf_desc.origin.x,
+                      sdf_desc.origin.y, sdf_desc.origin.z, d64};
+      launch_cavity_mask<double>(grid64(), sd, mask.as<uint8_t>(), st);
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    launch_static_cls<double>(grid64(), mask.as<uint8_t>(), outlet_box64(), static_cls.as<uint8_t>(), st);
     if (const char* e = getenv("SDR_REDIST_SEQUENTIAL")) redist_sequential = atoi(e) != 0;
     proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
     proj.init(nx, ny, nz, dx, st, &timer);
@@ -1145,7 +1165,7 @@ struct Solver : SolverBase {
   void set_outlet(const sdr_aabb* o) {
     has_outlet = o != nullptr;
     if (o) outlet = *o;
-    launch_static_cls<T>(grid(), mask.as<uint8_t>(), outlet_box(), static_cls.as<uint8_t>(), st);
+    launch_static_cls<double>(grid64(), mask.as<uint8_t>(), outlet_box64(), static_cls.as<uint8_t>(), st);
     HIPCHK(hipStreamSynchronize(st));
   }
   uint64_t seed_fill(uint32_t per_cell, double y_lo, double y_hi, uint64_t seed, const sdr_vec3* v) {

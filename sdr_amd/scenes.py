@@ -25,30 +25,41 @@ def smooth_min(a, b, k):
     return (b * (1.0 - h) + a * h) - k * h * (1.0 - h)
 
 
+def _lattice(a, b, c, seed, o):
+    """Hashed lattice values in [-1, 1] at integer points (a, b, c)."""
+    h = (a.astype(np.int64) * 73856093) ^ (b.astype(np.int64) * 19349663) ^ \
+        (c.astype(np.int64) * 83492791) ^ (seed * 2654435761 + o * 97)
+    h = (h * 0x5DEECE66D + 11) & 0xFFFFFFFF
+    return (h / 4294967295.0) * 2.0 - 1.0
+
+
 def value_noise(x, y, z, scale, seed, octaves=1):
-    """Seeded smooth value noise in [-1, 1] (trilinear over a hashed lattice, smoothstep
-    weights).  The build's stand-in for the wall noise of the larger configs."""
-    out = np.zeros_like(x)
+    """Seeded smooth value noise in [-1, 1]: per octave, the hashed lattice values
+    blended with smoothstep weights (a tensor product, evaluated axis by axis).  The
+    build's stand-in for the wall noise of the larger configs.  x, y, z are the (k, j, i)
+    coordinate arrays of a grid of nodes (as `Sdf.from_fn` passes them)."""
+    xs, ys, zs = x[0, 0, :], y[0, :, 0], z[:, 0, 0]
+    out = np.zeros(x.shape)
     amp, tot = 1.0, 0.0
     for o in range(octaves):
-        s = scale / (2 ** o)
-        gx, gy, gz = x / s, y / s, z / s
-        ix, iy, iz = np.floor(gx), np.floor(gy), np.floor(gz)
-        fx, fy, fz = gx - ix, gy - iy, gz - iz
-        ux, uy, uz = (f * f * (3 - 2 * f) for f in (fx, fy, fz))
-
-        def lat(a, b, c):
-            h = (a.astype(np.int64) * 73856093) ^ (b.astype(np.int64) * 19349663) ^ \
-                (c.astype(np.int64) * 83492791) ^ (seed * 2654435761 + o * 97)
-            h = (h * 0x5DEECE66D + 11) & 0xFFFFFFFF
-            return (h / 4294967295.0) * 2.0 - 1.0
-
-        v = 0.0
-        for dk in (0, 1):
-            for dj in (0, 1):
-                for di in (0, 1):
-                    w = (ux if di else 1 - ux) * (uy if dj else 1 - uy) * (uz if dk else 1 - uz)
-                    v = v + w * lat(ix + di, iy + dj, iz + dk)
+        sc = scale / (2 ** o)
+        ax = []
+        for c in (xs, ys, zs):
+            g = c / sc
+            i = np.floor(g)
+            f = g - i
+            ax.append((i.astype(np.int64), f * f * (3 - 2 * f)))
+        (ix, ux), (iy, uy), (iz, uz) = ax
+        lo = [a.min() for a in (ix, iy, iz)]
+        la = [np.arange(l, a.max() + 2) for l, a in zip(lo, (ix, iy, iz))]
+        kk, jj, ii = np.meshgrid(la[2], la[1], la[0], indexing="ij")
+        L = _lattice(ii, jj, kk, seed, o)                    # (Lz, Ly, Lx)
+        a0 = ix - lo[0]
+        Lx = L[:, :, a0] * (1 - ux) + L[:, :, a0 + 1] * ux  # (Lz, Ly, nx)
+        b0 = iy - lo[1]
+        Ly = Lx[:, b0, :] * (1 - uy)[None, :, None] + Lx[:, b0 + 1, :] * uy[None, :, None]
+        c0 = iz - lo[2]
+        v = Ly[c0] * (1 - uz)[:, None, None] + Ly[c0 + 1] * uz[:, None, None]
         out = out + amp * v
         tot += amp
         amp *= 0.5

@@ -104,3 +104,45 @@ def paired(scene_name: str, orc_lib, dev_kw=None, stride=1):
         d = scenes.make_solver(sc, **dev_kw)
         d.set_particles(pos)
     return sc, o, d
+
+
+def progress(msg: str) -> None:
+    """A progress line on the real stderr (past pytest's capture), for long GPU tests."""
+    import sys
+
+    print(f"[test] {msg}", file=sys.__stderr__, flush=True)
+
+
+def count_source(solver, scene, cells, f32: bool):
+    """volume_source (src/solver.rs:392-423) from the solver's particles, with cell_of in
+    the solver's own arithmetic (f32 storage puts a particle within rounding distance of
+    a cell face where the device puts it).  None when volume control is off."""
+    import ctypes as C
+
+    from sdr_amd import _abi as A
+
+    fp = scene.fluid
+    if fp.volume_stiffness <= 0.0:
+        return None
+    n = solver.particle_count()
+    pos = np.zeros((n, 3), np.float64)
+    solver.lib.solver_get_particles(solver._h, A.dptr(pos), None, n)
+    ft = np.float32 if f32 else np.float64
+    g = (pos.astype(ft) - np.asarray(solver.origin).astype(ft)) / ft(solver.dx)
+    del pos
+    ok = np.all(g >= 0, axis=1)
+    ijk = g.astype(np.int64)
+    del g
+    ok &= (ijk[:, 0] < solver.nx) & (ijk[:, 1] < solver.ny) & (ijk[:, 2] < solver.nz)
+    cnt = np.bincount(((ijk[ok, 2] * solver.ny + ijk[ok, 1]) * solver.nx + ijk[ok, 0]),
+                      minlength=solver.nx * solver.ny * solver.nz).reshape(cells.shape)
+    rest = float(max(fp.particles_per_cell, 1))
+    unit = solver.dx / scene.dt
+    over = cnt / rest - 1.0
+    return np.where((cells == FLUID) & (over > 0.0),
+                    np.minimum(fp.volume_stiffness * over * unit, 0.25 * unit), 0.0)
+
+
+def divergence(u, v, w):
+    """Per-cell (u+ - u-) + (v+ - v-) + (w+ - w-) (the projection's div, pressure.rs:152-168)."""
+    return (u[:, :, 1:] - u[:, :, :-1]) + (v[:, 1:, :] - v[:, :-1, :]) + (w[1:] - w[:-1])
