@@ -336,6 +336,26 @@ int sdr_project_capturing_with_source(const sdr_dims* grid, double* u, double* v
                                       double* pressure_out, sdr_solve_report* report,
                                       const sdr_options* opts);
 
+/* Working memory of sdr_project_device_noalloc: a fixed array per device (64 MiB). */
+#define SDR_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference (which allocates a fresh pressure Vec per call,
+ * pressure.rs:123-239): the projection of sdr_project_capturing_with_source, f64, for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  u, v, w (in/out), cells, source
+ * (nullable) and pressure_out (nullable) are DEVICE pointers of the same layouts; the
+ * work runs on `stream` (a hipStream_t, NULL = the null stream) and the call returns
+ * when it has finished.  All working memory (solve vectors, flags, reductions, halo
+ * words) comes from one static device array of SDR_NOALLOC_SCRATCH_BYTES per device,
+ * cleared by kernels where it must start at zero; nothing is allocated or freed.  A
+ * grid whose working set exceeds it (about 80^3 cells) is rejected with SDR_E_INVALID.
+ * Every call on a device shares that device's array: calls that overlap on one device
+ * (concurrent streams or threads) race and must be serialised by the caller. */
+int sdr_project_device_noalloc(const sdr_dims* grid, double* u, double* v, double* w, const uint8_t* cells,
+                               double rho, double dt, const sdr_solve_params* params, const double* source,
+                               double* pressure_out, sdr_solve_report* report, const sdr_options* opts,
+                               void* stream);
+
 /* ---------------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference runs one scenario per process,
  * main.rs:215-386).  Many irrigation scenarios of different sizes share one GPU: each

@@ -192,6 +192,9 @@ DEVICE_ONLY = {
                          c_u64_p],
     "solver_footprint": [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, c_u64_p],
     "serve": [C.POINTER(Scenario), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(ScenarioResult)],
+    # device pointers (u, v, w, cells, source, pressure) and a hipStream_t
+    "project_device_noalloc": [C.POINTER(Dims), P, P, P, P, C.c_double, C.c_double, C.POINTER(SolveParamsC),
+                               P, P, C.POINTER(SolveReportC), C.POINTER(Options), P],
 }
 
 

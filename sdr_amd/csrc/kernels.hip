@@ -1633,6 +1633,44 @@ void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st) 
 }
 
 // ------------------------------------------------------------------ instantiations
+// ------------------------------------------------------------------ fills without memset nodes
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = 0u;
+}
+void launch_zero_u32(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  k_zero_u32<<<(unsigned)b, kBlock, 0, st>>>(p, n);
+}
+
+// The vector kernels' unit table of a plane (one entry per (row J, 64-slot chunk) pair
+// holding in-grid slots, in row order, (J << 8) | chunk): row J's entries start after
+// the chunk counts of the rows before it.
+__device__ __host__ inline int unit_chunks(int J, int nx, int ny) {
+  const int imin = J - ny + 1 > 0 ? J - ny + 1 : 0, imax = J < nx - 1 ? J : nx - 1;
+  return imax / 64 - imin / 64 + 1;
+}
+__global__ void k_build_units(int nx, int ny, int NJ, uint32_t* __restrict__ out) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= NJ) return;
+  int start = 0;
+  for (int r = 0; r < J; ++r) start += unit_chunks(r, nx, ny);
+  const int imin = J - ny + 1 > 0 ? J - ny + 1 : 0;
+  const int n = unit_chunks(J, nx, ny);
+  for (int c = 0; c < n; ++c) out[start + c] = ((uint32_t)J << 8) | (uint32_t)(imin / 64 + c);
+}
+int units_per_plane(int nx, int ny) {
+  int n = 0;
+  for (int J = 0; J < nx + ny - 1; ++J) n += unit_chunks(J, nx, ny);
+  return n;
+}
+void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
+  const int NJ = nx + ny - 1;
+  k_build_units<<<(NJ + 63) / 64, 64, 0, st>>>(nx, ny, NJ, out);
+}
+
 #define SDR_INST(T)                                                                                \
   template void launch_keys<T>(const Parts<T>&, int64_t, GridD<T>, uint32_t*, uint64_t*, uint32_t*, \
                                bool, hipStream_t);                                                 \

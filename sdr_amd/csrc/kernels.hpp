@@ -193,4 +193,11 @@ void launch_static_cls(GridD<T> g, const uint8_t* mask, BoxD<T> outlet, uint8_t*
 template <class T>
 void launch_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask, hipStream_t st);
 
+// Zero n 32-bit words with a kernel (no memset node in a captured graph).
+void launch_zero_u32(uint32_t* p, int64_t n, hipStream_t st);
+// The per-plane work-unit table of the vector kernels on the device (units_per_plane
+// entries), the same table Shear::units holds.
+int units_per_plane(int nx, int ny);
+void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st);
+
 }  // namespace sdr

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,20 +49,28 @@ struct Error : std::runtime_error {
 struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: a view of memory the buffer does not own (never freed)
   Buf() = default;
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
   ~Buf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
   }
   void alloc(size_t b) {
     release();
     if (b == 0) b = 256;
     HIPCHK(hipMalloc(&p, b));
     bytes = b;
+  }
+  void view(void* q, size_t b) {
+    release();
+    p = q;
+    bytes = b;
+    owned = false;
   }
   void ensure(size_t b) {
     if (b > bytes) alloc(b);
@@ -210,30 +219,25 @@ struct Projector {
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
   int ncu = 0, dev = 0;  // compute units of the solver's device
   bool trace_refine = false;  // SDR_TRACE_REFINE=1: one stderr line per refinement pass
+  // All working memory is a view of caller-provided memory: nothing is allocated, and
+  // memory is cleared by a fill kernel instead of a memset (graph-capturable).
+  bool noalloc = false;
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
-  void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
+  // Sizes of the working buffers (elements) and the sweep shape for a grid.
+  struct Plan {
+    size_t n, nb, hx, hz, nunits;
+  };
+  Plan plan(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
     nx = x, ny = y, nz = zz, dx = h, st = stream, timer = t;
     sh = Shear{nx, ny, nz, nx + ny - 1};
+    Plan pl{};
     // the vector kernels' work units: the (row, 64-slot chunk) pairs of a plane that hold
     // in-grid slots, in row order
-    if (nx <= 256 * 64) {
-      std::vector<uint32_t> u;
-      for (int J = 0; J < sh.NJ; ++J) {
-        const int imin = std::max(0, J - ny + 1), imax = std::min(J, nx - 1);
-        for (int c = imin / 64; c <= imax / 64; ++c) u.push_back(((uint32_t)J << 8) | (uint32_t)c);
-      }
-      units.alloc(u.size() * sizeof(uint32_t));
-      HIPCHK(hipMemcpy(units.p, u.data(), u.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-      sh.units = units.as<uint32_t>();
-      sh.units_per_plane = (int)u.size();
-    }
-    size_t n = (size_t)sh.size();
-    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(n * sizeof(T));
-    flags.alloc(n * sizeof(uint16_t));
+    pl.nunits = nx <= 256 * 64 ? (size_t)units_per_plane(nx, ny) : 0;
+    const size_t n = (size_t)sh.size();
+    pl.n = n;
     int nb = std::max(dot_blocks(sh, true), dot_blocks(sh, false));
-    scal.alloc(sizeof(PcgScalars));
-    host_scal.ensure(sizeof(PcgScalars) + 16);
     // both precisions: 7-plane tiles with an I/O wave, halo prefetch depth 4 (k_mic_sweep3);
     // SDR_MIC_VARIANT (read once per solver) selects another tested shape
     mic_variant = kMicDefaultVariant;
@@ -255,15 +259,82 @@ struct Projector {
     }
     if (hx * sizeof(uint64_t) >= lim || hz * sizeof(uint64_t) >= lim) mic_variant = -1;
     for (int v = 0; v < mic_pipe_variants(); ++v) nb = std::max(nb, mic_pipe_dot_partials(mic_pipe_shape(sh, v, ncu)));
-    partials.alloc((size_t)nb * sizeof(double));
     if (mic_variant < 0) hx = hz = 1;
-    halo_x.alloc(hx * sizeof(uint64_t));
-    halo_z.alloc(hz * sizeof(uint64_t));
+    pl.nb = (size_t)nb, pl.hx = hx, pl.hz = hz;
+    return pl;
+  }
+
+  void init(int x, int y, int zz, double h, hipStream_t stream, Timer* t) {
+    const Plan pl = plan(x, y, zz, h, stream, t);
+    if (pl.nunits) {
+      std::vector<uint32_t> u;
+      for (int J = 0; J < sh.NJ; ++J) {
+        const int imin = std::max(0, J - ny + 1), imax = std::min(J, nx - 1);
+        for (int c = imin / 64; c <= imax / 64; ++c) u.push_back(((uint32_t)J << 8) | (uint32_t)c);
+      }
+      units.alloc(u.size() * sizeof(uint32_t));
+      HIPCHK(hipMemcpy(units.p, u.data(), u.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      sh.units = units.as<uint32_t>();
+      sh.units_per_plane = (int)u.size();
+    }
+    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) bb->alloc(pl.n * sizeof(T));
+    flags.alloc(pl.n * sizeof(uint16_t));
+    scal.alloc(sizeof(PcgScalars));
+    host_scal.ensure(sizeof(PcgScalars) + 16);
+    partials.alloc(pl.nb * sizeof(double));
+    halo_x.alloc(pl.hx * sizeof(uint64_t));
+    halo_z.alloc(pl.hz * sizeof(uint64_t));
     pipe_err.alloc(sizeof(int));
     sink.alloc(1024 * sizeof(T));
     HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
     HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
     HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
+  }
+
+  // Bytes of caller memory init_noalloc carves for a plan (256-byte aligned pieces).
+  static size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+  static size_t noalloc_bytes(const Plan& pl) {
+    return 7 * up256(pl.n * sizeof(T)) + up256(pl.n * 2) + up256(pl.nb * 8) + up256(sizeof(PcgScalars)) +
+           up256(pl.nunits * 4) + up256(pl.hx * 8) + up256(pl.hz * 8) + up256(sizeof(int)) + up256(1024 * sizeof(T));
+  }
+  // The same working set as init, as views of [base, base + cap): no allocation, and the
+  // memory that must start at zero (halo words, the wait-error flag) cleared by kernels.
+  void init_noalloc(int x, int y, int zz, double h, hipStream_t stream, Timer* t, uint8_t* base, size_t cap) {
+    const Plan pl = plan(x, y, zz, h, stream, t);
+    const size_t need = noalloc_bytes(pl);
+    if (need > cap)
+      throw Error(SDR_E_INVALID, "grid " + std::to_string(x) + "x" + std::to_string(y) + "x" + std::to_string(zz) +
+                                     " needs " + std::to_string(need) + " bytes of working memory; the no-alloc "
+                                     "projection holds " + std::to_string(cap));
+    noalloc = true;
+    size_t off = 0;
+    auto take = [&](Buf& bf, size_t bytes) {
+      bf.view(base + off, up256(bytes));
+      off += up256(bytes);
+    };
+    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) take(*bb, pl.n * sizeof(T));
+    take(flags, pl.n * 2);
+    take(partials, pl.nb * 8);
+    take(scal, sizeof(PcgScalars));
+    take(units, pl.nunits * 4);
+    take(halo_x, pl.hx * 8);
+    take(halo_z, pl.hz * 8);
+    take(pipe_err, sizeof(int));
+    take(sink, 1024 * sizeof(T));
+    if (pl.nunits) {
+      launch_build_units(nx, ny, units.as<uint32_t>(), st);
+      sh.units = units.as<uint32_t>();
+      sh.units_per_plane = (int)pl.nunits;
+    }
+    zero(halo_x, halo_x.bytes);
+    zero(halo_z, halo_z.bytes);
+    zero(pipe_err, sizeof(int));
+  }
+  void zero(Buf& bf, size_t bytes) {
+    if (noalloc)
+      launch_zero_u32(bf.as<uint32_t>(), (int64_t)((bytes + 3) / 4), st);
+    else
+      HIPCHK(hipMemsetAsync(bf.p, 0, bytes, st));
   }
 
   PcgScalars* sc() const { return scal.as<PcgScalars>(); }
@@ -283,8 +354,8 @@ struct Projector {
     if (op == kMicBuild && mic_pipe_shape(sh, variant, ncu).kind != 0) variant = sizeof(T) == 8 ? 1 : 0;
     MicPipeShape ps = mic_pipe_shape(sh, variant, ncu);
     if (++sweep_tag == 0) {  // tags wrapped: clear the halo words so no stale tag matches
-      HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
-      HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
+      zero(halo_x, halo_x.bytes);
+      zero(halo_z, halo_z.bytes);
       sweep_tag = 1;
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
@@ -299,7 +370,7 @@ struct Projector {
     HIPCHK(hipMemcpyAsync(&e, pipe_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (e) {
-      HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
+      zero(pipe_err, sizeof(int));
       throw Error(SDR_E_NUMERIC, "pipelined MIC(0) sweep: a tile's bounded wait expired");
     }
   }
@@ -311,6 +382,12 @@ struct Projector {
   }
 
   PcgScalars read_scalars() {
+    if (noalloc) {  // no pinned staging: a pageable copy
+      PcgScalars r;
+      HIPCHK(hipMemcpyAsync(&r, scal.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return r;
+    }
     HIPCHK(hipMemcpyAsync(host_scal.p, scal.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     PcgScalars r;
@@ -324,9 +401,9 @@ struct Projector {
     const size_t n = (size_t)sh.size();
     {
       Scope sc_(*timer, "pcg_setup");
-      for (Buf* bb : {&z, &s, &p, &as, &q}) HIPCHK(hipMemsetAsync(bb->p, 0, n * sizeof(T), st));
-      if (build) HIPCHK(hipMemsetAsync(precon.p, 0, n * sizeof(T), st));
-      HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(PcgScalars), st));
+      for (Buf* bb : {&z, &s, &p, &as, &q}) zero(*bb, n * sizeof(T));
+      if (build) zero(precon, n * sizeof(T));
+      zero(scal, sizeof(PcgScalars));
       int mb = maxabs_blocks(sh);
       launch_maxabs_partials<T>(sh, b.as<T>(), partials.as<double>(), mb, sc(), st);
       launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), tol, st, rel ? 1 : 0);
@@ -1298,6 +1375,30 @@ struct MetricsD {
   }
 };
 
+// ------------------------------------------------------------------ no-alloc working memory
+// The working memory of sdr_project_device_noalloc: one fixed array per device, held
+// until the process ends.  In host code `scratch` is only the host-side stand-in of the
+// symbol; its device address (per device) comes from hipGetSymbolAddress, taken once per
+// device with that device current, and is what kernels and fills are given.
+static __device__ __align__(256) unsigned char scratch[SDR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev`.
+void* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw Error(SDR_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(SDR_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 // ------------------------------------------------------------------ C ABI plumbing
 thread_local std::string g_err;
 
@@ -1593,6 +1694,40 @@ int sdr_project_capturing_with_source(const sdr_dims* gd, double* u, double* v, 
     HIPCHK(hipMemcpyAsync(w, dw.p, nw * 8, hipMemcpyDeviceToHost, st));
     if (pressure_out) HIPCHK(hipMemcpyAsync(pressure_out, dp.p, nc * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (report) *report = r;
+  });
+}
+
+int sdr_project_device_noalloc(const sdr_dims* gd, double* u, double* v, double* w, const uint8_t* cells,
+                               double rho, double dt, const sdr_solve_params* params, const double* source,
+                               double* pressure_out, sdr_solve_report* report, const sdr_options* opts,
+                               void* stream) {
+  return guard([&] {
+    if (!gd || !u || !v || !w || !cells) throw Error(SDR_E_INVALID, "null argument");
+    if (gd->nx < 2 || gd->ny < 2 || gd->nz < 2) throw Error(SDR_E_INVALID, "grid needs >= 2 cells per axis");
+    if (gd->nx > (1u << 20) || gd->ny > (1u << 20) || gd->nz > (1u << 20))
+      throw Error(SDR_E_INVALID, "grid too large for the no-alloc projection");
+    sdr_options o = opts ? *opts : sdr_options{SDR_PRECISION_F64, SDR_DOT_EXACT, 0, 1e-8, SDR_SOLVE_REFERENCE, 0};
+    if (o.precision != SDR_PRECISION_F64) throw Error(SDR_E_INVALID, "standalone projection is f64");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
+    HIPCHK(hipSetDevice(o.device));
+    uint8_t* base = (uint8_t*)sdr::scratch_on(o.device);
+    hipStream_t st = (hipStream_t)stream;
+    sdr::Timer timer;  // off: no events
+    timer.st = st;
+    const int nx = (int)gd->nx, ny = (int)gd->ny, nz = (int)gd->nz;
+    sdr::Projector<double> pj;
+    pj.exact_dot = o.dot_mode == SDR_DOT_EXACT;
+    pj.init_noalloc(nx, ny, nz, gd->dx, st, &timer, base, SDR_NOALLOC_SCRATCH_BYTES);
+    sdr::GridD<double> g{nx, ny, nz, gd->dx, gd->origin.x, gd->origin.y, gd->origin.z};
+    sdr_solve_params prm = params ? *params : sdr_solve_params{200, 1e-6};
+    int nanf = 0;
+    sdr_solve_report r = pj.project(g, sdr::FacesD<double>{u, v, w}, cells, nullptr, source, rho, dt,
+                                    prm.max_iters, prm.tol, false, 1.0, 0.0, pressure_out, &nanf,
+                                    o.solve_mode == SDR_SOLVE_RELATIVE, o.refine_tol);
+    HIPCHK(hipStreamSynchronize(st));
+    if (nanf) throw Error(SDR_E_NUMERIC, "pressure solve: NaN in the residual (non-finite faces or source?)");
     if (report) *report = r;
   });
 }
