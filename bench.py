@@ -41,6 +41,7 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
         "p2g": np_ * 6 * s + 3 * nc * s,        # read x, v; write u, v, w
         "g2p": np_ * 9 * s + 6 * nc * s,        # read x, v; write v; read u,v,w and saved
         "advect": np_ * 6 * s + 3 * nc * s + nsdf * s,  # read/write x; read u,v,w; SDF
+        "g2p_advect": np_ * 12 * s + 6 * nc * s + nsdf * s,  # read/write x, v; read u,v,w, saved; SDF
     }
     return table.get(name)
 

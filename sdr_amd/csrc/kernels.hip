@@ -961,6 +961,23 @@ __global__ void k_scale_cast(int64_t n, const double* __restrict__ r, double inv
   if (t < n) b[t] = (T)(r[t] * inv);
 }
 
+// x *= f (the search direction carried into new residual units)
+template <class T>
+__global__ void k_scale_inplace(int64_t n, T* __restrict__ x, double f) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = (T)((double)x[t] * f);
+}
+
+// The PCG scalars after a residual replacement: sigma into the new residual units (or
+// 1 for a restart, whose search direction has been cleared), the new stopping bound,
+// and the loop running again.
+__global__ void k_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int restart) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc->sigma = restart ? 1.0 : sc->sigma * f2;
+  sc->tol_eff = tol_eff;
+  sc->done = 0;
+}
+
 // p += scale * e in f64
 template <class T>
 __global__ void k_accum64(int64_t n, double* __restrict__ p, const T* __restrict__ e, double scale) {
@@ -1046,6 +1063,37 @@ __global__ void k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T
   if (len > g.dx) st = st * (g.dx / len);
   V3<T> q = project_inside(sdf, g.dx, p0 + st);
   V3<T> r = sdf.sample(q) <= T(0) ? q : p0;
+  p.px[t] = r.x;
+  p.py[t] = r.y;
+  p.pz[t] = r.z;
+}
+
+// g2p then advect of the same particle in one pass (solver.rs:650-711): the PIC sample
+// of g2p is advect's first RK2 sample (same grid, same position), so the particle is
+// read once and that sample is taken once; every value is the one the two kernels give.
+template <class T>
+__global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip,
+                             SdfD<T> sdf, T dt) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const V3<T> p0 = load_pos(p, t);
+  const V3<T> pic = velocity_at(g, f, p0);
+  const V3<T> l = g.local(p0);
+  T dvx = trilinear_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5));
+  T dvy = trilinear_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5));
+  T dvz = trilinear_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z);
+  const V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
+  const V3<T> nv = fv * flip + pic * (T(1) - flip);
+  p.vx[t] = nv.x;
+  p.vy[t] = nv.y;
+  p.vz[t] = nv.z;
+  const V3<T> mid = p0 + pic * (T(0.5) * dt);
+  const V3<T> vm = velocity_at(g, f, mid);
+  V3<T> st = vm * dt;
+  const T len = len3(st);
+  if (len > g.dx) st = st * (g.dx / len);
+  const V3<T> q = project_inside(sdf, g.dx, p0 + st);
+  const V3<T> r = sdf.sample(q) <= T(0) ? q : p0;
   p.px[t] = r.x;
   p.py[t] = r.y;
   p.pz[t] = r.z;
@@ -1386,16 +1434,18 @@ void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
     if (comp == 1) k_p2g<T, 1, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 2) k_p2g<T, 2, true><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
   } else {
-    // SDR_P2G_PERFACE=1 (read once per process) selects the face-by-face gather
-    static const bool perface = [] {
-      const char* pf = getenv("SDR_P2G_PERFACE");
-      return pf && atoi(pf) != 0;
-    }();
-    if (!perface && launch_p2g_tiled<T>(comp, g, p, cstart, cend, out, st)) return;
     if (comp == 0) k_p2g<T, 0, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 1) k_p2g<T, 1, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
     if (comp == 2) k_p2g<T, 2, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
   }
+}
+bool p2g_perface() {
+  // SDR_P2G_PERFACE=1 (read once per process) selects the face-by-face gather
+  static const bool perface = [] {
+    const char* pf = getenv("SDR_P2G_PERFACE");
+    return pf && atoi(pf) != 0;
+  }();
+  return perface;
 }
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st) {
@@ -1521,6 +1571,13 @@ void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t
   k_scale_cast<T><<<grid_for(n), kBlock, 0, st>>>(n, r, inv, b);
 }
 template <class T>
+void launch_scale_inplace(int64_t n, T* x, double f, hipStream_t st) {
+  k_scale_inplace<T><<<grid_for(n), kBlock, 0, st>>>(n, x, f);
+}
+void launch_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int restart, hipStream_t st) {
+  k_pcg_rescale<<<1, 64, 0, st>>>(sc, f2, tol_eff, restart);
+}
+template <class T>
 void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st) {
   k_accum64<T><<<grid_for(n), kBlock, 0, st>>>(n, p, e, scale);
 }
@@ -1548,6 +1605,12 @@ void launch_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<
                    hipStream_t st) {
   if (n <= 0) return;
   k_advect<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, f, sdf, dt);
+}
+template <class T>
+void launch_g2p_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> saved, T flip,
+                       SdfD<T> sdf, T dt, hipStream_t st) {
+  if (n <= 0) return;
+  k_g2p_advect<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, f, saved, flip, sdf, dt);
 }
 template <class T>
 void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
@@ -1704,6 +1767,7 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_unshear64<T>(Shear, const double*, T*, hipStream_t);                        \
   template void launch_scale_cast<T>(int64_t, const double*, double, T*, hipStream_t);             \
   template void launch_accum64<T>(int64_t, double*, const T*, double, hipStream_t);                \
+  template void launch_scale_inplace<T>(int64_t, T*, double, hipStream_t);                         \
   template void launch_extrap_init<T>(GridD<T>, int, const uint8_t*, uint8_t*, hipStream_t);       \
   template void launch_extrap_iter<T>(int, int, int, const T*, T*, const uint8_t*, uint8_t*,       \
                                       hipStream_t);                                                \
@@ -1711,6 +1775,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
                               hipStream_t);                                                        \
   template void launch_advect<T>(const Parts<T>&, int64_t, GridD<T>, CFacesD<T>, SdfD<T>, T,       \
                                  hipStream_t);                                                     \
+  template void launch_g2p_advect<T>(const Parts<T>&, int64_t, GridD<T>, CFacesD<T>, CFacesD<T>, T, \
+                                     SdfD<T>, T, hipStream_t);                                     \
   template void launch_bounds<T>(const Parts<T>&, int64_t, SdfD<T>, BoxD<T>, uint32_t*, uint32_t*, \
                                  unsigned long long*, hipStream_t);                                \
   template void launch_compact<T>(const Parts<T>&, const Parts<T>&, const uint32_t*,               \

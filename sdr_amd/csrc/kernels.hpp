@@ -63,10 +63,14 @@ void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const ui
 template <class T>
 void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
                 const uint32_t* cend, T* out, bool exact, hipStream_t st);
-// P2G as an LDS-staged gather (p2g.hip); false when the grid is too large for it.
+// All three P2G components as one LDS-staged gather over the sorted particles
+// (p2g.hip), the same sums in the same order as the face-by-face gather (fast mode);
+// false when the grid is too large for it.
 template <class T>
-bool launch_p2g_tiled(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend,
-                      T* out, hipStream_t st);
+bool launch_p2g_fused(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend, FacesD<T> f,
+                      hipStream_t st);
+// SDR_P2G_PERFACE=1: the face-by-face gather kernels instead of the fused one.
+bool p2g_perface();
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st);
 template <class T>
@@ -120,6 +124,9 @@ void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t
 template <class T>
 void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st);
 template <class T>
+void launch_scale_inplace(int64_t n, T* x, double f, hipStream_t st);
+void launch_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int restart, hipStream_t st);
+template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st);
 template <class T>
 void launch_extrap_iter(int nx, int ny, int nz, const T* src, T* dst, const uint8_t* ksrc,
@@ -130,6 +137,10 @@ void launch_g2p(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<
 template <class T>
 void launch_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt,
                    hipStream_t st);
+// g2p and advect in one pass (the substep's path; the stages stay separate kernels)
+template <class T>
+void launch_g2p_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> saved, T flip,
+                       SdfD<T> sdf, T dt, hipStream_t st);
 template <class T>
 void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
                    uint32_t* cull_flag, unsigned long long* counters, hipStream_t st);

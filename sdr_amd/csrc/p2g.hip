@@ -1,21 +1,26 @@
-// p2g.hip — P2G transfer (solver.rs:560-585, grid.rs:52-88, solver.rs:944-952) as an
-// LDS-staged gather over the cell-sorted particles.
+// p2g.hip — P2G transfer (solver.rs:560-585, grid.rs:52-88, solver.rs:944-952) as one
+// LDS-staged gather over the cell-sorted particles for all three face components.
 //
-// A face (fi, fj, fk) of component COMP receives from the particles of the cells
-// [f-1, f] along its own axis and [f-1, f+1] across (clipped to the grid), visited slab
-// (k) by slab, row (j) by row, cell (i) by cell, particle by particle in sorted order;
-// each visit adds w * v and w where w = (wx * wy) * wz from the clamped trilinear base
-// of the particle.  This kernel keeps exactly that order, so it reproduces the
-// face-by-face gather bit for bit, but it stages each particle once per slab instead of
-// once per visit:
-//   * a workgroup owns a 16 x 16 column of faces (fi, fj) over a range of levels fk and
-//     marches up the slabs ck that feed them;
-//   * a slab's particles (18 x 18 cells) are copied into LDS with their base indices and
-//     fractions already computed (the (p - origin) / dx divisions happen once per
-//     particle and slab, not once per visit);
-//   * a particle of slab ck feeds levels bk and bk + 1, both in {ck - 1, ck, ck + 1}, so
-//     each thread keeps three rolling accumulator pairs and finalises a level after its
-//     last slab: out = sum(w v) / sum(w) if sum(w) > 1e-12, else 0.
+// A face (fi, fj, fk) of component c receives from the particles of the cells [f-1, f]
+// along its own axis and [f-1, f+1] across (clipped to the grid), visited slab (k) by
+// slab, row (j) by row, cell (i) by cell, particle by particle in sorted order; each
+// visit adds w * v and w where w = (wx * wy) * wz from the clamped trilinear base of the
+// particle.  This kernel keeps exactly that order, so it reproduces the face-by-face
+// gather bit for bit, and it reads each particle once per slab for all three
+// components instead of once per visit and component:
+//   * a workgroup owns a 16 x 16 column of faces (fi, fj) over 32 levels fk and marches
+//     up the slabs ck that feed them; waves 0-3 gather the u and v faces of the column,
+//     waves 4-7 the w faces (about the same number of candidate particles each);
+//   * a slab's particles (18 x 18 cells) are copied into LDS once (structure of arrays)
+//     with everything a visit needs precomputed: the clamped trilinear fractions of each
+//     axis for the along-axis stencil (l) and the across-axis one (l - 0.5), the three
+//     velocities, and a 16-bit code of the particle's cell in the slab and its half-cell
+//     bits (which side of the cell centre each across-axis base falls on).  The bases of
+//     every stencil follow from that code, so a visit tests the code, then reads the four
+//     values its component needs;
+//   * a particle of slab ck feeds u/v levels bk and bk + 1 with bk in {ck - 1, ck}, and w
+//     levels ck and ck + 1, so each thread keeps rolling accumulator pairs and finalises
+//     a level after its last slab: out = sum(w v) / sum(w) if sum(w) > 1e-12, else 0.
 // Slabs holding more particles than the LDS staging area are processed in passes over
 // consecutive windows of the staged order, which keeps the order.
 #include <hip/hip_runtime.h>
@@ -25,67 +30,71 @@
 namespace sdr {
 namespace {
 
-constexpr int kTX = 16, kTY = 16;         // face columns per workgroup
+constexpr int kTX = 16, kTY = 16;          // face columns per workgroup
 constexpr int kRows = kTY + 2, kCols = kTX + 2;
 constexpr int kSlabCells = kRows * kCols;  // 324
 constexpr int kZChunk = 32;                // face levels per workgroup
+constexpr int kThreads = 512;
 
 template <class T>
-struct Staged {
-  T fx, fy, fz, v;
-};
-template <class T>
-struct StageCap {
-  static constexpr int n = sizeof(T) == 4 ? 3072 : 1536;
+struct StageCap {  // about 70 KB of LDS: two workgroups per CU
+  static constexpr int n = sizeof(T) == 4 ? 1792 : 896;
 };
 
-template <class T, int COMP>
-__global__ void __launch_bounds__(256) k_p2g_tiled(GridD<T> g, Parts<T> p, const uint32_t* __restrict__ cstart,
-                                                   const uint32_t* __restrict__ cend, T* __restrict__ out) {
+// staged code: bits 0-4 the cell's column in the slab, 5-9 its row, then the half-cell
+// bits: bit 10 + a is 1 when the across-axis base of axis a (grid.rs:26-35 on l - 0.5)
+// is the particle's own cell, 0 when it is the cell before
+constexpr int kHx = 10, kHy = 11, kHz = 12;
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_p2g_fused(GridD<T> g, Parts<T> p, const uint32_t* __restrict__ cstart,
+                                                        const uint32_t* __restrict__ cend, T* __restrict__ ou,
+                                                        T* __restrict__ ov, T* __restrict__ ow) {
   constexpr int CAP = StageCap<T>::n;
-  __shared__ Staged<T> sp[CAP];
-  __shared__ uint32_t sbase[CAP];         // bi | bj << 11 | bk << 22
+  // fractions along (a) and across (c) per axis, velocities, cell/half-cell code
+  __shared__ T sfax[CAP], sfay[CAP], sfaz[CAP], sfcx[CAP], sfcy[CAP], sfcz[CAP];
+  __shared__ T svx[CAP], svy[CAP], svz[CAP];
+  __shared__ uint16_t spk[CAP];
   __shared__ uint32_t soff[kSlabCells + 1];  // staged offset of each cell of the slab
   __shared__ uint32_t sglob[kSlabCells];     // its first particle in the sorted arrays
-  __shared__ uint32_t wsum_[4];
+  __shared__ uint32_t wsum_[kThreads / 64];
 
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int fnx = nx + (COMP == 0), fny = ny + (COMP == 1), fnz = nz + (COMP == 2);
-  const int ntx = (fnx + kTX - 1) / kTX, nty = (fny + kTY - 1) / kTY;
+  const int ntx = (nx + 1 + kTX - 1) / kTX, nty = (ny + 1 + kTY - 1) / kTY;
   const int b = blockIdx.x;
   const int x0 = (b % ntx) * kTX, y0 = ((b / ntx) % nty) * kTY;
   const int z0 = (b / (ntx * nty)) * kZChunk;
-  const int z1 = z0 + kZChunk < fnz ? z0 + kZChunk : fnz;
-  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  const int z1 = z0 + kZChunk < nz ? z0 + kZChunk : nz;        // u, v levels [z0, z1)
+  const int z1w = z0 + kZChunk < nz + 1 ? z0 + kZChunk : nz + 1;  // w levels [z0, z1w)
+  const int col = threadIdx.x & 255;
+  const bool wwave = threadIdx.x >= 256;  // waves 4-7: the w faces
+  const int tx = col % kTX, ty = col / kTX;
   const int fi = x0 + tx, fj = y0 + ty;
-  const bool face_ok = fi < fnx && fj < fny;
-  const T* vel = COMP == 0 ? p.vx : (COMP == 1 ? p.vy : p.vz);
+  const bool u_ok = !wwave && fi <= nx && fj < ny;
+  const bool v_ok = !wwave && fi < nx && fj <= ny;
+  const bool w_ok = wwave && fi < nx && fj < ny;
 
   // the slab's cells: rows cj in [rlo, rhi], columns ci in [clo, chi]
   const int rlo = y0 - 1 > 0 ? y0 - 1 : 0, rhi = y0 + kTY < ny - 1 ? y0 + kTY : ny - 1;
   const int clo = x0 - 1 > 0 ? x0 - 1 : 0, chi = x0 + kTX < nx - 1 ? x0 + kTX : nx - 1;
   const int nrows = rhi - rlo + 1, ncols = chi - clo + 1, ncell = nrows * ncols;
-  // this face's candidate cells (the same clipped ranges as the face-by-face gather)
-  const int cilo = (fi - 1 > 0 ? fi - 1 : 0) - clo;
-  const int cihi = ((COMP == 0 ? fi : fi + 1) < nx - 1 ? (COMP == 0 ? fi : fi + 1) : nx - 1) - clo;
-  const int cjlo = (fj - 1 > 0 ? fj - 1 : 0) - rlo;
-  const int cjhi = ((COMP == 1 ? fj : fj + 1) < ny - 1 ? (COMP == 1 ? fj : fj + 1) : ny - 1) - rlo;
 
-  // slabs feeding levels [z0, z1): a level L takes ck in [L-1, L+1] (COMP 2: [L-1, L])
+  // slabs feeding the levels: u/v level L takes ck in [L-1, L+1], w level L [L-1, L]
   const int cs = z0 - 1 > 0 ? z0 - 1 : 0;
-  const int ce_ = (COMP == 2 ? z1 - 1 : z1);
-  const int ce = ce_ < nz - 1 ? ce_ : nz - 1;
+  const int ce = z1 < nz - 1 ? z1 : nz - 1;
 
-  // accumulators of levels ck-1, ck, ck+1 while slab ck is processed
-  T A0 = T(0), A1 = T(0), A2 = T(0), W0 = T(0), W1 = T(0), W2 = T(0);
-  auto finalize = [&](int L, T a, T w) {
-    if (!face_ok || L < z0 || L >= z1) return;
-    out[((int64_t)L * fny + fj) * fnx + fi] = w > T(1e-12) ? a / w : T(0);
+  // accumulators: u/v of levels ck-1, ck, ck+1 while slab ck is processed; w of ck, ck+1
+  T uA0 = T(0), uA1 = T(0), uA2 = T(0), uW0 = T(0), uW1 = T(0), uW2 = T(0);
+  T vA0 = T(0), vA1 = T(0), vA2 = T(0), vW0 = T(0), vW1 = T(0), vW2 = T(0);
+  T wA0 = T(0), wA1 = T(0), wW0 = T(0), wW1 = T(0);
+  auto put = [&](T* out, int fnx, int fny, int L, int lhi, int a, int bb, T acc, T w) {
+    if (L < z0 || L >= lhi) return;
+    out[((int64_t)L * fny + bb) * fnx + a] = w > T(1e-12) ? acc / w : T(0);
   };
 
   for (int ck = cs; ck <= ce; ++ck) {
     // ---- cell table of the slab and its prefix sum (staged offsets)
-    for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ncell; t += kThreads) {
       const int r = t / ncols, c = t % ncols;
       const int64_t cell = ((int64_t)ck * ny + (rlo + r)) * nx + (clo + c);
       const uint32_t s0 = cstart[cell], e0 = cend[cell];
@@ -94,13 +103,10 @@ __global__ void __launch_bounds__(256) k_p2g_tiled(GridD<T> g, Parts<T> p, const
     }
     if (threadIdx.x == 0) soff[0] = 0;
     __syncthreads();
-    // block prefix sum over <= 324 entries: 256 threads, two entries for some
-    {
+    {  // inclusive scan of entries 1..ncell (one per thread), then the wave totals
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      // each thread owns entries 2t+1, 2t+2 (1-based), inclusive scan of pairs
-      const int e1 = 2 * threadIdx.x + 1, e2 = e1 + 1;
-      const uint32_t a = e1 <= ncell ? soff[e1] : 0u, bb = e2 <= ncell ? soff[e2] : 0u;
-      uint32_t s = a + bb;
+      const int e1 = threadIdx.x + 1;
+      uint32_t s = e1 <= ncell ? soff[e1] : 0u;
       for (int o = 1; o < 64; o <<= 1) {
         const uint32_t up = __shfl_up(s, o);
         if (lane >= o) s += up;
@@ -109,17 +115,14 @@ __global__ void __launch_bounds__(256) k_p2g_tiled(GridD<T> g, Parts<T> p, const
       __syncthreads();
       uint32_t base = 0;
       for (int w = 0; w < wave; ++w) base += wsum_[w];
-      const uint32_t incl = base + s;  // sum of entries 1..e2
-      if (e1 <= ncell) soff[e1] = incl - bb;
-      if (e2 <= ncell) soff[e2] = incl;
+      if (e1 <= ncell) soff[e1] = base + s;
       __syncthreads();
     }
     const uint32_t total = soff[ncell];
     // ---- passes over windows of the staged order
     for (uint32_t w0 = 0; w0 < total; w0 += CAP) {
       const uint32_t w1 = w0 + CAP < total ? w0 + CAP : total;
-      // stage particles w0..w1 (row r's particles are contiguous in the sorted arrays)
-      for (uint32_t s = w0 + threadIdx.x; s < w1; s += blockDim.x) {
+      for (uint32_t s = w0 + threadIdx.x; s < w1; s += kThreads) {
         int lo = 0, hi = ncell - 1;  // the cell holding staged index s
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -128,53 +131,146 @@ __global__ void __launch_bounds__(256) k_p2g_tiled(GridD<T> g, Parts<T> p, const
         }
         const uint32_t gidx = sglob[lo] + (s - soff[lo]);
         const V3<T> l = g.local(mk3(p.px[gidx], p.py[gidx], p.pz[gidx]));
-        const T gx = l.x - (COMP == 0 ? T(0) : T(0.5));
-        const T gy = l.y - (COMP == 1 ? T(0) : T(0.5));
-        const T gz = l.z - (COMP == 2 ? T(0) : T(0.5));
-        int bi, bj, bk;
-        T ffx, ffy, ffz;
-        clamp_base(gx, fnx, bi, ffx);
-        clamp_base(gy, fny, bj, ffy);
-        clamp_base(gz, fnz, bk, ffz);
-        sp[s - w0] = Staged<T>{ffx, ffy, ffz, vel[gidx]};
-        sbase[s - w0] = (uint32_t)bi | ((uint32_t)bj << 11) | ((uint32_t)bk << 22);
+        const int ci = clo + lo % ncols, cj = rlo + lo / ncols;
+        int bx, by, bz, ax_, ay_, az_;
+        T cx, cy, cz, fx, fy, fz;
+        clamp_base(l.x, nx + 1, ax_, fx);
+        clamp_base(l.y, ny + 1, ay_, fy);
+        clamp_base(l.z, nz + 1, az_, fz);
+        clamp_base(l.x - T(0.5), nx, bx, cx);
+        clamp_base(l.y - T(0.5), ny, by, cy);
+        clamp_base(l.z - T(0.5), nz, bz, cz);
+        const uint32_t q = s - w0;
+        sfax[q] = fx, sfay[q] = fy, sfaz[q] = fz;
+        sfcx[q] = cx, sfcy[q] = cy, sfcz[q] = cz;
+        svx[q] = p.vx[gidx], svy[q] = p.vy[gidx], svz[q] = p.vz[gidx];
+        spk[q] = (uint16_t)((ci - clo) | ((cj - rlo) << 5) | ((bx == ci) << kHx) | ((by == cj) << kHy) |
+                            ((bz == ck) << kHz));
       }
       __syncthreads();
-      if (face_ok) {
-        for (int r = cjlo; r <= cjhi; ++r) {
-          uint32_t a = soff[r * ncols + cilo], e = soff[r * ncols + cihi + 1];
-          a = a > w0 ? a : w0;
-          e = e < w1 ? e : w1;
-          for (uint32_t s = a; s < e; ++s) {
-            const uint32_t pb = sbase[s - w0];
-            const int di = fi - (int)(pb & 0x7ffu), dj = fj - (int)((pb >> 11) & 0x7ffu);
-            if ((unsigned)di > 1u || (unsigned)dj > 1u) continue;
-            const Staged<T> q = sp[s - w0];
-            const int bk = (int)(pb >> 22);
-            const T wx = di ? q.fx : T(1) - q.fx;
-            const T wy = dj ? q.fy : T(1) - q.fy;
-            const T wxy = wx * wy;
-            const T wlo = wxy * (T(1) - q.fz), whi = wxy * q.fz;
-            const T vlo = wlo * q.v, vhi = whi * q.v;
-            if (bk < ck) {  // levels ck-1, ck
-              A0 += vlo, W0 += wlo, A1 += vhi, W1 += whi;
-            } else {        // levels ck, ck+1
-              A1 += vlo, W1 += wlo, A2 += vhi, W2 += whi;
+      // staged run of the row cj's cells i0..i1 of the slab (consecutive in the sorted
+      // order), clipped to the window
+      auto run = [&](int i0, int i1, int cj, uint32_t& a, uint32_t& e) {
+        const int q = (cj - rlo) * ncols;
+        a = soff[q + (i0 - clo)], e = soff[q + (i1 - clo) + 1];
+        a = a > w0 ? a : w0;
+        e = e < w1 ? e : w1;
+      };
+      // a visit: the particle's along-axis base is its cell, its across-axis base the
+      // cell before or the cell itself (half-cell bit); d = face - base must be 0 or 1.
+      // Runs are walked four particles at a time with every load issued before the
+      // first use; a particle outside the stencil (or past the run) is skipped by
+      // selects, so the sums keep the sequential order.
+      constexpr int UN = 4;
+      if (!wwave) {
+        if (u_ok) {  // along x: cells fi-1, fi; across y, z
+          const int i0 = fi - 1 > 0 ? fi - 1 : 0, i1 = fi < nx - 1 ? fi : nx - 1;
+          const int j0 = fj - 1 > 0 ? fj - 1 : 0, j1 = fj + 1 < ny - 1 ? fj + 1 : ny - 1;
+          for (int cj = j0; cj <= j1; ++cj) {
+            uint32_t a, e;
+            run(i0, i1, cj, a, e);
+            for (uint32_t s = a; s < e; s += UN) {
+              int pk[UN];
+              T fx[UN], fy[UN], fz[UN], v[UN];
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const uint32_t q = s + u < e ? s + u - w0 : 0u;
+                pk[u] = s + u < e ? (int)spk[q] : 0x7fff;
+                fx[u] = sfax[q], fy[u] = sfcy[q], fz[u] = sfcz[q], v[u] = svx[q];
+              }
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const int di = fi - (clo + (pk[u] & 31));
+                const int dj = fj - (cj - 1 + ((pk[u] >> kHy) & 1));
+                const bool ok = (unsigned)di <= 1u && (unsigned)dj <= 1u && pk[u] != 0x7fff;
+                const bool lo = ((pk[u] >> kHz) & 1) == 0;  // base ck - 1: levels ck - 1, ck
+                const T wxy = (di ? fx[u] : T(1) - fx[u]) * (dj ? fy[u] : T(1) - fy[u]);
+                const T wlo = wxy * (T(1) - fz[u]), whi = wxy * fz[u];
+                const T vlo = wlo * v[u], vhi = whi * v[u];
+                uA0 = ok && lo ? uA0 + vlo : uA0, uW0 = ok && lo ? uW0 + wlo : uW0;
+                uA1 = ok ? uA1 + (lo ? vhi : vlo) : uA1, uW1 = ok ? uW1 + (lo ? whi : wlo) : uW1;
+                uA2 = ok && !lo ? uA2 + vhi : uA2, uW2 = ok && !lo ? uW2 + whi : uW2;
+              }
+            }
+          }
+        }
+        if (v_ok) {  // along y: cells fj-1, fj; across x, z
+          const int i0 = fi - 1 > 0 ? fi - 1 : 0, i1 = fi + 1 < nx - 1 ? fi + 1 : nx - 1;
+          const int j0 = fj - 1 > 0 ? fj - 1 : 0, j1 = fj < ny - 1 ? fj : ny - 1;
+          for (int cj = j0; cj <= j1; ++cj) {
+            uint32_t a, e;
+            run(i0, i1, cj, a, e);
+            const int dj = fj - cj;
+            for (uint32_t s = a; s < e; s += UN) {
+              int pk[UN];
+              T fx[UN], fy[UN], fz[UN], v[UN];
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const uint32_t q = s + u < e ? s + u - w0 : 0u;
+                pk[u] = s + u < e ? (int)spk[q] : 0x7fff;
+                fx[u] = sfcx[q], fy[u] = sfay[q], fz[u] = sfcz[q], v[u] = svy[q];
+              }
+#pragma unroll
+              for (int u = 0; u < UN; ++u) {
+                const int di = fi - (clo + (pk[u] & 31) - 1 + ((pk[u] >> kHx) & 1));
+                const bool ok = (unsigned)di <= 1u && pk[u] != 0x7fff;
+                const bool lo = ((pk[u] >> kHz) & 1) == 0;
+                const T wxy = (di ? fx[u] : T(1) - fx[u]) * (dj ? fy[u] : T(1) - fy[u]);
+                const T wlo = wxy * (T(1) - fz[u]), whi = wxy * fz[u];
+                const T vlo = wlo * v[u], vhi = whi * v[u];
+                vA0 = ok && lo ? vA0 + vlo : vA0, vW0 = ok && lo ? vW0 + wlo : vW0;
+                vA1 = ok ? vA1 + (lo ? vhi : vlo) : vA1, vW1 = ok ? vW1 + (lo ? whi : wlo) : vW1;
+                vA2 = ok && !lo ? vA2 + vhi : vA2, vW2 = ok && !lo ? vW2 + whi : vW2;
+              }
+            }
+          }
+        }
+      } else if (w_ok) {  // along z (levels ck, ck+1); across x, y
+        const int i0 = fi - 1 > 0 ? fi - 1 : 0, i1 = fi + 1 < nx - 1 ? fi + 1 : nx - 1;
+        const int j0 = fj - 1 > 0 ? fj - 1 : 0, j1 = fj + 1 < ny - 1 ? fj + 1 : ny - 1;
+        for (int cj = j0; cj <= j1; ++cj) {
+          uint32_t a, e;
+          run(i0, i1, cj, a, e);
+          for (uint32_t s = a; s < e; s += UN) {
+            int pk[UN];
+            T fx[UN], fy[UN], fz[UN], v[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+              const uint32_t q = s + u < e ? s + u - w0 : 0u;
+              pk[u] = s + u < e ? (int)spk[q] : 0x7fff;
+              fx[u] = sfcx[q], fy[u] = sfcy[q], fz[u] = sfaz[q], v[u] = svz[q];
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+              const int di = fi - (clo + (pk[u] & 31) - 1 + ((pk[u] >> kHx) & 1));
+              const int dj = fj - (cj - 1 + ((pk[u] >> kHy) & 1));
+              const bool ok = (unsigned)di <= 1u && (unsigned)dj <= 1u && pk[u] != 0x7fff;
+              const T wxy = (di ? fx[u] : T(1) - fx[u]) * (dj ? fy[u] : T(1) - fy[u]);
+              const T wlo = wxy * (T(1) - fz[u]), whi = wxy * fz[u];
+              wA0 = ok ? wA0 + wlo * v[u] : wA0, wW0 = ok ? wW0 + wlo : wW0;
+              wA1 = ok ? wA1 + whi * v[u] : wA1, wW1 = ok ? wW1 + whi : wW1;
             }
           }
         }
       }
       __syncthreads();
     }
-    // ---- finalise the levels whose last slab this was, then roll the window
-    if (COMP == 2) {
-      finalize(ck, A1, W1);
-      if (ck == nz - 1) finalize(ck + 1, A2, W2);
-    } else {
-      finalize(ck - 1, A0, W0);
-      if (ck == nz - 1) finalize(ck, A1, W1);
+    // ---- finalise the levels whose last slab this was, then roll the windows
+    if (u_ok) {
+      put(ou, nx + 1, ny, ck - 1, z1, fi, fj, uA0, uW0);
+      if (ck == nz - 1) put(ou, nx + 1, ny, ck, z1, fi, fj, uA1, uW1);
     }
-    A0 = A1, W0 = W1, A1 = A2, W1 = W2, A2 = T(0), W2 = T(0);
+    if (v_ok) {
+      put(ov, nx, ny + 1, ck - 1, z1, fi, fj, vA0, vW0);
+      if (ck == nz - 1) put(ov, nx, ny + 1, ck, z1, fi, fj, vA1, vW1);
+    }
+    if (w_ok) {
+      put(ow, nx, ny, ck, z1w, fi, fj, wA0, wW0);
+      if (ck == nz - 1) put(ow, nx, ny, ck + 1, z1w, fi, fj, wA1, wW1);
+    }
+    uA0 = uA1, uW0 = uW1, uA1 = uA2, uW1 = uW2, uA2 = T(0), uW2 = T(0);
+    vA0 = vA1, vW0 = vW1, vA1 = vA2, vW1 = vW2, vA2 = T(0), vW2 = T(0);
+    wA0 = wA1, wW0 = wW1, wA1 = T(0), wW1 = T(0);
     __syncthreads();  // the slab's tables are rewritten by the next slab
   }
 }
@@ -182,22 +278,18 @@ __global__ void __launch_bounds__(256) k_p2g_tiled(GridD<T> g, Parts<T> p, const
 }  // namespace
 
 template <class T>
-bool launch_p2g_tiled(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend,
-                      T* out, hipStream_t st) {
-  // base indices are packed in 11 bits per axis
-  if (g.nx + 1 >= 2048 || g.ny + 1 >= 2048 || g.nz + 1 >= 1024) return false;
-  const int fnx = g.nx + (comp == 0), fny = g.ny + (comp == 1), fnz = g.nz + (comp == 2);
-  const int64_t blocks = (int64_t)((fnx + kTX - 1) / kTX) * ((fny + kTY - 1) / kTY) *
-                         ((fnz + kZChunk - 1) / kZChunk);
-  if (comp == 0) k_p2g_tiled<T, 0><<<(unsigned)blocks, 256, 0, st>>>(g, p, cstart, cend, out);
-  if (comp == 1) k_p2g_tiled<T, 1><<<(unsigned)blocks, 256, 0, st>>>(g, p, cstart, cend, out);
-  if (comp == 2) k_p2g_tiled<T, 2><<<(unsigned)blocks, 256, 0, st>>>(g, p, cstart, cend, out);
+bool launch_p2g_fused(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend, FacesD<T> f,
+                      hipStream_t st) {
+  const int64_t blocks = (int64_t)((g.nx + 1 + kTX - 1) / kTX) * ((g.ny + 1 + kTY - 1) / kTY) *
+                         ((g.nz + 1 + kZChunk - 1) / kZChunk);
+  if (blocks >= (int64_t)1 << 31) return false;
+  k_p2g_fused<T><<<(unsigned)blocks, kThreads, 0, st>>>(g, p, cstart, cend, f.u, f.v, f.w);
   return true;
 }
 
-template bool launch_p2g_tiled<float>(int, GridD<float>, const Parts<float>&, const uint32_t*, const uint32_t*,
-                                      float*, hipStream_t);
-template bool launch_p2g_tiled<double>(int, GridD<double>, const Parts<double>&, const uint32_t*,
-                                       const uint32_t*, double*, hipStream_t);
+template bool launch_p2g_fused<float>(GridD<float>, const Parts<float>&, const uint32_t*, const uint32_t*,
+                                      FacesD<float>, hipStream_t);
+template bool launch_p2g_fused<double>(GridD<double>, const Parts<double>&, const uint32_t*, const uint32_t*,
+                                       FacesD<double>, hipStream_t);
 
 }  // namespace sdr

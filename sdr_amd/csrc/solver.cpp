@@ -219,6 +219,11 @@ struct Projector {
   int mic_variant = 0;   // pipelined sweep tile shape (mic_pipe_shape); -1 = plane-by-plane
   int ncu = 0, dev = 0;  // compute units of the solver's device
   bool trace_refine = false;  // SDR_TRACE_REFINE=1: one stderr line per refinement pass
+  // The precision form's solve: reliable updates (default), or restarted refinement
+  // passes (SDR_REFINE_RESTART=1); kReplace is the drop of the recursive residual that
+  // triggers a replacement (SDR_REPLACE_DROP overrides it, read once per solver).
+  bool restart_refine = false;
+  double kReplace = 1e-1;
   // All working memory is a view of caller-provided memory: nothing is allocated, and
   // memory is cleared by a fill kernel instead of a memset (graph-capturable).
   bool noalloc = false;
@@ -244,6 +249,8 @@ struct Projector {
     if (const char* e = getenv("SDR_MIC_VARIANT")) mic_variant = atoi(e);
     if (mic_variant >= mic_pipe_variants()) mic_variant = kMicDefaultVariant;
     if (const char* e = getenv("SDR_TRACE_REFINE")) trace_refine = atoi(e) != 0;
+    if (const char* e = getenv("SDR_REFINE_RESTART")) restart_refine = atoi(e) != 0;
+    if (const char* e = getenv("SDR_REPLACE_DROP")) kReplace = atof(e);
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -554,6 +561,172 @@ struct Projector {
     return tot;
   }
 
+  // Mixed-precision PCG with reliable updates (residual replacement; van der Vorst & Ye,
+  // "Residual replacement strategies for Krylov subspace iterative methods", 2000): one
+  // f32 Krylov run on the residual scaled by `scale` (its max norm at the last
+  // replacement), with dots and norms in f64.  The f32 iterate is a correction that is
+  // folded into the f64 solution p64 whenever the recursive residual has dropped by
+  // kReplace (or reached the stopping bound); the f64 true residual r = b - A p64 then
+  // replaces the recursive one, and the search direction and sigma are carried into the
+  // new units (s * f, sigma * f^2 with f = old scale / new scale), so the Krylov space
+  // built so far is kept instead of restarted.  The run stops when the true residual
+  // reaches rel_tol * max|b|.  A replacement that has not halved the true residual since
+  // the previous one restarts the space (s = z).  Returns the total PCG iterations; the
+  // f64 pressure is left in p64.
+  PcgScalars reliable(uint64_t max_iters, double rel_tol, double* final_res) {
+    const size_t n = (size_t)sh.size();
+    const int mb = maxabs_blocks(sh);
+    double* hn = (double*)host_scal.p;
+    auto true_residual = [&]() {
+      launch_residual64(sh, flags.as<uint16_t>(), b64.as<double>(), p64.as<double>(), r64.as<double>(),
+                        partials.as<double>(), mb, norms.as<double>(), st);
+      HIPCHK(hipMemcpyAsync(hn, norms.p, 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return hn[0];
+    };
+    HIPCHK(hipMemsetAsync(p64.p, 0, n * 8, st));
+    const double bnorm = true_residual();  // p = 0: r = b
+    PcgScalars tot{};
+    tot.rhs_norm = bnorm;
+    if (!(bnorm == bnorm)) {
+      tot.nan_seen = 1;
+      return tot;
+    }
+    if (!(bnorm > 0.0)) {  // zero right-hand side: zero pressure, no iterations
+      if (final_res) *final_res = 0.0;
+      return tot;
+    }
+    const double target = rel_tol * bnorm;
+    double scale = bnorm, rnorm = bnorm, last = bnorm;
+    T* r = b.as<T>();
+    {
+      Scope sc_(*timer, "refine");
+      launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / scale, r, st);
+    }
+    {
+      Scope sc_(*timer, "pcg_setup");
+      for (Buf* bb : {&z, &s, &p, &as, &q, &precon}) zero(*bb, n * sizeof(T));
+      zero(scal, sizeof(PcgScalars));
+      launch_maxabs_partials<T>(sh, r, partials.as<double>(), mb, sc(), st);
+      launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), target / scale, st, 0);
+    }
+    {
+      Scope sc_(*timer, "mic_build");
+      mic_sweep(kMicBuild, nullptr, nullptr);
+    }
+    {
+      Scope sc_(*timer, "mic_fwd");
+      mic_sweep(kMicFwd, r, q.as<T>());
+    }
+    {
+      Scope sc_(*timer, "mic_bwd");
+      mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
+    }
+    HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    dot(z.as<T>(), r, kOpSigma0, 0.0);
+    const bool fz = fused();
+    const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
+    const uint64_t check_every = 8;
+    uint64_t it = 0;
+    int replacements = 0, restarts = 0;
+    bool converged = false;
+    while (it < max_iters) {
+      {
+        Scope sc1(*timer, "spmv_dot");
+        if (fz) {
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(), st);
+          launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), 0.0, st);
+        } else {
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, nullptr, st);
+          dot(s.as<T>(), as.as<T>(), kOpDenom, 0.0);
+        }
+      }
+      {
+        Scope sc1(*timer, "axpy_norm");
+        launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+        launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), 0.0, st);
+      }
+      ++it;
+      if (it % check_every == 0 || it == max_iters) {
+        const PcgScalars sc_h = read_scalars();
+        if (sc_h.nan_seen) {
+          tot.nan_seen = 1;
+          break;
+        }
+        it = (uint64_t)sc_h.iters;  // iterations skipped after the loop stopped did nothing
+        const bool reached = sc_h.residual <= sc_h.tol_eff;
+        if (sc_h.done || sc_h.residual <= kReplace) {
+          Scope sc1(*timer, "refine");
+          launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
+          zero(p, n * sizeof(T));
+          const double rn = true_residual();
+          ++replacements;
+          if (trace_refine)
+            fprintf(stderr, "[reliable] it %llu: recursive %.3e (bound %.3e) -> true %.3e of %.3e (%.3e)%s\n",
+                    (unsigned long long)it, sc_h.residual, sc_h.tol_eff, rn, bnorm, rn / bnorm,
+                    sc_h.done && !reached ? " breakdown" : "");
+          if (!(rn == rn)) {
+            tot.nan_seen = 1;
+            break;
+          }
+          rnorm = rn;
+          if (rn <= target) {
+            converged = true;
+            break;
+          }
+          if (it >= max_iters) break;
+          // no progress since the last replacement, or a breakdown: restart the space
+          const bool restart = rn > 0.5 * last || (sc_h.done && !reached);
+          last = rn;
+          const double f = scale / rn;
+          scale = rn;
+          launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / scale, r, st);
+          if (restart) {
+            zero(s, n * sizeof(T));
+            ++restarts;
+          } else {
+            launch_scale_inplace<T>((int64_t)n, s.as<T>(), f, st);
+          }
+          launch_pcg_rescale(sc(), f * f, target / scale, restart ? 1 : 0, st);
+        } else if (sc_h.done) {
+          break;
+        }
+      }
+      {
+        Scope sc1(*timer, "mic_fwd");
+        mic_sweep(kMicFwd, r, q.as<T>());
+      }
+      if (fz) {
+        Scope sc1(*timer, "mic_bwd_dot");
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
+        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+      } else {
+        {
+          Scope sc1(*timer, "mic_bwd");
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
+        }
+        dot(z.as<T>(), r, kOpSigmaNew, 0.0);
+      }
+      {
+        Scope sc1(*timer, "update_s");
+        launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
+      }
+    }
+    if (!converged && !tot.nan_seen) {  // the cap hit: fold in what the f32 run has
+      launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
+      zero(p, n * sizeof(T));
+      rnorm = true_residual();
+    }
+    if (mic_variant >= 0) check_pipe();
+    if (trace_refine)
+      fprintf(stderr, "[reliable] %llu iterations, %d replacements, %d restarts, residual %.3e of %.3e\n",
+              (unsigned long long)it, replacements, restarts, rnorm, bnorm);
+    tot.iters = (int32_t)it;
+    tot.residual = rnorm;
+    if (final_res) *final_res = rnorm / bnorm;
+    return tot;
+  }
+
   // Steps 1-5 of project_capturing_with_source on the device faces (MAC layout).
   // relative: stop at rel_tol * max|b| (f32 storage: by f64 iterative refinement).
   sdr_solve_report project(GridD<T> g, FacesD<T> f, const uint8_t* cells, const uint32_t* count,
@@ -577,7 +750,7 @@ struct Projector {
     }
     PcgScalars r;
     if (ref) {
-      r = refine(max_iters, rel_tol, nullptr);
+      r = restart_refine ? refine(max_iters, rel_tol, nullptr) : reliable(max_iters, rel_tol, nullptr);
     } else {
       r = pcg(max_iters, relative ? rel_tol : tol, relative);
       if (mic_variant >= 0) check_pipe();
@@ -872,6 +1045,9 @@ struct Solver : SolverBase {
   void p2g() {
     ensure_counts();
     Scope sc_(timer, "p2g");
+    if (!exact() && !p2g_perface() &&
+        launch_p2g_fused<T>(grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), faces(), st))
+      return;
     launch_p2g<T>(0, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fu.as<T>(), exact(), st);
     launch_p2g<T>(1, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fv.as<T>(), exact(), st);
     launch_p2g<T>(2, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fw.as<T>(), exact(), st);
@@ -1108,8 +1284,23 @@ struct Solver : SolverBase {
       default: throw Error(SDR_E_INVALID, "unknown stage");
     }
   }
+  // g2p then advect as one pass over the particles (the same values as the two stages)
+  void g2p_advect(double dt) {
+    if (!have_saved) throw Error(SDR_E_STATE, "g2p needs the saved post-P2G grid");
+    Scope sc_(timer, "g2p_advect");
+    double fr = fluid.flip_ratio < 0.0 ? 0.0 : (fluid.flip_ratio > 1.0 ? 1.0 : fluid.flip_ratio);
+    launch_g2p_advect<T>(parts(cur), n, grid(), cfaces(), csaved(), (T)fr, sdf(), (T)dt, st);
+    sorted_valid = false;
+  }
   void substep(double dt) {  // solver.rs:284-310
-    for (int s = 0; s < SDR_STAGE_COUNT; ++s) run_stage(s, dt);
+    for (int s = 0; s < SDR_STAGE_COUNT; ++s) {
+      if (s == SDR_STAGE_G2P) {
+        g2p_advect(dt);
+        ++s;  // SDR_STAGE_ADVECT done too
+        continue;
+      }
+      run_stage(s, dt);
+    }
     timer.collect();
   }
 
