@@ -284,6 +284,45 @@ def pmc_traffic(kernel_region: str):
             "write_bytes": r["write_bytes"], "source": "profiles/pmc_traffic.json"}
 
 
+def mesh_sdf_run(subdivisions: int = 6, n: int = 96, cpu_nodes: int = 2048) -> dict:
+    """The mesh -> SDF build (`Sdf::from_mesh`, sdf.rs:47-86) of a closed icosphere of
+    20 * 4^s triangles on an n^3 node grid, on the device, with the CPU oracle timed on
+    `cpu_nodes` nodes of the same grid (the build is O(nodes x triangles)).  FP64-bound:
+    about 120 flops per node-triangle pair (point-triangle distance and solid angle)."""
+    import oracle
+    from sdr_amd import scenes
+    from sdr_amd.solver import Aabb, Sdf
+
+    v, f = scenes.icosphere(1.0, subdivisions)
+    dx = 2.6 / (n - 1)
+    box = Aabb((-1.3, -1.3, -1.3), (-1.3 + (n - 1) * dx,) * 3)
+    Sdf.from_mesh(v, f, dx, bounds=box)  # warm-up (code object load)
+    t = time.perf_counter()
+    s = Sdf.from_mesh(v, f, dx, bounds=box)
+    gpu_s = time.perf_counter() - t
+    nn = s.nx * s.ny * s.nz
+    pairs = nn * f.shape[0]
+    # the oracle on a row of the same grid
+    lib = oracle.load()
+    row = Aabb((-1.3, 0.0, 0.0), (-1.3 + (cpu_nodes - 1) * (2.6 / (cpu_nodes - 1)), 0.0, 0.0))
+    t = time.perf_counter()
+    Sdf.from_mesh(v, f, 2.6 / (cpu_nodes - 1), bounds=row, lib=lib)
+    cpu_s = time.perf_counter() - t
+    flops = 120.0 * pairs
+    out = {"triangles": int(f.shape[0]), "nodes": int(nn), "gpu_s": round(gpu_s, 4),
+           "node_triangle_pairs_per_s": round(pairs / gpu_s, 1),
+           "roofline": {"bound": "fp64", "achieved": round(flops / gpu_s / 1e12, 3), "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": round(flops / gpu_s / 1e12 / 78.6, 4),
+                        "flops_per_pair": 120},
+           "cpu_oracle_pairs_per_s": round(cpu_nodes * f.shape[0] / cpu_s, 1),
+           "cpu_threads": oracle.get_threads(),
+           "speedup_vs_cpu": round((pairs / gpu_s) / (cpu_nodes * f.shape[0] / cpu_s), 1),
+           "workload": f"icosphere of {f.shape[0]} triangles, {s.nx}^3 nodes (host-timed call incl. "
+                       f"copies); oracle on {cpu_nodes} nodes"}
+    print(f"[bench] mesh_sdf: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def serving_run(count: int, slots: int) -> dict:
     """The serving form: `count` irrigation scenarios (seeded Poisson arrivals at 8/s,
     grids 32^3-128^3, needle angle +-30 deg, 1-6 ml/s, 0.5-2 ml) through the native
@@ -324,6 +363,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the mesh -> SDF build")
     ap.add_argument("--serve-count", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
@@ -411,6 +451,12 @@ def main() -> int:
             serve = serving_run(args.serve_count, args.serve_slots)
         except Exception as e:  # the throughput number stands on its own
             serve = {"error": repr(e)}
+    mesh = None
+    if not args.no_mesh:
+        try:
+            mesh = mesh_sdf_run()
+        except Exception as e:  # the throughput number stands on its own
+            mesh = {"error": repr(e)}
     prec_txt = ("f32 storage + f64 dots/norms/refinement" if prec == A.PRECISION_F32 else "f64")
     out = {
         "metric": METRIC,
@@ -444,6 +490,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serve,
+        "mesh_sdf": mesh,
     }
     if cpu and "value" in cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

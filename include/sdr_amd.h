@@ -241,6 +241,19 @@ int sdr_solver_diagnostics(sdr_solver* s, sdr_diagnostics* out);
  * take next before its floor and the frame remainder apply. */
 int sdr_solver_cfl_dt(sdr_solver* s, double* dt);
 
+/* `Sdf::from_mesh(mesh, dx, padding)` and `Sdf::from_mesh_in_bounds(mesh, dx, bounds)`
+ * (sdf.rs:47-86) with `signed_distance_to_mesh` (sdf.rs:187-272): the signed distance of
+ * every node of a grid to a closed triangle mesh, negative inside (generalised winding
+ * number > 1/2), built on device `device` (one thread per node over all triangles).
+ * vertices: nv xyz triples; tris: nt vertex-index triples (`TriMesh`, mesh.rs:16-45);
+ * bounds: NULL = the vertex bounds padded by `padding` (mesh.rs:47-49, math.rs:241-246).
+ * Fills out's origin, dx and dims (nx = ceil(size / dx) + 1, ...) and, when data is not
+ * NULL, writes the nx*ny*nz node values there (x fastest; cap = values data holds) and
+ * sets out->data = data.  Call with data NULL to learn the dims. */
+int sdr_sdf_from_mesh(const double* vertices, uint64_t nv, const uint32_t* tris, uint64_t nt, double dx,
+                      const sdr_aabb* bounds, double padding, int32_t device, sdr_sdf* out, double* data,
+                      uint64_t cap);
+
 /* ---------------------------------------------------------------------------------
  * Clinical metrics (`MetricsCollector`, metrics.rs:116-446) on the device.  The
  * collector marks wall cells (cavity cells with a non-cavity or out-of-domain face

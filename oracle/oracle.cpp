@@ -1425,4 +1425,100 @@ int orc_metrics_get_cells(const void* m, uint8_t* washed, uint8_t* is_wall) {
   return 0;
 }
 
+// ---- Sdf::from_mesh / from_mesh_in_bounds (sdf.rs:47-86), signed_distance_to_mesh,
+// point_triangle_distance_sq, solid_angle (sdf.rs:187-272)
+namespace {
+V3 vmin(V3 a, V3 b) { return v3(rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)); }  // math.rs min
+V3 vmax(V3 a, V3 b) { return v3(rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z)); }
+double length_squared(V3 a) { return dot(a, a); }
+double point_triangle_distance_sq(V3 p, V3 a, V3 b, V3 c) {  // sdf.rs:201-249
+  V3 ab = b - a, ac = c - a, ap = p - a;
+  double d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return length_squared(ap);
+  V3 bp = p - b;
+  double d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return length_squared(bp);
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    double v = d1 / (d1 - d3);
+    return length_squared(p - (a + ab * v));
+  }
+  V3 cp = p - c;
+  double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return length_squared(cp);
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    double w = d2 / (d2 - d6);
+    return length_squared(p - (a + ac * w));
+  }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return length_squared(p - (b + (c - b) * w));
+  }
+  double denom = 1.0 / (va + vb + vc);
+  double v = vb * denom, w = vc * denom;
+  return length_squared(p - (a + ab * v + ac * w));
+}
+double solid_angle(V3 p, V3 a, V3 b, V3 c) {  // sdf.rs:254-272
+  V3 av = a - p, bv = b - p, cv = c - p;
+  double la = length(av), lb = length(bv), lc = length(cv);
+  if (la < 1e-12 || lb < 1e-12 || lc < 1e-12) return 0.0;
+  double numerator = dot(av, cross(bv, cv));
+  double denominator = la * lb * lc + dot(av, bv) * lc + dot(bv, cv) * la + dot(cv, av) * lb;
+  return 2.0 * std::atan2(numerator, denominator);
+}
+}  // namespace
+
+int orc_sdf_from_mesh(const double* verts, uint64_t nv, const uint32_t* tris, uint64_t nt, double dx,
+                      const sdr_aabb* bounds, double padding, int32_t /*device*/, sdr_sdf* out, double* data,
+                      uint64_t cap) {
+  if (!verts || !tris || !out || !(dx > 0.0)) return fail(SDR_E_INVALID, "invalid argument");
+  for (uint64_t t = 0; t < 3 * nt; ++t)
+    if (tris[t] >= nv) return fail(SDR_E_INVALID, "triangle index out of range");
+  Aabb bb;
+  if (bounds) {
+    bb = Aabb{v3(bounds->min.x, bounds->min.y, bounds->min.z), v3(bounds->max.x, bounds->max.y, bounds->max.z)};
+  } else {  // mesh.rs:47-49 (Aabb::from_points), then padded (math.rs:241-246)
+    const double inf = std::numeric_limits<double>::infinity();
+    bb = Aabb{v3(inf, inf, inf), v3(-inf, -inf, -inf)};
+    for (uint64_t q = 0; q < nv; ++q) {
+      V3 p = v3(verts[3 * q], verts[3 * q + 1], verts[3 * q + 2]);
+      bb.min = vmin(bb.min, p), bb.max = vmax(bb.max, p);
+    }
+    bb.min = bb.min - v3(padding, padding, padding);
+    bb.max = bb.max + v3(padding, padding, padding);
+  }
+  V3 size = bb.max - bb.min;
+  uint64_t nx = as_usize(std::ceil(size.x / dx)) + 1, ny = as_usize(std::ceil(size.y / dx)) + 1,
+           nz = as_usize(std::ceil(size.z / dx)) + 1;
+  out->origin = sdr_vec3{bb.min.x, bb.min.y, bb.min.z};
+  out->dx = dx;
+  out->nx = nx, out->ny = ny, out->nz = nz;
+  out->data = data;
+  if (!data) return 0;
+  if (cap < nx * ny * nz) return fail(SDR_E_INVALID, "output holds fewer values than the node grid");
+  std::vector<V3> tv(3 * nt);
+  for (uint64_t t = 0; t < 3 * nt; ++t) {
+    const double* q = verts + 3 * (uint64_t)tris[t];
+    tv[t] = v3(q[0], q[1], q[2]);
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)nz; ++k)
+    for (uint64_t j = 0; j < ny; ++j)
+      for (uint64_t i = 0; i < nx; ++i) {
+        V3 p = bb.min + v3((double)i * dx, (double)j * dx, (double)k * dx);
+        double min_d2 = std::numeric_limits<double>::infinity(), winding = 0.0;
+        for (uint64_t t = 0; t < nt; ++t) {  // signed_distance_to_mesh (sdf.rs:187-198)
+          double d2 = point_triangle_distance_sq(p, tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]);
+          if (d2 < min_d2) min_d2 = d2;
+          winding += solid_angle(p, tv[3 * t], tv[3 * t + 1], tv[3 * t + 2]);
+        }
+        winding /= 4.0 * M_PI;
+        double dist = std::sqrt(min_d2);
+        data[((uint64_t)k * ny + j) * nx + i] = winding > 0.5 ? -dist : dist;
+      }
+  return 0;
+}
+
 }  // extern "C"

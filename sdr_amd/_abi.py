@@ -181,6 +181,8 @@ SIGNATURES = {
     "metrics_record": [P, P, C.c_uint64, C.POINTER(FrameMetrics)],
     "metrics_summary": [P, C.POINTER(Summary)],
     "metrics_get_cells": [P, c_u8_p, c_u8_p],
+    "sdf_from_mesh": [c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.c_double, C.POINTER(Aabb),
+                      C.c_double, C.c_int32, C.POINTER(SdfDesc), c_double_p, C.c_uint64],
 }
 # Entry points only the device library has.
 DEVICE_ONLY = {

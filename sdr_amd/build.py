@@ -28,7 +28,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["kernels.hip", "mic_pipe.hip", "redist.hip", "p2g.hip", "sort.hip", "metrics.hip", "solver.cpp",
+SOURCES = ["kernels.hip", "mic_pipe.hip", "redist.hip", "p2g.hip", "sort.hip", "metrics.hip", "mesh_sdf.hip",
+           "solver.cpp",
            "serve.cpp"]
 
 

@@ -26,6 +26,7 @@
 
 #include "../../include/sdr_amd.h"
 #include "kernels.hpp"
+#include "mesh_sdf.hpp"
 #include "metrics.hpp"
 #include "mic_pipe.hpp"
 #include "sort.hpp"
@@ -1920,6 +1921,63 @@ int sdr_project_device_noalloc(const sdr_dims* gd, double* u, double* v, double*
     HIPCHK(hipStreamSynchronize(st));
     if (nanf) throw Error(SDR_E_NUMERIC, "pressure solve: NaN in the residual (non-finite faces or source?)");
     if (report) *report = r;
+  });
+}
+
+int sdr_sdf_from_mesh(const double* verts, uint64_t nv, const uint32_t* tris, uint64_t nt, double dx,
+                      const sdr_aabb* bounds, double padding, int32_t device, sdr_sdf* out, double* data,
+                      uint64_t cap) {
+  return guard([&] {
+    if (!verts || !tris || !out || !(dx > 0.0)) throw Error(SDR_E_INVALID, "invalid argument");
+    for (uint64_t t = 0; t < 3 * nt; ++t)
+      if (tris[t] >= nv) throw Error(SDR_E_INVALID, "triangle index out of range");
+    double lo[3], hi[3];
+    if (bounds) {
+      lo[0] = bounds->min.x, lo[1] = bounds->min.y, lo[2] = bounds->min.z;
+      hi[0] = bounds->max.x, hi[1] = bounds->max.y, hi[2] = bounds->max.z;
+    } else {  // mesh.rs:47-49 (Aabb::from_points), then padded (math.rs:241-246)
+      for (int a = 0; a < 3; ++a) lo[a] = INFINITY, hi[a] = -INFINITY;
+      for (uint64_t q = 0; q < nv; ++q)
+        for (int a = 0; a < 3; ++a) lo[a] = std::fmin(lo[a], verts[3 * q + a]), hi[a] = std::fmax(hi[a], verts[3 * q + a]);
+      for (int a = 0; a < 3; ++a) lo[a] = lo[a] - padding, hi[a] = hi[a] + padding;
+    }
+    uint64_t dims[3];
+    for (int a = 0; a < 3; ++a) {  // (size / dx).ceil() as usize + 1 (sdf.rs:54-57)
+      const double c = std::ceil((hi[a] - lo[a]) / dx);
+      dims[a] = (c > 0.0 ? (uint64_t)c : 0) + 1;
+      if (dims[a] > (1u << 16)) throw Error(SDR_E_INVALID, "node grid too large");
+    }
+    out->origin = sdr_vec3{lo[0], lo[1], lo[2]};
+    out->dx = dx;
+    out->nx = dims[0], out->ny = dims[1], out->nz = dims[2];
+    out->data = data;
+    if (!data) return;
+    const uint64_t nn = dims[0] * dims[1] * dims[2];
+    if (cap < nn) throw Error(SDR_E_INVALID, "output holds fewer values than the node grid");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
+    HIPCHK(hipSetDevice(device));
+    std::vector<sdr::MeshTri> tv((size_t)nt);
+    for (uint64_t t = 0; t < nt; ++t) {
+      const double* a = verts + 3 * (uint64_t)tris[3 * t];
+      const double* b = verts + 3 * (uint64_t)tris[3 * t + 1];
+      const double* c = verts + 3 * (uint64_t)tris[3 * t + 2];
+      tv[(size_t)t] = sdr::MeshTri{a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+    }
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    sdr::Buf dt, dout;
+    dt.alloc(tv.size() * sizeof(sdr::MeshTri));
+    dout.alloc(nn * sizeof(double));
+    if (nt) HIPCHK(hipMemcpyAsync(dt.p, tv.data(), tv.size() * sizeof(sdr::MeshTri), hipMemcpyHostToDevice, st));
+    sdr::launch_mesh_sdf(dt.as<sdr::MeshTri>(), (int64_t)nt, (int)dims[0], (int)dims[1], (int)dims[2], lo[0], lo[1],
+                         lo[2], dx, dout.as<double>(), st);
+    HIPCHK(hipMemcpyAsync(data, dout.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
   });
 }
 

@@ -287,3 +287,43 @@ def fill_particles(solver, scene: Scene, mask: np.ndarray, stride: int = 1) -> n
     cyy = o[1] + (cj + 0.5) * dx
     cz = o[2] + (ck + 0.5) * dx
     return np.stack([cx + j[:, 0] * dx, cyy + j[:, 1] * dx, cz + j[:, 2] * dx], axis=1)
+
+
+def icosphere(radius: float, subdivisions: int, center=(0.0, 0.0, 0.0)):
+    """A closed, outward-wound triangle mesh of a sphere: the icosahedron subdivided
+    `subdivisions` times (20 * 4^s triangles), vertices pushed to the sphere.  The mesh
+    workload of the GPU mesh -> SDF build.  Returns (vertices (nv, 3), tris (nt, 3))."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    vv = np.array(verts) * radius + np.asarray(center, np.float64)
+    return vv, np.array(f, np.uint32)
+
+
+def box_mesh(half: float = 1.0):
+    """The box mesh of the reference's winding test (src/sdf.rs:301-322), scaled."""
+    v = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1),
+                  (-1, 1, 1)], np.float64) * half
+    f = np.array([(0, 3, 2), (0, 2, 1), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4), (3, 7, 6), (3, 6, 2),
+                  (0, 4, 7), (0, 7, 3), (1, 2, 6), (1, 6, 5)], np.uint32)
+    return v, f

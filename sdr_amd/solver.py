@@ -118,6 +118,25 @@ class Sdf:
             data[k0 * nx * ny:k1 * nx * ny] = np.asarray(f(x, y, z), dtype=np.float64).reshape(-1)
         return Sdf(tuple(map(float, origin)), float(dx), nx, ny, nz, data)
 
+    @staticmethod
+    def from_mesh(vertices, tris, dx: float, padding: float = 0.0, bounds: "Aabb | None" = None, *,
+                  lib: A.Lib | None = None, device: int = 0) -> "Sdf":
+        """`Sdf::from_mesh(mesh, dx, padding)` / `Sdf::from_mesh_in_bounds(mesh, dx, bounds)`
+        (src/sdf.rs:47-86): signed distance to a closed triangle mesh (vertices (nv, 3),
+        tris (nt, 3) vertex indices), negative inside by the generalised winding number."""
+        lib = lib if lib is not None else _lib_default()
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        tp = t.ctypes.data_as(C.POINTER(C.c_uint32))
+        bb = C.byref(bounds.c()) if bounds is not None else None
+        out = A.SdfDesc()
+        lib.sdf_from_mesh(A.dptr(v), v.shape[0], tp, t.shape[0], dx, bb, padding, device, C.byref(out), None, 0)
+        data = np.empty(out.nx * out.ny * out.nz, np.float64)
+        lib.sdf_from_mesh(A.dptr(v), v.shape[0], tp, t.shape[0], dx, bb, padding, device, C.byref(out),
+                          A.dptr(data), data.size)
+        return Sdf((out.origin.x, out.origin.y, out.origin.z), float(out.dx), int(out.nx), int(out.ny),
+                   int(out.nz), data)
+
     def bounds(self) -> Aabb:
         o = self.origin
         return Aabb(o, (o[0] + (self.nx - 1) * self.dx, o[1] + (self.ny - 1) * self.dx,
