@@ -323,6 +323,31 @@ def mesh_sdf_run(subdivisions: int = 6, n: int = 96, cpu_nodes: int = 2048) -> d
     return out
 
 
+def surface_run(s, ms_per_step: float, reps: int = 3) -> dict:
+    """Particle surfacing of the run's final state on the device: the particle distance
+    field (balls of radius dx / cbrt(ppc) on nodes at the MAC cell corners) and naive
+    surface nets (src/surface_nets.rs:48-221), as a run surfacing every 10 substeps does.
+    The first call cell-sorts the particles (the next substep's classify re-sorts them
+    anyway); later calls reuse the order."""
+    t = time.perf_counter()
+    info = s.surface()
+    first_s = time.perf_counter() - t
+    t = time.perf_counter()
+    for _ in range(reps):
+        info = s.surface()
+    per_s = (time.perf_counter() - t) / reps
+    nodes = int(info["nx"] * info["ny"] * info["nz"])
+    out = {"ms_first_call": round(first_s * 1e3, 3), "ms": round(per_s * 1e3, 3), "nodes": nodes,
+           "vertices": int(info["vertices"]), "triangles": int(info["triangles"]),
+           "area_m2": info["area"], "volume_ml": info["volume"] * 1e6,
+           "nodes_per_s": round(nodes / per_s, 1),
+           "overhead_every_10_substeps": round(first_s * 1e3 / (10.0 * ms_per_step), 5),
+           "workload": f"{info['nx']}x{info['ny']}x{info['nz']} nodes, r = {info['radius'] * 1e3:.4f} mm, "
+                       "host-timed call incl. the count read-backs"}
+    print(f"[bench] surface: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def serving_run(count: int, slots: int) -> dict:
     """The serving form: `count` irrigation scenarios (seeded Poisson arrivals at 8/s,
     grids 32^3-128^3, needle angle +-30 deg, 1-6 ml/s, 0.5-2 ml) through the native
@@ -364,6 +389,7 @@ def main() -> int:
                     help="first run one 256 MiB device-to-device copy (the reference kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh -> SDF build")
+    ap.add_argument("--no-surface", action="store_true", help="skip the particle surfacing")
     ap.add_argument("--serve-count", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
@@ -418,6 +444,12 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": tr["bytes_per_launch"] if tr else None, "traffic_detail": tr,
                 "bytes_per_call": int(by), "avg_call_ms": round(per_call_s * 1e3, 4)}
+    surf = None
+    if not args.no_surface:
+        try:
+            surf = surface_run(s, ms_per_step)
+        except Exception as e:  # the throughput number stands on its own
+            surf = {"error": repr(e)}
     err = None
     if not args.no_error and smode == A.SOLVE_RELATIVE:
         try:
@@ -491,6 +523,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "serving": serve,
         "mesh_sdf": mesh,
+        "surface": surf,
     }
     if cpu and "value" in cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -34,6 +34,7 @@ extern "C" {
 #define SDR_E_NODEV (-4)    /* no usable gfx950 device */
 #define SDR_E_STATE (-5)    /* call not valid in the current state */
 #define SDR_E_NUMERIC (-6)  /* NaN residual or persistent-kernel wait limit hit */
+#define SDR_E_IO (-7)       /* a file could not be opened, read or written */
 
 /* Arithmetic of the device path.
  *   SDR_PRECISION_F64: the plain drop-in; every field in f64, reference operand order.
@@ -253,6 +254,84 @@ int sdr_solver_cfl_dt(sdr_solver* s, double* dt);
 int sdr_sdf_from_mesh(const double* vertices, uint64_t nv, const uint32_t* tris, uint64_t nt, double dx,
                       const sdr_aabb* bounds, double padding, int32_t device, sdr_sdf* out, double* data,
                       uint64_t cap);
+
+/* ---------------------------------------------------------------------------------
+ * Frame output (output.rs:23-122, mesh.rs:356-456).  Positions are xyz triples in
+ * reference order.  Coordinates in VTK and OBJ text are the shortest decimal that
+ * parses back to the same double, in positional notation (Rust's f64 Display), so a
+ * frame reads back bit for bit.  File errors return SDR_E_IO. */
+#define SDR_FORMAT_VTK 0 /* ASCII legacy VTK, `write_points_vtk` */
+#define SDR_FORMAT_XYZ 1 /* raw little-endian f32 triples, `write_points_xyz` */
+
+/* `write_points_vtk(path, positions)` (output.rs:23-47). */
+int sdr_write_points_vtk(const char* path, const double* pos_xyz, uint64_t n);
+/* `read_points_vtk(path)` (output.rs:56-95): the POINTS block of a legacy VTK file.
+ * Sets *n to the point count and, when pos_xyz is not NULL, writes the points there
+ * (cap = points pos_xyz holds; fewer than *n is SDR_E_INVALID). */
+int sdr_read_points_vtk(const char* path, double* pos_xyz, uint64_t cap, uint64_t* n);
+/* `write_points_xyz(path, positions)` (output.rs:99-112). */
+int sdr_write_points_xyz(const char* path, const double* pos_xyz, uint64_t n);
+/* `frame_path(dir, prefix, frame, ext)` (output.rs:115-122): "dir/prefix_NNNNNN.ext"
+ * (frame zero-padded to 6 digits) into buf of len bytes, NUL included. */
+int sdr_frame_path(const char* dir, const char* prefix, uint64_t frame, const char* ext, char* buf, size_t len);
+/* `TriMesh::save_obj` (mesh.rs:356-379) and `TriMesh::save_stl` (mesh.rs:427-456). */
+int sdr_write_mesh_obj(const char* path, const double* vertices, uint64_t nv, const uint32_t* tris, uint64_t nt);
+int sdr_write_mesh_stl(const char* path, const double* vertices, uint64_t nv, const uint32_t* tris, uint64_t nt);
+
+/* The reference writes each frame synchronously from host memory (main.rs:255-265).
+ * The frame writer instead stages a solver's positions into a device buffer of its own
+ * on the solver's stream (one scatter into reference order), copies them to pinned host
+ * memory on a copy stream, and writes the file from a host thread: the substeps queued
+ * after a submit overlap the copy and the write.  `slots` (1-8) staging buffers bound
+ * the frames in flight; submit blocks only while all are busy.  Files are named
+ * frame_path(dir, prefix, frame, "vtk" | "xyz"). */
+typedef struct sdr_frame_writer sdr_frame_writer;
+int sdr_frame_writer_new(const char* dir, const char* prefix, int32_t format, uint32_t slots,
+                         sdr_frame_writer** out);
+/* Queue frame `frame` of solver s (its particles as of the work queued so far).  An
+ * error of an earlier frame's write is returned by the next submit or flush. */
+int sdr_frame_writer_submit(sdr_frame_writer* w, sdr_solver* s, uint64_t frame);
+/* Wait until every submitted frame is written; *written = frames written so far. */
+int sdr_frame_writer_flush(sdr_frame_writer* w, uint64_t* written);
+/* Flushes, then frees; a solver submitted to must outlive its pending frames. */
+int sdr_frame_writer_free(sdr_frame_writer* w);
+
+/* ---------------------------------------------------------------------------------
+ * Particle surfacing.  The reference hands a frame to the external splashsurf tool
+ * (recon.rs:60-88); here the surface is built on the device from the resident
+ * particles: the particle distance field phi(x) = min(min_j |x - x_j|, 2 r) - r on a
+ * node grid (negative inside the union of balls of radius r), then naive surface nets
+ * (surface_nets.rs:48-221) over it.  The mesh stays on the device; its counts, area
+ * and signed volume come back in sdr_surface_info. */
+typedef struct sdr_surface_params {
+  double radius;  /* ball radius r (m); 0 = dx / cbrt(particles_per_cell), the seeding spacing */
+  double spacing; /* node spacing h (m); 0 = dx (nodes on the MAC cell corners) */
+} sdr_surface_params;
+typedef struct sdr_surface_info {
+  uint64_t nx, ny, nz; /* node grid: node (i,j,k) at origin + (i,j,k) * spacing */
+  sdr_vec3 origin;
+  double spacing;
+  double radius;
+  uint64_t vertices;
+  uint64_t triangles;
+  double area;   /* `TriMesh::surface_area` (mesh.rs:52-60), summed in a fixed tree */
+  double volume; /* `TriMesh::signed_volume` (mesh.rs:63-70), summed in a fixed tree */
+} sdr_surface_info;
+/* Surface the solver's current particles over the grid box [origin, origin + n dx]. */
+int sdr_solver_surface(sdr_solver* s, const sdr_surface_params* params, sdr_surface_info* out);
+/* The last surface: vertices (xyz, vcap triples) and triangles (tcap index triples);
+ * either may be NULL.  Vertices are in cell order, triangles in node order (x, y, z
+ * edges per node), as surface_nets.rs numbers them. */
+int sdr_solver_get_surface(sdr_solver* s, double* vertices, uint64_t vcap, uint32_t* tris, uint64_t tcap);
+/* The last surface's node field phi (nx*ny*nz values, x fastest). */
+int sdr_solver_get_surface_field(sdr_solver* s, double* values, uint64_t cap);
+/* `surface_nets(origin, dx, dims, field)` (surface_nets.rs:48-195) over sampled node
+ * values (nx*ny*nz, x fastest, nodes >= 2 per axis) on device `device`.  Fills info and,
+ * for non-NULL buffers, the mesh (vcap / tcap triples held; fewer than needed is
+ * SDR_E_INVALID).  Call with NULL buffers to learn the counts. */
+int sdr_surface_nets(const double* values, uint64_t nx, uint64_t ny, uint64_t nz, sdr_vec3 origin, double dx,
+                     int32_t device, sdr_surface_info* info, double* vertices, uint64_t vcap, uint32_t* tris,
+                     uint64_t tcap);
 
 /* ---------------------------------------------------------------------------------
  * Clinical metrics (`MetricsCollector`, metrics.rs:116-446) on the device.  The

@@ -531,6 +531,11 @@ struct Solver {
   Rng rng{0x5deece66dull | 1};
   double emit_accum = 0;
   uint64_t redist_active = 0, redist_moved = 0;  // diagnostics of the last redistribute
+  // the last particle surface (orc_solver_surface)
+  std::vector<double> surf_phi, surf_v;
+  std::vector<uint32_t> surf_t;
+  sdr_surface_info surf_info{};
+  bool have_surface = false;
 
   void create(Sdf s, double sim_dx, Fluid f, Needle nd) {  // solver.rs:184-222
     solid = std::move(s);
@@ -1518,6 +1523,225 @@ int orc_sdf_from_mesh(const double* verts, uint64_t nv, const uint32_t* tris, ui
         double dist = std::sqrt(min_d2);
         data[((uint64_t)k * ny + j) * nx + i] = winding > 0.5 ? -dist : dist;
       }
+  return 0;
+}
+
+
+// ---- Particle surfacing: naive surface nets (surface_nets.rs:48-221) over sampled node
+// values, and the particle distance field the device surfaces (include/sdr_amd.h:
+// phi(x) = min(min_j |x - x_j|, 2 r) - r).  Sequential, in the reference's order.
+// One rule is the build's, on the grid's max faces: a ring cell outside the grid counts
+// as missing (the reference indexes past the row or the array there, surface_nets.rs:
+// 150-192); a field that straddles there is an open surface either way.
+namespace {
+const int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                           {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};  // surface_nets.rs:16-25
+const int kEdge[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                          {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};  // surface_nets.rs:28-42
+
+void net_emit_quad(const int64_t ring[4], V3 desired, const std::vector<V3>& verts,
+                   std::vector<uint32_t>& idx) {  // surface_nets.rs:197-221
+  for (int q = 0; q < 4; ++q)
+    if (ring[q] < 0) return;
+  const uint32_t a = (uint32_t)ring[0], b = (uint32_t)ring[1], c = (uint32_t)ring[2], d = (uint32_t)ring[3];
+  const V3 pa = verts[a], pb = verts[b], pc = verts[c];
+  const V3 n = cross(pb - pa, pc - pa);
+  if (dot(n, desired) >= 0.0) {
+    idx.insert(idx.end(), {a, b, c, a, c, d});
+  } else {
+    idx.insert(idx.end(), {a, c, b, a, d, c});
+  }
+}
+
+void surface_nets(const double* values, uint64_t nx, uint64_t ny, uint64_t nz, V3 origin, double dx,
+                  std::vector<V3>& vertices, std::vector<uint32_t>& indices) {  // surface_nets.rs:48-195
+  auto node = [&](uint64_t i, uint64_t j, uint64_t k) { return values[(k * ny + j) * nx + i]; };
+  const uint64_t cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  std::vector<int64_t> cell_vertex(cx * cy * cz, -1);
+  auto cell_idx = [&](uint64_t i, uint64_t j, uint64_t k) { return (k * cy + j) * cx + i; };
+  vertices.clear(), indices.clear();
+  for (uint64_t k = 0; k < cz; ++k)
+    for (uint64_t j = 0; j < cy; ++j)
+      for (uint64_t i = 0; i < cx; ++i) {
+        double vals[8];
+        for (int c = 0; c < 8; ++c) vals[c] = node(i + kCorner[c][0], j + kCorner[c][1], k + kCorner[c][2]);
+        std::vector<V3> crossings;
+        for (const auto& e : kEdge) {
+          const double va = vals[e[0]], vb = vals[e[1]];
+          if ((va < 0.0) != (vb < 0.0)) {
+            const double t = va / (va - vb);
+            const V3 pa = v3(kCorner[e[0]][0], kCorner[e[0]][1], kCorner[e[0]][2]);
+            const V3 pb = v3(kCorner[e[1]][0], kCorner[e[1]][1], kCorner[e[1]][2]);
+            crossings.push_back(pa + (pb - pa) * t);
+          }
+        }
+        if (crossings.empty()) continue;
+        V3 sum = v3(0.0, 0.0, 0.0);
+        for (const V3& c : crossings) sum = sum + c;
+        const V3 local = sum / (double)crossings.size();
+        const V3 world = origin + v3(((double)i + local.x) * dx, ((double)j + local.y) * dx, ((double)k + local.z) * dx);
+        cell_vertex[cell_idx(i, j, k)] = (int64_t)vertices.size();
+        vertices.push_back(world);
+      }
+  auto get = [&](uint64_t i, uint64_t j, uint64_t k) -> int64_t {
+    if (i >= cx || j >= cy || k >= cz) return -1;
+    return cell_vertex[cell_idx(i, j, k)];
+  };
+  for (uint64_t k = 0; k < nz; ++k)
+    for (uint64_t j = 0; j < ny; ++j)
+      for (uint64_t i = 0; i < nx; ++i) {
+        const double v0 = node(i, j, k);
+        if (i + 1 < nx && j >= 1 && k >= 1) {
+          const double v1 = node(i + 1, j, k);
+          if ((v0 < 0.0) != (v1 < 0.0)) {
+            const int64_t ring[4] = {get(i, j - 1, k - 1), get(i, j, k - 1), get(i, j, k), get(i, j - 1, k)};
+            net_emit_quad(ring, v0 < v1 ? v3(1.0, 0.0, 0.0) : v3(-1.0, 0.0, 0.0), vertices, indices);
+          }
+        }
+        if (j + 1 < ny && i >= 1 && k >= 1) {
+          const double v1 = node(i, j + 1, k);
+          if ((v0 < 0.0) != (v1 < 0.0)) {
+            const int64_t ring[4] = {get(i - 1, j, k - 1), get(i, j, k - 1), get(i, j, k), get(i - 1, j, k)};
+            net_emit_quad(ring, v0 < v1 ? v3(0.0, 1.0, 0.0) : v3(0.0, -1.0, 0.0), vertices, indices);
+          }
+        }
+        if (k + 1 < nz && i >= 1 && j >= 1) {
+          const double v1 = node(i, j, k + 1);
+          if ((v0 < 0.0) != (v1 < 0.0)) {
+            const int64_t ring[4] = {get(i - 1, j - 1, k), get(i, j - 1, k), get(i, j, k), get(i - 1, j, k)};
+            net_emit_quad(ring, v0 < v1 ? v3(0.0, 0.0, 1.0) : v3(0.0, 0.0, -1.0), vertices, indices);
+          }
+        }
+      }
+}
+
+void mesh_measure(const std::vector<V3>& v, const std::vector<uint32_t>& t, double* area, double* volume) {
+  double a = 0.0, vol = 0.0;
+  for (size_t f = 0; 3 * f < t.size(); ++f) {
+    const V3 p = v[t[3 * f]], q = v[t[3 * f + 1]], r = v[t[3 * f + 2]];
+    a += 0.5 * length(cross(q - p, r - p));  // mesh.rs:52-60
+    vol += dot(p, cross(q, r)) / 6.0;          // mesh.rs:63-70
+  }
+  *area = a, *volume = vol;
+}
+
+// floor(v) clamped to [0, n - 1] (NaN -> 0): the cells a node's reach overlaps.
+int64_t bin_clamp(double v, uint64_t n) {
+  const double f = std::floor(v);
+  if (!(f >= 0.0)) return 0;
+  if (f > (double)(n - 1)) return (int64_t)n - 1;
+  return (int64_t)f;
+}
+}  // namespace
+
+int orc_surface_nets(const double* values, uint64_t nx, uint64_t ny, uint64_t nz, sdr_vec3 origin, double dx,
+                     int32_t /*device*/, sdr_surface_info* info, double* vertices, uint64_t vcap, uint32_t* tris,
+                     uint64_t tcap) {
+  if (!values || !info) return fail(SDR_E_INVALID, "null argument");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(SDR_E_INVALID, "surface_nets needs at least 2 nodes per axis");
+  std::vector<V3> v;
+  std::vector<uint32_t> t;
+  surface_nets(values, nx, ny, nz, v3(origin.x, origin.y, origin.z), dx, v, t);
+  memset(info, 0, sizeof *info);
+  info->nx = nx, info->ny = ny, info->nz = nz, info->origin = origin, info->spacing = dx;
+  info->vertices = v.size(), info->triangles = t.size() / 3;
+  mesh_measure(v, t, &info->area, &info->volume);
+  if ((vertices && vcap < v.size()) || (tris && tcap < t.size() / 3))
+    return fail(SDR_E_INVALID, "output holds fewer vertices or triangles than the surface");
+  if (vertices)
+    for (size_t q = 0; q < v.size(); ++q) vertices[3 * q] = v[q].x, vertices[3 * q + 1] = v[q].y, vertices[3 * q + 2] = v[q].z;
+  if (tris && !t.empty()) memcpy(tris, t.data(), t.size() * 4);
+  return 0;
+}
+
+int orc_solver_surface(void* h, const sdr_surface_params* prm, sdr_surface_info* info) {
+  Solver* s = (Solver*)h;
+  if (!info) return fail(SDR_E_INVALID, "null output");
+  const MacGrid& g = s->grid;
+  const uint64_t ppc = s->fluid.ppc > 1 ? s->fluid.ppc : 1;
+  const double r = prm && prm->radius > 0.0 ? prm->radius : g.dx / std::cbrt((double)ppc);
+  const double hh = prm && prm->spacing > 0.0 ? prm->spacing : g.dx;
+  const uint64_t ncell[3] = {g.nx, g.ny, g.nz};
+  uint64_t nn[3];
+  for (int a = 0; a < 3; ++a)
+    nn[a] = (uint64_t)(hh == g.dx ? (double)ncell[a] : std::ceil((double)ncell[a] * g.dx / hh)) + 1;
+  // particles bucketed by their cell_of clamped into the grid
+  const uint64_t nc = g.nx * g.ny * g.nz;
+  std::vector<uint32_t> start(nc + 1, 0), order(s->pos.size());
+  std::vector<uint64_t> cell(s->pos.size());
+  for (size_t q = 0; q < s->pos.size(); ++q) {
+    const V3 l = (s->pos[q] - g.origin) / g.dx;
+    uint64_t i = as_usize(l.x), j = as_usize(l.y), k = as_usize(l.z);
+    i = i < g.nx ? i : g.nx - 1, j = j < g.ny ? j : g.ny - 1, k = k < g.nz ? k : g.nz - 1;
+    cell[q] = (k * g.ny + j) * g.nx + i;
+    start[cell[q] + 1] += 1;
+  }
+  for (uint64_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+  {
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (size_t q = 0; q < s->pos.size(); ++q) order[fill[cell[q]]++] = (uint32_t)q;
+  }
+  const uint64_t nnodes = nn[0] * nn[1] * nn[2];
+  s->surf_phi.assign(nnodes, 0.0);
+  const double reach = 2.0 * r;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)nn[2]; ++k)
+    for (uint64_t j = 0; j < nn[1]; ++j)
+      for (uint64_t i = 0; i < nn[0]; ++i) {
+        const double x = g.origin.x + (double)i * hh, y = g.origin.y + (double)j * hh,
+                     z = g.origin.z + (double)k * hh;
+        const int64_t i0 = bin_clamp(((x - reach) - g.origin.x) / g.dx, g.nx),
+                      i1 = bin_clamp(((x + reach) - g.origin.x) / g.dx, g.nx);
+        const int64_t j0 = bin_clamp(((y - reach) - g.origin.y) / g.dx, g.ny),
+                      j1 = bin_clamp(((y + reach) - g.origin.y) / g.dx, g.ny);
+        const int64_t k0 = bin_clamp(((z - reach) - g.origin.z) / g.dx, g.nz),
+                      k1 = bin_clamp(((z + reach) - g.origin.z) / g.dx, g.nz);
+        double best = std::numeric_limits<double>::infinity();
+        for (int64_t kk = k0; kk <= k1; ++kk)
+          for (int64_t jj = j0; jj <= j1; ++jj)
+            for (int64_t ii = i0; ii <= i1; ++ii) {
+              const uint64_t c = ((uint64_t)kk * g.ny + (uint64_t)jj) * g.nx + (uint64_t)ii;
+              for (uint32_t q = start[c]; q < start[c + 1]; ++q) {
+                const V3 p = s->pos[order[q]];
+                const double ex = x - p.x, ey = y - p.y, ez = z - p.z;
+                const double d2 = ex * ex + ey * ey + ez * ez;
+                best = d2 < best ? d2 : best;
+              }
+            }
+        const double d = std::sqrt(best);
+        s->surf_phi[((uint64_t)k * nn[1] + j) * nn[0] + i] = (d < reach ? d : reach) - r;
+      }
+  std::vector<V3> v;
+  surface_nets(s->surf_phi.data(), nn[0], nn[1], nn[2], g.origin, hh, v, s->surf_t);
+  s->surf_v.resize(3 * v.size());
+  for (size_t q = 0; q < v.size(); ++q) s->surf_v[3 * q] = v[q].x, s->surf_v[3 * q + 1] = v[q].y, s->surf_v[3 * q + 2] = v[q].z;
+  sdr_surface_info o{};
+  o.nx = nn[0], o.ny = nn[1], o.nz = nn[2];
+  o.origin = sdr_vec3{g.origin.x, g.origin.y, g.origin.z};
+  o.spacing = hh, o.radius = r;
+  o.vertices = v.size(), o.triangles = s->surf_t.size() / 3;
+  mesh_measure(v, s->surf_t, &o.area, &o.volume);
+  s->surf_info = o;
+  s->have_surface = true;
+  *info = o;
+  return 0;
+}
+
+int orc_solver_get_surface(void* h, double* vertices, uint64_t vcap, uint32_t* tris, uint64_t tcap) {
+  Solver* s = (Solver*)h;
+  if (!s->have_surface) return fail(SDR_E_STATE, "no surface built yet");
+  if ((vertices && vcap < s->surf_info.vertices) || (tris && tcap < s->surf_info.triangles))
+    return fail(SDR_E_INVALID, "output holds fewer vertices or triangles than the surface");
+  if (vertices && !s->surf_v.empty()) memcpy(vertices, s->surf_v.data(), s->surf_v.size() * 8);
+  if (tris && !s->surf_t.empty()) memcpy(tris, s->surf_t.data(), s->surf_t.size() * 4);
+  return 0;
+}
+
+int orc_solver_get_surface_field(void* h, double* values, uint64_t cap) {
+  Solver* s = (Solver*)h;
+  if (!s->have_surface) return fail(SDR_E_STATE, "no surface built yet");
+  if (!values || cap < s->surf_phi.size()) return fail(SDR_E_INVALID, "output holds fewer values than the node grid");
+  memcpy(values, s->surf_phi.data(), s->surf_phi.size() * 8);
   return 0;
 }
 

@@ -22,6 +22,10 @@ SDR_E_HIP = -3
 SDR_E_NODEV = -4
 SDR_E_STATE = -5
 SDR_E_NUMERIC = -6
+SDR_E_IO = -7
+
+FORMAT_VTK = 0
+FORMAT_XYZ = 1
 
 PRECISION_F64 = 0
 PRECISION_F32 = 1
@@ -139,6 +143,16 @@ class Summary(C.Structure):
                 ("resident", C.c_uint64), ("drained", C.c_uint64), ("culled", C.c_uint64)]
 
 
+class SurfaceParams(C.Structure):
+    _fields_ = [("radius", C.c_double), ("spacing", C.c_double)]
+
+
+class SurfaceInfo(C.Structure):
+    _fields_ = [("nx", C.c_uint64), ("ny", C.c_uint64), ("nz", C.c_uint64), ("origin", Vec3),
+                ("spacing", C.c_double), ("radius", C.c_double), ("vertices", C.c_uint64),
+                ("triangles", C.c_uint64), ("area", C.c_double), ("volume", C.c_double)]
+
+
 def struct_dict(st) -> dict:
     return {name: getattr(st, name) for name, _ in st._fields_}
 
@@ -183,6 +197,11 @@ SIGNATURES = {
     "metrics_get_cells": [P, c_u8_p, c_u8_p],
     "sdf_from_mesh": [c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, C.c_double, C.POINTER(Aabb),
                       C.c_double, C.c_int32, C.POINTER(SdfDesc), c_double_p, C.c_uint64],
+    "solver_surface": [P, C.POINTER(SurfaceParams), C.POINTER(SurfaceInfo)],
+    "solver_get_surface": [P, c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64],
+    "solver_get_surface_field": [P, c_double_p, C.c_uint64],
+    "surface_nets": [c_double_p, C.c_uint64, C.c_uint64, C.c_uint64, Vec3, C.c_double, C.c_int32,
+                     C.POINTER(SurfaceInfo), c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64],
 }
 # Entry points only the device library has.
 DEVICE_ONLY = {
@@ -197,6 +216,17 @@ DEVICE_ONLY = {
     # device pointers (u, v, w, cells, source, pressure) and a hipStream_t
     "project_device_noalloc": [C.POINTER(Dims), P, P, P, P, C.c_double, C.c_double, C.POINTER(SolveParamsC),
                                P, P, C.POINTER(SolveReportC), C.POINTER(Options), P],
+    # frame output (host code of the device library)
+    "write_points_vtk": [C.c_char_p, c_double_p, C.c_uint64],
+    "read_points_vtk": [C.c_char_p, c_double_p, C.c_uint64, c_u64_p],
+    "write_points_xyz": [C.c_char_p, c_double_p, C.c_uint64],
+    "frame_path": [C.c_char_p, C.c_char_p, C.c_uint64, C.c_char_p, C.c_char_p, C.c_size_t],
+    "write_mesh_obj": [C.c_char_p, c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64],
+    "write_mesh_stl": [C.c_char_p, c_double_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64],
+    "frame_writer_new": [C.c_char_p, C.c_char_p, C.c_int32, C.c_uint32, C.POINTER(P)],
+    "frame_writer_submit": [P, P, C.c_uint64],
+    "frame_writer_flush": [P, c_u64_p],
+    "frame_writer_free": [P],
 }
 
 

@@ -1331,9 +1331,22 @@ __global__ void k_to_aos(Parts<T> p, int64_t n, double* pos, double* vel) {
   pos[3 * (int64_t)r] = p.px[t];
   pos[3 * (int64_t)r + 1] = p.py[t];
   pos[3 * (int64_t)r + 2] = p.pz[t];
+  if (!vel) return;  // positions only (frame output)
   vel[3 * (int64_t)r] = p.vx[t];
   vel[3 * (int64_t)r + 1] = p.vy[t];
   vel[3 * (int64_t)r + 2] = p.vz[t];
+}
+
+// Positions as f32 triples in reference order (`p.x as f32`, output.rs:104-106).
+template <class T>
+__global__ void k_to_aos_f32(Parts<T> p, int64_t n, float* pos) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t r = p.rank[t];
+  if ((int64_t)r >= n) return;
+  pos[3 * (int64_t)r] = (float)p.px[t];
+  pos[3 * (int64_t)r + 1] = (float)p.py[t];
+  pos[3 * (int64_t)r + 2] = (float)p.pz[t];
 }
 
 template <class T>
@@ -1672,6 +1685,11 @@ void launch_to_aos(const Parts<T>& p, int64_t n, double* pos, double* vel, hipSt
   k_to_aos<T><<<grid_for(n), kBlock, 0, st>>>(p, n, pos, vel);
 }
 template <class T>
+void launch_to_aos_f32(const Parts<T>& p, int64_t n, float* pos, hipStream_t st) {
+  if (n <= 0) return;
+  k_to_aos_f32<T><<<grid_for(n), kBlock, 0, st>>>(p, n, pos);
+}
+template <class T>
 void launch_from_aos(const Parts<T>& p, int64_t n, int64_t base, const double* pos,
                      const double* vel, hipStream_t st) {
   if (n <= 0) return;
@@ -1787,6 +1805,7 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
                                        GridD<T>, CFacesD<T>, SdfD<T>, hipStream_t);                \
   template void launch_maxabs_faces<T>(const T*, int64_t, unsigned long long*, hipStream_t);       \
   template void launch_to_aos<T>(const Parts<T>&, int64_t, double*, double*, hipStream_t);         \
+  template void launch_to_aos_f32<T>(const Parts<T>&, int64_t, float*, hipStream_t);               \
   template void launch_from_aos<T>(const Parts<T>&, int64_t, int64_t, const double*, const double*, \
                                    hipStream_t);                                                   \
   template void launch_seed_fill<T>(const Parts<T>&, int64_t, GridD<T>, const uint8_t*,            \

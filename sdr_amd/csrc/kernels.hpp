@@ -192,6 +192,8 @@ void launch_maxabs_faces(const T* a, int64_t n, unsigned long long* out_bits, hi
 template <class T>
 void launch_to_aos(const Parts<T>& p, int64_t n, double* pos, double* vel, hipStream_t st);
 template <class T>
+void launch_to_aos_f32(const Parts<T>& p, int64_t n, float* pos, hipStream_t st);
+template <class T>
 void launch_from_aos(const Parts<T>& p, int64_t n, int64_t base, const double* pos,
                      const double* vel, hipStream_t st);
 template <class T>

@@ -25,77 +25,15 @@
 #include <vector>
 
 #include "../../include/sdr_amd.h"
+#include "host.hpp"
 #include "kernels.hpp"
 #include "mesh_sdf.hpp"
 #include "metrics.hpp"
 #include "mic_pipe.hpp"
 #include "sort.hpp"
+#include "surface.hpp"
 
 namespace sdr {
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(x)                                                                            \
-  do {                                                                                       \
-    hipError_t e_ = (x);                                                                     \
-    if (e_ != hipSuccess)                                                                    \
-      throw Error(e_ == hipErrorOutOfMemory ? SDR_E_OOM : SDR_E_HIP,                         \
-                  std::string(#x) + ": " + hipGetErrorString(e_));                           \
-  } while (0)
-
-// ------------------------------------------------------------------ device buffer
-struct Buf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  bool owned = true;  // false: a view of memory the buffer does not own (never freed)
-  Buf() = default;
-  Buf(const Buf&) = delete;
-  Buf& operator=(const Buf&) = delete;
-  ~Buf() { release(); }
-  void release() {
-    if (p && owned) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    owned = true;
-  }
-  void alloc(size_t b) {
-    release();
-    if (b == 0) b = 256;
-    HIPCHK(hipMalloc(&p, b));
-    bytes = b;
-  }
-  void view(void* q, size_t b) {
-    release();
-    p = q;
-    bytes = b;
-    owned = false;
-  }
-  void ensure(size_t b) {
-    if (b > bytes) alloc(b);
-  }
-  template <class U>
-  U* as() const {
-    return (U*)p;
-  }
-};
-
-struct PinnedBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~PinnedBuf() {
-    if (p) (void)hipHostFree(p);
-  }
-  void ensure(size_t b) {
-    if (b <= bytes) return;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
-    bytes = b;
-  }
-};
 
 // ------------------------------------------------------------------ host math mirror
 // Host-side pieces of the substep (emission, CFL) restate the reference in f64 with the
@@ -132,6 +70,16 @@ static int bits_for(uint64_t v) {
   int b = 1;
   while (b < 64 && (v >> b) != 0) ++b;
   return b;
+}
+
+// Copy a resident surface mesh to host buffers (either may be NULL).
+static void download_mesh(const SurfaceNets& m, double* v, uint64_t vcap, uint32_t* t, uint64_t tcap,
+                          hipStream_t st) {
+  if ((v && vcap < m.nverts) || (t && tcap < m.ntris))
+    throw Error(SDR_E_INVALID, "output holds fewer vertices or triangles than the surface");
+  if (v && m.nverts) HIPCHK(hipMemcpyAsync(v, m.verts.p, m.nverts * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (t && m.ntris) HIPCHK(hipMemcpyAsync(t, m.tris.p, m.ntris * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
 }
 
 // ------------------------------------------------------------------ timing
@@ -821,6 +769,12 @@ struct Solver : SolverBase {
       rd_big, rd_rec, rd_rmx;
   PinnedBuf pin;
   Projector<T> proj;
+  // particle surfacing: node field and the resident mesh of the last sdr_solver_surface
+  Buf surf_phi;
+  SurfaceNets nets;
+  NodeGrid surf_ng{0, 0, 0, 0, 0, 0, 0};
+  double surf_r = 0;
+  bool have_surface = false;
 
   GridD<T> grid() const { return GridD<T>{nx, ny, nz, (T)dx, (T)origin.x, (T)origin.y, (T)origin.z}; }
   SdfD<T> sdf() const {
@@ -1385,6 +1339,68 @@ struct Solver : SolverBase {
     return d;
   }
 
+  // ---------------------------------------------------------------- particle surfacing
+  // The particle distance field on a node grid over the grid box, then surface nets.
+  // The particles are cell-sorted first if a moving stage ran since the last sort; the
+  // next classify re-sorts the same order, so surfacing leaves the run unchanged.
+  sdr_surface_info surface(const sdr_surface_params* prm) {
+    const uint64_t ppc = fluid.particles_per_cell > 1 ? fluid.particles_per_cell : 1;
+    const double r = prm && prm->radius > 0.0 ? prm->radius : dx / std::cbrt((double)ppc);
+    const double h = prm && prm->spacing > 0.0 ? prm->spacing : dx;
+    if (!(r > 0.0) || !std::isfinite(r) || !(h > 0.0) || !std::isfinite(h))
+      throw Error(SDR_E_INVALID, "surface radius and spacing must be positive");
+    int nn[3];
+    const int ncell[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+      // nodes at origin + i h for i = 0 .. ceil(n dx / h): the grid box and its faces
+      const double c = h == dx ? (double)ncell[a] : std::ceil((double)ncell[a] * dx / h);
+      if (!(c >= 1.0) || c > 65535.0) throw Error(SDR_E_INVALID, "surface node grid out of range");
+      nn[a] = (int)c + 1;
+    }
+    surf_ng = NodeGrid{nn[0], nn[1], nn[2], origin.x, origin.y, origin.z, h};
+    surf_r = r;
+    if (!sorted_valid) sort_particles();
+    Scope sc_(timer, "surface");
+    surf_phi.ensure((size_t)surf_ng.nodes() * sizeof(double));
+    const Parts<T> p = parts(cur);
+    launch_particle_field<T>(p.px, p.py, p.pz, cstart.as<uint32_t>(), cend.as<uint32_t>(),
+                             BinGrid{nx, ny, nz, origin.x, origin.y, origin.z, dx}, surf_ng, r,
+                             surf_phi.as<double>(), st);
+    nets.run(surf_phi.as<double>(), surf_ng, st);
+    have_surface = true;
+    return surface_info();
+  }
+  sdr_surface_info surface_info() const {
+    sdr_surface_info o{};
+    o.nx = (uint64_t)surf_ng.nx, o.ny = (uint64_t)surf_ng.ny, o.nz = (uint64_t)surf_ng.nz;
+    o.origin = sdr_vec3{surf_ng.ox, surf_ng.oy, surf_ng.oz};
+    o.spacing = surf_ng.h;
+    o.radius = surf_r;
+    o.vertices = nets.nverts, o.triangles = nets.ntris;
+    o.area = nets.area, o.volume = nets.volume;
+    return o;
+  }
+  void get_surface(double* v, uint64_t vcap, uint32_t* t, uint64_t tcap) {
+    if (!have_surface) throw Error(SDR_E_STATE, "no surface built yet");
+    download_mesh(nets, v, vcap, t, tcap, st);
+  }
+  void get_surface_field(double* out, uint64_t cap) {
+    if (!have_surface) throw Error(SDR_E_STATE, "no surface built yet");
+    const uint64_t m = (uint64_t)surf_ng.nodes();
+    if (!out || cap < m) throw Error(SDR_E_INVALID, "output holds fewer values than the node grid");
+    HIPCHK(hipMemcpyAsync(out, surf_phi.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  // frame output: positions in reference order, staged into dst on this stream
+  void stage_positions(void* dst, bool as_f32, hipEvent_t ev) {
+    HIPCHK(hipSetDevice(opts.device));
+    if (n > 0) {
+      if (as_f32) launch_to_aos_f32<T>(parts(cur), n, (float*)dst, st);
+      else launch_to_aos<T>(parts(cur), n, (double*)dst, nullptr, st);
+    }
+    HIPCHK(hipEventRecord(ev, st));
+  }
+
   // ---------------------------------------------------------------- host I/O
   void get_particles(double* pos, double* vel, uint64_t m) {
     if ((int64_t)m != n) throw Error(SDR_E_INVALID, "particle count mismatch");
@@ -1594,24 +1610,6 @@ void* scratch_on(int dev) {
 // ------------------------------------------------------------------ C ABI plumbing
 thread_local std::string g_err;
 
-template <class F>
-int guard(F&& f) {
-  try {
-    f();
-    g_err.clear();
-    return SDR_OK;
-  } catch (const Error& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
-    return SDR_E_OOM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return SDR_E_INVALID;
-  }
-}
-
 }  // namespace sdr
 
 using sdr::Error;
@@ -1635,6 +1633,19 @@ static int with(sdr_solver* s, F&& fn) {
   });
 }
 #define WITH(s, ...) with(s, [&](auto& S) { __VA_ARGS__; })
+
+namespace sdr {
+// Frame writer support (output.cpp); called inside its guard.
+uint64_t solver_particle_total(sdr_solver* s) {
+  if (!s) throw Error(SDR_E_INVALID, "null solver");
+  return s->d ? (uint64_t)s->d->n : (uint64_t)s->f->n;
+}
+void solver_stage_positions(sdr_solver* s, void* dst, bool as_f32, hipEvent_t ev, int* device) {
+  if (!s) throw Error(SDR_E_INVALID, "null solver");
+  if (s->d) s->d->stage_positions(dst, as_f32, ev), *device = s->d->opts.device;
+  else s->f->stage_positions(dst, as_f32, ev), *device = s->f->opts.device;
+}
+}  // namespace sdr
 
 extern "C" {
 
@@ -1774,6 +1785,47 @@ int sdr_solver_diagnostics(sdr_solver* s, sdr_diagnostics* out) {
 }
 int sdr_solver_cfl_dt(sdr_solver* s, double* dt) {
   return WITH(s, if (!dt) throw Error(SDR_E_INVALID, "null output"); *dt = S.cfl_dt());
+}
+
+int sdr_solver_surface(sdr_solver* s, const sdr_surface_params* params, sdr_surface_info* out) {
+  return WITH(s, if (!out) throw Error(SDR_E_INVALID, "null output"); *out = S.surface(params));
+}
+int sdr_solver_get_surface(sdr_solver* s, double* v, uint64_t vcap, uint32_t* t, uint64_t tcap) {
+  return WITH(s, S.get_surface(v, vcap, t, tcap));
+}
+int sdr_solver_get_surface_field(sdr_solver* s, double* values, uint64_t cap) {
+  return WITH(s, S.get_surface_field(values, cap));
+}
+
+int sdr_surface_nets(const double* values, uint64_t nx, uint64_t ny, uint64_t nz, sdr_vec3 origin, double dx,
+                     int32_t device, sdr_surface_info* info, double* vertices, uint64_t vcap, uint32_t* tris,
+                     uint64_t tcap) {
+  return guard([&] {
+    if (!values || !info) throw Error(SDR_E_INVALID, "null argument");
+    if (nx < 2 || ny < 2 || nz < 2) throw Error(SDR_E_INVALID, "surface_nets needs at least 2 nodes per axis");
+    if (nx > 65536 || ny > 65536 || nz > 65536) throw Error(SDR_E_INVALID, "node grid too large");
+    if (!(dx > 0.0)) throw Error(SDR_E_INVALID, "node spacing must be positive");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) throw Error(SDR_E_NODEV, "no HIP device");
+    HIPCHK(hipSetDevice(device));
+    const sdr::NodeGrid ng{(int)nx, (int)ny, (int)nz, origin.x, origin.y, origin.z, dx};
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    sdr::Buf phi;
+    phi.alloc((size_t)ng.nodes() * sizeof(double));
+    HIPCHK(hipMemcpyAsync(phi.p, values, (size_t)ng.nodes() * sizeof(double), hipMemcpyHostToDevice, st));
+    sdr::SurfaceNets nets;
+    nets.run(phi.as<double>(), ng, st);
+    sdr_surface_info o{};
+    o.nx = nx, o.ny = ny, o.nz = nz, o.origin = origin, o.spacing = dx, o.radius = 0.0;
+    o.vertices = nets.nverts, o.triangles = nets.ntris, o.area = nets.area, o.volume = nets.volume;
+    *info = o;
+    sdr::download_mesh(nets, vertices, vcap, tris, tcap, st);
+  });
 }
 
 struct sdr_metrics {

@@ -344,6 +344,37 @@ class Solver:
         self.lib.solver_cfl_dt(self._h, C.byref(out))
         return out.value
 
+    # ---- particle surfacing (device; no reference counterpart in-process: the reference
+    # hands frames to splashsurf, src/recon.rs:60-88)
+    def surface(self, radius: float = 0.0, spacing: float = 0.0) -> dict:
+        """Surface the resident particles: the particle distance field
+        min(min_j |x - x_j|, 2 r) - r on a node grid of `spacing` (0 = dx) over the grid
+        box, polygonised by naive surface nets (src/surface_nets.rs:48-221).  radius 0 is
+        dx / cbrt(particles_per_cell).  Returns the counts, area and signed volume; the
+        mesh stays resident (`surface_mesh`)."""
+        info = A.SurfaceInfo()
+        self.lib.solver_surface(self._h, C.byref(A.SurfaceParams(radius, spacing)), C.byref(info))
+        d = A.struct_dict(info)
+        d["origin"] = (info.origin.x, info.origin.y, info.origin.z)
+        self._surface_info = d
+        return d
+
+    def surface_mesh(self):
+        """The last surface as (vertices[nv, 3] f64, triangles[nt, 3] u32)."""
+        d = self._surface_info
+        v = np.zeros((int(d["vertices"]), 3))
+        t = np.zeros((int(d["triangles"]), 3), np.uint32)
+        self.lib.solver_get_surface(self._h, A.dptr(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    t.shape[0])
+        return v, t
+
+    def surface_field(self) -> np.ndarray:
+        """The last surface's node field, shape (nz, ny, nx)."""
+        d = self._surface_info
+        out = np.zeros((int(d["nz"]), int(d["ny"]), int(d["nx"])))
+        self.lib.solver_get_surface_field(self._h, A.dptr(out), out.size)
+        return out
+
     def cell_center(self, i, j, k):
         o, d = self.origin, self.dx
         return (o[0] + (i + 0.5) * d, o[1] + (j + 0.5) * d, o[2] + (k + 0.5) * d)
