@@ -7,8 +7,10 @@ cavity workload: a box-filling rounded cavity with a socket channel, ~130M parti
 seeded 8 per cell, fixed dt.  Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line with particle-substeps/s (whole job), the pressure-solve time,
-the HBM-roofline fraction of the dominant kernel (HIP events on the solver's stream)
-and the CPU oracle timed on a bounded sample of the same workload.
+the HBM-roofline fraction of the dominant kernel (HIP events on the solver's stream),
+and the CPU oracle measured on the reference's own CPU configuration (200 substeps of
+C1, with the device run on the same workload beside it) plus a composed estimate of
+the headline workload from a bounded sample.
 """
 from __future__ import annotations
 
@@ -26,8 +28,9 @@ METRIC = "particle-substeps/s + pressure-solve ms at 256³ vs FP64 GPU; p99 scen
 
 def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> float | None:
     """Algorithmic HBM bytes of one call of a timed region (the traffic the mathematics
-    needs, independent of the implementation): s = scalar size, nc = cells, np_ = particles."""
-    flags = 1
+    needs, independent of the implementation): s = scalar size, nc = cells, np_ = particles,
+    flags = the u16 coupling flags of a cell."""
+    flags = 2
     table = {
         "mic_fwd": nc * (3 * s + flags),        # read r, precon, flags; write q
         "mic_bwd": nc * (3 * s + flags),        # read q, precon, flags; write z
@@ -46,12 +49,92 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
     return table.get(name)
 
 
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def c1_run(lib, substeps: int, **kw):
+    """The reference's own CPU configuration (36x42x34 cells at 1 mm, 8 jittered
+    particles per cavity cell, reference rule tol 1e-6 within 200 iterations, f64) for
+    `substeps` substeps on `lib` (None = the device library); returns (seconds,
+    particle-substeps, mean PCG iterations)."""
+    import oracle
+    from sdr_amd import scenes
+
+    sc = scenes.cavity_scene("c1")
+    o = scenes.make_solver(sc, lib=oracle.load())
+    pos = scenes.fill_particles(o, sc, oracle.cavity_mask(o))
+    o.close()
+    s = scenes.make_solver(sc, lib=lib, **kw)
+    s.set_particles(pos)
+    ps, its = 0, []
+    t = time.perf_counter()
+    for _ in range(substeps):
+        ps += s.particle_count()
+        s.substep(sc.dt)
+        its.append(s.state().last_pcg_iters)
+    if lib is None:
+        s.synchronize()
+    el = time.perf_counter() - t
+    s.close()
+    return el, ps, float(np.mean(its))
+
+
+def cpu_baseline_c1(threads: int, substeps: int = 200, single_substeps: int = 60) -> dict:
+    """The CPU oracle measured on the reference's own CPU configuration (C1, 200
+    substeps) with `threads` threads and with one thread (on the first
+    `single_substeps`), next to the device f64 path on the same run (same scene, fill,
+    dt, solve rule and iteration cap)."""
+    import oracle
+    from sdr_amd import _abi as A
+
+    lib = oracle.load()
+    oracle.set_threads(threads)
+    el, ps, it = c1_run(lib, substeps)
+    print(f"[cpu] c1 {substeps} substeps, {threads} threads: {el:.2f} s", file=sys.stderr, flush=True)
+    oracle.set_threads(1)
+    el1, ps1, _ = c1_run(lib, single_substeps)
+    print(f"[cpu] c1 {single_substeps} substeps, 1 thread: {el1:.2f} s", file=sys.stderr, flush=True)
+    oracle.set_threads(threads)
+    c1_run(None, 5, dot_mode=A.DOT_EXACT)  # warm-up (code objects)
+    gel, gps, git = c1_run(None, substeps, dot_mode=A.DOT_EXACT)
+    print(f"[gpu] c1 {substeps} substeps (f64, exact): {gel:.3f} s", file=sys.stderr, flush=True)
+    out = {"value": round(ps / el, 1), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+           "sample": (f"C++ f64 oracle (reference operand order, the reference's rayon loops on OpenMP) "
+                      f"on the reference's own CPU configuration C1 (36x42x34 cells, ~60k particles, "
+                      f"tol 1e-6 / 200 iterations): {substeps} substeps measured ({el:.1f} s, "
+                      f"{it:.1f} PCG iterations per substep); one thread on the first {single_substeps}"),
+           "ms_per_substep": round(el / substeps * 1e3, 3),
+           "single_thread_value": round(ps1 / el1, 1),
+           "single_thread_ms_per_substep": round(el1 / single_substeps * 1e3, 3),
+           "gpu_same_run": {"value": round(gps / gel, 1), "ms_per_substep": round(gel / substeps * 1e3, 3),
+                            "precision": "f64, reference dot tree (bit-identical to the oracle)",
+                            "pcg_iters_mean": round(git, 2)},
+           "speedup_same_run": round((gps / gel) / (ps / el), 1),
+           "speedup_same_run_single_thread": round((gps / gel) / (ps1 / el1), 1)}
+    out.update(host_cpu())
+    return out
+
+
 def cpu_baseline(scene, sdf, gpu_iters: float, budget_s: float, threads: int):
     """The CPU oracle (f64, reference operand order and threading) on a bounded sample
-    of the same workload: every stage on the full 256^3 grid, the particle stages on a
-    strided 1/`stride` subset of the particles, the PCG at 2 and 4 iterations.  The
-    per-substep time is composed as grid stages + PCG per-iteration x the GPU's
-    iteration count + particle stages x stride."""
+    of the headline workload, an extra beside the measured C1 baseline: every stage on
+    the full 256^3 grid, the particle stages on a strided 1/`stride` subset of the
+    particles, the PCG at 2 and 4 iterations.  The per-substep time is composed as grid
+    stages + PCG per-iteration x the GPU's iteration count + particle stages x stride."""
     import oracle
     from sdr_amd import scenes
 
@@ -251,26 +334,42 @@ def solve_error(s, scene, rel_ref, rel_tol, f32):
 
 
 def copy_calibration(nbytes: int) -> None:
-    """One plain device-to-device copy of `nbytes` (HIP runtime copy kernel): under
-    rocprofv3 --pmc its FETCH_SIZE and WRITE_SIZE per dispatch read `nbytes`, which
-    gives the factor from counter units to bytes."""
+    """Calibration copies for rocprofv3 --pmc:
+      * cold: a device-to-device copy (HIP runtime copy kernel) of `nbytes` from a
+        buffer that a later 2 x nbytes fill pushed out of the 256 MB Infinity Cache, so
+        FETCH_SIZE reads the whole source from HBM;
+      * warm: a 256 MiB copy by a plain elementwise kernel (one 16-byte load and one
+        16-byte store per lane) from a buffer written just before, which the Infinity
+        Cache still holds: FETCH_SIZE reads (almost) nothing, so the counter measures HBM
+        reads behind that cache, not the bytes a kernel requests; its WRITE_SIZE reads
+        the bytes written."""
     import ctypes as C
 
     hip = C.CDLL("libamdhip64.so")
-    a, b = C.c_void_p(), C.c_void_p()
+    a, b, e = C.c_void_p(), C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(e), C.c_size_t(2 * nbytes)) == 0
     assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(e, 2, C.c_size_t(2 * nbytes)) == 0  # evicts a from the Infinity Cache
     assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
     assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
+    for q in (a, b, e):
+        hip.hipFree(q)
+    import torch
+
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def pmc_traffic(kernel_region: str):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration), if it covers
-    this region; None otherwise."""
+    summary (reads from the TCC_EA0_RDREQ 32/64/128-byte request counts, writes from
+    WRITE_SIZE; both read 1.00x on a cold 1 GiB copy), if it covers this region; None
+    otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
@@ -280,8 +379,9 @@ def pmc_traffic(kernel_region: str):
     r = d.get("regions", {}).get(kernel_region)
     if not r:
         return None
-    return {"bytes_per_launch": r["bytes_per_launch"], "fetch_bytes": r["fetch_bytes"],
-            "write_bytes": r["write_bytes"], "source": "profiles/pmc_traffic.json"}
+    return {"bytes_per_launch": r["bytes_per_launch"], "read_bytes": r["read_bytes"],
+            "write_bytes": r["write_bytes"], "algorithmic_read_bytes": r["algorithmic_read_bytes"],
+            "algorithmic_write_bytes": r["algorithmic_write_bytes"], "source": "profiles/pmc_traffic.json"}
 
 
 def mesh_sdf_run(subdivisions: int = 6, n: int = 96, cpu_nodes: int = 2048) -> dict:
@@ -382,6 +482,8 @@ def main() -> int:
     ap.add_argument("--no-error", action="store_true", help="skip the error-vs-f64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-sample", action="store_true",
+                    help="skip the composed CPU estimate of the headline workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the 64-scenario serving run")
     ap.add_argument("--no-regions", action="store_true",
                     help="time the substeps without the per-region HIP events (no roofline)")
@@ -405,7 +507,7 @@ def main() -> int:
 
     ndev, arch = sdr_amd.device_info()
     if args.copy_calibration:
-        copy_calibration(256 << 20)
+        copy_calibration(1 << 30)
     if ndev < 1:
         print("no HIP device", file=sys.stderr)
         return 3
@@ -474,9 +576,14 @@ def main() -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(scene, scene.sdf, float(np.mean(iters)), 20.0, threads)
+            cpu = cpu_baseline_c1(threads)
         except Exception as e:  # the GPU number stands on its own
             cpu = {"error": repr(e)}
+        if args.config == "c3" and not args.no_cpu_sample:
+            try:
+                cpu["c3_composed_sample"] = cpu_baseline(scene, scene.sdf, float(np.mean(iters)), 20.0, threads)
+            except Exception as e:
+                cpu["c3_composed_sample"] = {"error": repr(e)}
     serve = None
     if not args.no_serve:
         try:
@@ -526,7 +633,12 @@ def main() -> int:
         "surface": surf,
     }
     if cpu and "value" in cpu:
-        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+        # like for like: the device on the CPU run's own workload (C1); the headline
+        # workload's CPU figure is the composed estimate in cpu_baseline.c3_composed_sample
+        out["speedup_vs_cpu"] = cpu["speedup_same_run"]
+        c3 = cpu.get("c3_composed_sample") or {}
+        if "value" in c3:
+            out["speedup_vs_cpu_c3_composed"] = round(value / c3["value"], 1)
     print(json.dumps(out))
     return 0
 

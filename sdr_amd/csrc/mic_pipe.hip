@@ -742,7 +742,10 @@ __global__ void __launch_bounds__(64 * (TK + 1))
         zbuf[cb][wk][lane] = pub;
         __builtin_amdgcn_sched_barrier(0);
         prev = pub;
-        bst<T>(ro, oc, res);
+        // slots outside the grid (j < 0 or j >= ny) hold +0 from the solve's set-up and
+        // are never stored to
+        const bool inc = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny;
+        bst<T>(ro, inc ? oc : kFarElem * SZ, res);
         if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
         const bool inn = (unsigned)jn < (unsigned)sh.ny;
         const uint32_t en2 = inn ? of2 : kFarElem * 2u, enS = inn ? ofn : kFarElem * SZ;

@@ -245,6 +245,10 @@ struct Projector {
     HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
     HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
     HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
+    // the pipelined sweeps store only in-grid slots: the sheared slots outside the grid
+    // of their outputs hold +0 from here on
+    HIPCHK(hipMemsetAsync(q.p, 0, q.bytes, st));
+    HIPCHK(hipMemsetAsync(z.p, 0, z.bytes, st));
   }
 
   // Bytes of caller memory init_noalloc carves for a plan (256-byte aligned pieces).
@@ -285,6 +289,10 @@ struct Projector {
     zero(halo_x, halo_x.bytes);
     zero(halo_z, halo_z.bytes);
     zero(pipe_err, sizeof(int));
+    // the sweeps store only in-grid slots; an earlier call may have left another grid's
+    // values in this memory
+    zero(q, pl.n * sizeof(T));
+    zero(z, pl.n * sizeof(T));
   }
   void zero(Buf& bf, size_t bytes) {
     if (noalloc)
