@@ -377,7 +377,8 @@ def pmc_traffic(kernel_region: str):
     except (OSError, ValueError):
         return None
     r = d.get("regions", {}).get(kernel_region)
-    if not r:
+    keys = ("bytes_per_launch", "read_bytes", "write_bytes", "algorithmic_read_bytes", "algorithmic_write_bytes")
+    if not r or any(k not in r for k in keys):
         return None
     return {"bytes_per_launch": r["bytes_per_launch"], "read_bytes": r["read_bytes"],
             "write_bytes": r["write_bytes"], "algorithmic_read_bytes": r["algorithmic_read_bytes"],
