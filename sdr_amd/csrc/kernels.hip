@@ -388,9 +388,10 @@ __device__ inline void block_partial(double v, double* partials) {
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv(Shear sh, const uint16_t* __restrict__ flags,
                                               const T* __restrict__ x, T* __restrict__ out,
-                                              const PcgScalars* sc, int it, double* partials) {
+                                              const PcgScalars* sc, double* partials) {
   if (sc->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
+  // one more iteration of a running solve (the count a graph replay cannot pass in)
+  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters += 1;
   const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
   double acc = 0.0;  // dot(x, A x) of the stored values (fast reduction mode)
   auto cell = [&](int64_t t, int i, int J, int k, uint16_t f) -> T {
@@ -491,10 +492,11 @@ inline int unit_span_len(Shear sh, int nblocks, int U) {
 template <class T, int U>
 __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __restrict__ flags,
                                                     const T* __restrict__ x, T* __restrict__ out,
-                                                    const PcgScalars* sc, int it, double* partials,
+                                                    const PcgScalars* sc, double* partials,
                                                     int span) {
   if (sc->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters = it + 1;
+  // one more iteration of a running solve (the count a graph replay cannot pass in)
+  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgScalars*>(sc)->iters += 1;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
   const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * SZ), ro = make_rsrc(out, n * SZ);
@@ -1488,14 +1490,14 @@ int cell_blocks(Shear sh) {
   return (int)(b < 1 ? 1 : b);
 }
 template <class T>
-void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc, int it,
+void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
                  double* partials, hipStream_t st) {
   const int nb = cell_blocks(sh);
   if (units_fit<T>(sh)) {
-    k_spmv_units<T, 4><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials, unit_span_len(sh, nb, 4));
+    k_spmv_units<T, 4><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, partials, unit_span_len(sh, nb, 4));
     return;
   }
-  k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, it, partials);
+  k_spmv<T><<<nb, 256, 0, st>>>(sh, flags, x, out, sc, partials);
 }
 int dot_blocks(Shear sh, bool exact) {
   if (exact) {
@@ -1766,7 +1768,7 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_rhs_flags<T>(GridD<T>, Shear, CFacesD<T>, const uint8_t*, const uint32_t*,  \
                                     const double*, double, int, double, double, double, double,    \
                                     T*, uint16_t*, double*, hipStream_t);                          \
-  template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*, int,       \
+  template void launch_spmv<T>(Shear, const uint16_t*, const T*, T*, const PcgScalars*,            \
                                double*, hipStream_t);                                              \
   template void launch_dot_partials<T>(Shear, const T*, const T*, double*, int, bool,              \
                                        const PcgScalars*, hipStream_t);                            \

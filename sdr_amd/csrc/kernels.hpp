@@ -84,7 +84,7 @@ void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
 // leaves the per-block sums of x . As for the fast reduction mode.
 template <class T>
 void launch_spmv(Shear sh, const uint16_t* flags, const T* x, T* out, const PcgScalars* sc,
-                 int it, double* partials, hipStream_t st);
+                 double* partials, hipStream_t st);
 int cell_blocks(Shear sh);
 template <class T>
 void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,

@@ -58,6 +58,20 @@ __device__ inline T words_value(const uint64_t* w) {
   return v;
 }
 
+// The sweep's tag: the launch argument, or the device word a graph-replayed sweep reads
+// (read once per block, uniform).
+template <class T>
+__device__ inline uint32_t sweep_tag(const MicPipeArgs<T>& pa) {
+  if (!pa.tagp) return pa.tag;
+  const uint32_t t = __hip_atomic_load(const_cast<uint32_t*>(pa.tagp), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+}
+
+__global__ void k_tag_next(uint32_t* tagp) {
+  const uint32_t t = tagp[0] + 1u;
+  tagp[0] = t < kDeviceTagBase ? kDeviceTagBase : t;
+}
+
 // Value of the neighbouring lane (lane - 1 forward, lane + 1 backward) by a DPP wave
 // shift of each 32-bit half; the edge lane gets garbage and is overwritten by the caller.
 template <bool REV, class T>
@@ -140,7 +154,7 @@ __global__ void __launch_bounds__(64 * TI * TK)
   const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
   const int nxp = nti * TI * 64;
   const int dJ = REV ? 1 : -1;  // J offset of the x and y neighbours
-  const uint32_t tag = pa.tag;
+  const uint32_t tag = sweep_tag(pa);
   // Which halo inputs this thread consumes.
   const bool x_consumer = (REV ? (lane == 63 && wi == TI - 1 && it + 1 < nti)
                                : (lane == 0 && wi == 0 && it > 0)) && k < sh.nz;
@@ -409,7 +423,7 @@ __global__ void __launch_bounds__(64 * TI * TK)
   const rsrc_t rf = make_rsrc(flags, ncell * 2u), rp = make_rsrc(precon, ncell * SZ),
                ri = make_rsrc(in, ncell * SZ), ro = make_rsrc(out, ncell * SZ),
                rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
-  const uint32_t tag = pa.tag;
+  const uint32_t tag = sweep_tag(pa);
   const int dJ = REV ? -1 : 1;  // row advance per step; the x/y neighbours sit at J - dJ
 
   // Where this wave's neighbour values come from and where its boundary values go.
@@ -657,7 +671,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   const int NJ = sh.NJ, nx = sh.nx, nz = sh.nz;
   const int nxp = nti * 64;
   const int dJ = REV ? -1 : 1;
-  const uint32_t tag = pa.tag;
+  const uint32_t tag = sweep_tag(pa);
   // row of compute wave w at step s
   auto row = [&](int w, int s) { return REV ? (NJ - 1) - s + (TK - 1 - w) : s - w; };
   const int nsteps = NJ + TK - 1;  // compute steps; the I/O wave runs one more to publish
@@ -1017,6 +1031,8 @@ MicPipeShape mic_pipe_shape(Shear sh, int variant, int ncu) {
   s.ntile_k = (sh.nz + s.tk - 1) / s.tk;
   return s;
 }
+
+void launch_tag_next(uint32_t* tagp, hipStream_t st) { k_tag_next<<<1, 1, 0, st>>>(tagp); }
 
 template <class T>
 void launch_mic_pipe(int op, int variant, Shear sh, const uint16_t* flags, T* precon, const T* in,

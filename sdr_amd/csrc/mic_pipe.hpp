@@ -29,7 +29,16 @@ struct MicPipeArgs {
   // dot_partials.
   const T* dot_r;
   double* dot_partials;
+  // When set, the sweep's tag is read from device memory instead of `tag`: sweeps
+  // replayed from a captured graph keep their launch arguments, so launch_tag_next
+  // advances this word before each of them.  Device tags lie in [2^31, 2^32), host tags
+  // below 2^31, so the two never meet in the halo words.
+  const uint32_t* tagp;
 };
+
+// tagp[0] = next device tag (2^31 after a wrap); one thread.
+void launch_tag_next(uint32_t* tagp, hipStream_t st);
+constexpr uint32_t kDeviceTagBase = 0x80000000u;
 
 // kind 0: k_mic_pipe (factor) / k_mic_solve tiles of 64*ti values of i by tk planes;
 // kind 4: k_mic_sweep3 tiles of 64 values of i by tk planes with an I/O wave (halo

@@ -176,7 +176,41 @@ struct Projector {
   // All working memory is a view of caller-provided memory: nothing is allocated, and
   // memory is cleared by a fill kernel instead of a memset (graph-capturable).
   bool noalloc = false;
+  // PCG iterations replayed from captured graphs, 8 at a time (the polling interval):
+  // on small grids, where launch latency and host launch overhead rival the kernels.
+  // SDR_PCG_GRAPH=0 / 1 (read once per solver) forces them off / on; never while timing
+  // or in the no-alloc projection.  The graphs hold exactly the launches of the eager
+  // loop, with sweep tags read from tagbuf, so they give the same bits.
+  bool graph_ok = false;
+  Buf tagbuf;
+  hipGraphExec_t g_pcg = nullptr, g_rel_first = nullptr, g_rel_block = nullptr;
+  double g_pcg_tol = 0.0;
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
+
+  ~Projector() {
+    for (hipGraphExec_t g : {g_pcg, g_rel_first, g_rel_block})
+      if (g) (void)hipGraphExecDestroy(g);
+  }
+  bool use_graphs() const { return graph_ok && !(timer && timer->on); }
+  // Capture the launches `body` issues on the stream into an executable graph.
+  template <class F>
+  hipGraphExec_t capture(F&& body) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    return ex;
+  }
 
   // Sizes of the working buffers (elements) and the sweep shape for a grid.
   struct Plan {
@@ -200,6 +234,8 @@ struct Projector {
     if (const char* e = getenv("SDR_TRACE_REFINE")) trace_refine = atoi(e) != 0;
     if (const char* e = getenv("SDR_REFINE_RESTART")) restart_refine = atoi(e) != 0;
     if (const char* e = getenv("SDR_REPLACE_DROP")) kReplace = atof(e);
+    graph_ok = (int64_t)nx * ny * nz <= (int64_t)1 << 21;
+    if (const char* e = getenv("SDR_PCG_GRAPH")) graph_ok = atoi(e) != 0;
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
     if (n * sizeof(T) >= lim) mic_variant = -1;
@@ -249,6 +285,12 @@ struct Projector {
     // of their outputs hold +0 from here on
     HIPCHK(hipMemsetAsync(q.p, 0, q.bytes, st));
     HIPCHK(hipMemsetAsync(z.p, 0, z.bytes, st));
+    if (graph_ok) {
+      tagbuf.alloc(sizeof(uint32_t));
+      const uint32_t t0 = kDeviceTagBase - 1;  // the first launch_tag_next gives the base
+      HIPCHK(hipMemcpyAsync(tagbuf.p, &t0, sizeof t0, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
   }
 
   // Bytes of caller memory init_noalloc carves for a plan (256-byte aligned pieces).
@@ -267,6 +309,7 @@ struct Projector {
                                      " needs " + std::to_string(need) + " bytes of working memory; the no-alloc "
                                      "projection holds " + std::to_string(cap));
     noalloc = true;
+    graph_ok = false;  // the caller may capture this projection itself
     size_t off = 0;
     auto take = [&](Buf& bf, size_t bytes) {
       bf.view(base + off, up256(bytes));
@@ -306,7 +349,7 @@ struct Projector {
   // The fast reduction mode fuses dot(z, r) into the backward solve (I/O-wave sweeps only).
   bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 4; }
 
-  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr) {
+  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr, bool dev_tag = false) {
     if (mic_variant < 0) {
       const int planes = sh.NJ + sh.nz - 1;
       for (int sgm = 0; sgm < planes; ++sgm)
@@ -317,14 +360,17 @@ struct Projector {
     int variant = mic_variant;
     if (op == kMicBuild && mic_pipe_shape(sh, variant, ncu).kind != 0) variant = sizeof(T) == 8 ? 1 : 0;
     MicPipeShape ps = mic_pipe_shape(sh, variant, ncu);
-    if (++sweep_tag == 0) {  // tags wrapped: clear the halo words so no stale tag matches
+    if (dev_tag) {
+      launch_tag_next(tagbuf.as<uint32_t>(), st);
+    } else if (++sweep_tag >= kDeviceTagBase) {  // host tags wrapped: clear the halo words
       zero(halo_x, halo_x.bytes);
       zero(halo_z, halo_z.bytes);
       sweep_tag = 1;
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, dot_r, partials.as<double>()};
+                      (uint32_t)halo_z.bytes, ncu, dot_r, partials.as<double>(),
+                      dev_tag ? tagbuf.as<uint32_t>() : nullptr};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
 
@@ -389,12 +435,14 @@ struct Projector {
     dot(z.as<T>(), r, kOpSigma0, tol);
     const int mb = maxabs_blocks(sh);
     const uint64_t check_every = 8;
-    if (fused()) {
-      const int ndot = mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu));
-      for (uint64_t it = 0; it < max_iters; ++it) {
+    const bool fz = fused();
+    const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
+    // one PCG iteration (pressure.rs:372-396); dev: sweep tags from the device word
+    auto iteration = [&](bool dev) {
+      if (fz) {
         {
           Scope sc1(*timer, "spmv_dot");
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(), st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), partials.as<double>(), st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
         }
         {
@@ -404,57 +452,69 @@ struct Projector {
         }
         {
           Scope sc1(*timer, "mic_fwd");
-          mic_sweep(kMicFwd, r, q.as<T>());
+          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
         }
         {
           Scope sc1(*timer, "mic_bwd_dot");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r, dev);
           launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
         }
         {
           Scope sc1(*timer, "update_s");
           launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
         }
-        if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
-          if (read_scalars().done) break;
-        }
+        return;
       }
-      return read_scalars();
-    }
-    for (uint64_t it = 0; it < max_iters; ++it) {
       {
         Scope sc_(*timer, "spmv_dot");
         if (exact_dot) {
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, nullptr, st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), nullptr, st);
           dot(s.as<T>(), as.as<T>(), kOpDenom, tol);
         } else {  // dot(s, As) fused into the product
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it,
-                         partials.as<double>(), st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), partials.as<double>(), st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
         }
       }
       {
         Scope sc_(*timer, "axpy_norm");
-        launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb,
-                          st);
+        launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
         launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
       }
       {
         Scope sc_(*timer, "mic_fwd");
-        mic_sweep(kMicFwd, r, q.as<T>());
+        mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
       }
       {
         Scope sc_(*timer, "mic_bwd");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), nullptr, dev);
       }
       {
         Scope sc_(*timer, "dot_update");
         dot(z.as<T>(), r, kOpSigmaNew, tol);
         launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
       }
-      if ((it + 1) % check_every == 0 && it + 1 < max_iters) {
-        if (read_scalars().done) break;
+    };
+    // blocks of check_every iterations, the scalars polled between blocks
+    uint64_t it = 0;
+    while (it < max_iters) {
+      const uint64_t nblk = std::min<uint64_t>(check_every, max_iters - it);
+      if (nblk == check_every && use_graphs()) {
+        if (g_pcg && g_pcg_tol != tol) {  // the stopping tolerance is a launch argument
+          HIPCHK(hipGraphExecDestroy(g_pcg));
+          g_pcg = nullptr;
+        }
+        if (!g_pcg) {
+          g_pcg = capture([&] {
+            for (uint64_t k = 0; k < check_every; ++k) iteration(true);
+          });
+          g_pcg_tol = tol;
+        }
+        HIPCHK(hipGraphLaunch(g_pcg, st));
+      } else {
+        for (uint64_t k = 0; k < nblk; ++k) iteration(false);
       }
+      it += nblk;
+      if (it < max_iters && read_scalars().done) break;
     }
     return read_scalars();
   }
@@ -587,14 +647,16 @@ struct Projector {
     uint64_t it = 0;
     int replacements = 0, restarts = 0;
     bool converged = false;
-    while (it < max_iters) {
+    // An iteration is split where the scalars are polled: head = the product, step and
+    // residual norm; tail = the preconditioner solve and the direction update.
+    auto head = [&] {
       {
         Scope sc1(*timer, "spmv_dot");
         if (fz) {
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, partials.as<double>(), st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), partials.as<double>(), st);
           launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), 0.0, st);
         } else {
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), (int)it, nullptr, st);
+          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), nullptr, st);
           dot(s.as<T>(), as.as<T>(), kOpDenom, 0.0);
         }
       }
@@ -603,8 +665,56 @@ struct Projector {
         launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
         launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), 0.0, st);
       }
-      ++it;
-      if (it % check_every == 0 || it == max_iters) {
+    };
+    auto tail = [&](bool dev) {
+      {
+        Scope sc1(*timer, "mic_fwd");
+        mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
+      }
+      if (fz) {
+        Scope sc1(*timer, "mic_bwd_dot");
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r, dev);
+        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+      } else {
+        {
+          Scope sc1(*timer, "mic_bwd");
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), nullptr, dev);
+        }
+        dot(z.as<T>(), r, kOpSigmaNew, 0.0);
+      }
+      {
+        Scope sc1(*timer, "update_s");
+        launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
+      }
+    };
+    bool head_done = false;  // the first block starts with a head, every later one with a tail
+    while (it < max_iters) {
+      // up to the next poll: check_every heads (and the tails between them), as graphs
+      // when a whole block remains
+      const uint64_t next = std::min<uint64_t>((it / check_every + 1) * check_every, max_iters);
+      if (next - it == check_every && it % check_every == 0 && use_graphs()) {
+        hipGraphExec_t& g = head_done ? g_rel_block : g_rel_first;
+        if (!g) {
+          const bool first = !head_done;
+          g = capture([&] {
+            for (uint64_t k = 0; k < check_every; ++k) {
+              if (k > 0 || !first) tail(true);
+              head();
+            }
+          });
+        }
+        HIPCHK(hipGraphLaunch(g, st));
+        it = next;
+      } else {
+        while (it < next) {
+          if (head_done || it > 0) tail(false);
+          head();
+          head_done = true;
+          ++it;
+        }
+      }
+      head_done = true;
+      {
         const PcgScalars sc_h = read_scalars();
         if (sc_h.nan_seen) {
           tot.nan_seen = 1;
@@ -648,25 +758,6 @@ struct Projector {
         } else if (sc_h.done) {
           break;
         }
-      }
-      {
-        Scope sc1(*timer, "mic_fwd");
-        mic_sweep(kMicFwd, r, q.as<T>());
-      }
-      if (fz) {
-        Scope sc1(*timer, "mic_bwd_dot");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r);
-        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
-      } else {
-        {
-          Scope sc1(*timer, "mic_bwd");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
-        }
-        dot(z.as<T>(), r, kOpSigmaNew, 0.0);
-      }
-      {
-        Scope sc1(*timer, "update_s");
-        launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
       }
     }
     if (!converged && !tot.nan_seen) {  // the cap hit: fold in what the f32 run has
