@@ -8,9 +8,10 @@
 // particle.  This kernel keeps exactly that order, so it reproduces the face-by-face
 // gather bit for bit, and it reads each particle once per slab for all three
 // components instead of once per visit and component:
-//   * a workgroup owns a 16 x 16 column of faces (fi, fj) over 32 levels fk and marches
-//     up the slabs ck that feed them; waves 0-3 gather the u and v faces of the column,
-//     waves 4-7 the w faces (about the same number of candidate particles each);
+//   * a workgroup owns a 16 x 16 column of faces (fi, fj) over 32 levels fk (4-16 on
+//     grids too small to fill the chip) and marches up the slabs ck that feed them;
+//     waves 0-3 gather the u and v faces of the column, waves 4-7 the w faces (about the
+//     same number of candidate particles each);
 //   * a slab's particles (18 x 18 cells) are copied into LDS once (structure of arrays)
 //     with everything a visit needs precomputed: the clamped trilinear fractions of each
 //     axis for the along-axis stencil (l) and the across-axis one (l - 0.5), the three
@@ -33,7 +34,7 @@ namespace {
 constexpr int kTX = 16, kTY = 16;          // face columns per workgroup
 constexpr int kRows = kTY + 2, kCols = kTX + 2;
 constexpr int kSlabCells = kRows * kCols;  // 324
-constexpr int kZChunk = 32;                // face levels per workgroup
+constexpr int kZChunk = 32;                // face levels per workgroup (fewer on small grids)
 constexpr int kThreads = 512;
 
 template <class T>
@@ -49,7 +50,7 @@ constexpr int kHx = 10, kHy = 11, kHz = 12;
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_p2g_fused(GridD<T> g, Parts<T> p, const uint32_t* __restrict__ cstart,
                                                         const uint32_t* __restrict__ cend, T* __restrict__ ou,
-                                                        T* __restrict__ ov, T* __restrict__ ow) {
+                                                        T* __restrict__ ov, T* __restrict__ ow, int zc) {
   constexpr int CAP = StageCap<T>::n;
   // fractions along (a) and across (c) per axis, velocities, cell/half-cell code
   __shared__ T sfax[CAP], sfay[CAP], sfaz[CAP], sfcx[CAP], sfcy[CAP], sfcz[CAP];
@@ -63,9 +64,9 @@ __global__ void __launch_bounds__(kThreads) k_p2g_fused(GridD<T> g, Parts<T> p, 
   const int ntx = (nx + 1 + kTX - 1) / kTX, nty = (ny + 1 + kTY - 1) / kTY;
   const int b = blockIdx.x;
   const int x0 = (b % ntx) * kTX, y0 = ((b / ntx) % nty) * kTY;
-  const int z0 = (b / (ntx * nty)) * kZChunk;
-  const int z1 = z0 + kZChunk < nz ? z0 + kZChunk : nz;        // u, v levels [z0, z1)
-  const int z1w = z0 + kZChunk < nz + 1 ? z0 + kZChunk : nz + 1;  // w levels [z0, z1w)
+  const int z0 = (b / (ntx * nty)) * zc;
+  const int z1 = z0 + zc < nz ? z0 + zc : nz;        // u, v levels [z0, z1)
+  const int z1w = z0 + zc < nz + 1 ? z0 + zc : nz + 1;  // w levels [z0, z1w)
   const int col = threadIdx.x & 255;
   const bool wwave = threadIdx.x >= 256;  // waves 4-7: the w faces
   const int tx = col % kTX, ty = col / kTX;
@@ -280,10 +281,14 @@ __global__ void __launch_bounds__(kThreads) k_p2g_fused(GridD<T> g, Parts<T> p, 
 template <class T>
 bool launch_p2g_fused(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend, FacesD<T> f,
                       hipStream_t st) {
-  const int64_t blocks = (int64_t)((g.nx + 1 + kTX - 1) / kTX) * ((g.ny + 1 + kTY - 1) / kTY) *
-                         ((g.nz + 1 + kZChunk - 1) / kZChunk);
+  const int64_t cols = (int64_t)((g.nx + 1 + kTX - 1) / kTX) * ((g.ny + 1 + kTY - 1) / kTY);
+  // levels per workgroup: halved from 32 (down to 4) until the launch fills the chip with
+  // two workgroups per CU; a face's sums do not depend on which workgroup owns its level
+  int zc = kZChunk;
+  while (zc > 4 && cols * ((g.nz + 1 + zc - 1) / zc) < 512) zc /= 2;
+  const int64_t blocks = cols * ((g.nz + 1 + zc - 1) / zc);
   if (blocks >= (int64_t)1 << 31) return false;
-  k_p2g_fused<T><<<(unsigned)blocks, kThreads, 0, st>>>(g, p, cstart, cend, f.u, f.v, f.w);
+  k_p2g_fused<T><<<(unsigned)blocks, kThreads, 0, st>>>(g, p, cstart, cend, f.u, f.v, f.w, zc);
   return true;
 }
 

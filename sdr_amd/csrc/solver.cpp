@@ -176,11 +176,11 @@ struct Projector {
   // All working memory is a view of caller-provided memory: nothing is allocated, and
   // memory is cleared by a fill kernel instead of a memset (graph-capturable).
   bool noalloc = false;
-  // PCG iterations replayed from captured graphs, 8 at a time (the polling interval):
-  // on small grids, where launch latency and host launch overhead rival the kernels.
-  // SDR_PCG_GRAPH=0 / 1 (read once per solver) forces them off / on; never while timing
-  // or in the no-alloc projection.  The graphs hold exactly the launches of the eager
-  // loop, with sweep tags read from tagbuf, so they give the same bits.
+  // PCG iterations replayed from captured graphs, 8 at a time (the polling interval),
+  // when SDR_PCG_GRAPH=1 (read once per solver); never while timing or in the no-alloc
+  // projection.  The graphs hold exactly the launches of the eager loop, with sweep tags
+  // read from tagbuf, so they give the same bits.  Off by default: on 48^3-128^3 grids
+  // an iteration is bound by the sweeps' latency, and replays measured no faster.
   bool graph_ok = false;
   Buf tagbuf;
   hipGraphExec_t g_pcg = nullptr, g_rel_first = nullptr, g_rel_block = nullptr;
@@ -196,7 +196,8 @@ struct Projector {
   template <class F>
   hipGraphExec_t capture(F&& body) {
     hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    // relaxed: solvers on other threads keep allocating and copying meanwhile
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     try {
       body();
     } catch (...) {
@@ -234,7 +235,7 @@ struct Projector {
     if (const char* e = getenv("SDR_TRACE_REFINE")) trace_refine = atoi(e) != 0;
     if (const char* e = getenv("SDR_REFINE_RESTART")) restart_refine = atoi(e) != 0;
     if (const char* e = getenv("SDR_REPLACE_DROP")) kReplace = atof(e);
-    graph_ok = (int64_t)nx * ny * nz <= (int64_t)1 << 21;
+    graph_ok = false;
     if (const char* e = getenv("SDR_PCG_GRAPH")) graph_ok = atoi(e) != 0;
     // The pipelined sweeps address with 32-bit byte offsets.
     const size_t lim = (size_t)1 << 31;
