@@ -454,7 +454,8 @@ int sdr_project_device_noalloc(const sdr_dims* grid, double* u, double* v, doubl
  * request is admitted when a slot is free and its estimated HBM footprint fits the
  * budget (first come, first served), runs on its own solver and HIP stream from its own
  * host thread, so the work of admitted scenarios overlaps on the device, and is
- * released when it finishes.  Each scenario's result depends only on its own inputs
+ * released when it finishes; with resident pools the slot keeps its memory for the
+ * next request.  Each scenario's result depends only on its own inputs
  * (no float atomics, no shared state): it is the same alone and in the mix. */
 typedef struct sdr_scenario {
   sdr_sdf solid;            /* host SDF, read while the scenario is being set up */
@@ -476,6 +477,8 @@ typedef struct sdr_scenario {
 typedef struct sdr_serve_config {
   double hbm_budget_bytes;  /* admission budget over the estimated footprints */
   uint32_t max_slots;       /* scenarios in flight at once */
+  uint32_t resident_pools;  /* 1: each slot keeps its device memory in a pool across
+                               requests (sdr_pool_*); 0: every request allocates */
 } sdr_serve_config;
 
 typedef struct sdr_scenario_result {
@@ -487,7 +490,33 @@ typedef struct sdr_scenario_result {
   uint64_t particle_substeps;           /* sum over substeps of the particle count */
   uint64_t pcg_iters;                   /* sum over substeps */
   uint64_t checksum;        /* FNV-1a of the final positions and velocities (f64 bits) */
+  uint64_t device_allocs;   /* hipMalloc / hipHostMalloc calls of the request (pool misses) */
 } sdr_scenario_result;
+
+/* Memory pools (no reference counterpart).  A pool keeps the device and pinned host
+ * blocks released by the solvers made from it and hands them to later allocations of a
+ * similar size, so a serving slot runs request after request without hipMalloc/hipFree
+ * (hipFree waits for the whole device).  sdr_pool_bind(p) makes the calling thread's
+ * allocations come from p (NULL: plain allocation again); a buffer returns to the pool
+ * it came from, whichever thread frees it.  A pool serves one solver at a time: a
+ * solver's blocks return to it when sdr_solver_free has drained the solver's stream.
+ * A failed allocation frees the pool's cached blocks and tries once more. */
+typedef struct sdr_pool sdr_pool;
+typedef struct sdr_pool_stats {
+  uint64_t hits;          /* allocations served from cached blocks */
+  uint64_t misses;        /* allocations that called hipMalloc / hipHostMalloc */
+  uint64_t cached_bytes;  /* held for reuse */
+  uint64_t live_bytes;    /* in use by buffers */
+  uint64_t cached_blocks;
+  uint64_t live_blocks;
+} sdr_pool_stats;
+int sdr_pool_new(int32_t device, sdr_pool** out);
+int sdr_pool_bind(sdr_pool* p /* nullable */);
+int sdr_pool_get_stats(const sdr_pool* p, sdr_pool_stats* out);
+/* Free the cached blocks (the live ones stay). */
+int sdr_pool_trim(sdr_pool* p);
+/* SDR_E_STATE while buffers made from it are alive; unbind it from every thread first. */
+int sdr_pool_free(sdr_pool* p);
 
 /* Estimated device footprint (bytes) of a solver of the given grid, particle capacity
  * and precision (the admission estimate). */

@@ -100,14 +100,19 @@ class Scenario(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_slots", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_slots", C.c_uint32), ("resident_pools", C.c_uint32)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("hits", C.c_uint64), ("misses", C.c_uint64), ("cached_bytes", C.c_uint64),
+                ("live_bytes", C.c_uint64), ("cached_blocks", C.c_uint64), ("live_blocks", C.c_uint64)]
 
 
 class ScenarioResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("slot", C.c_uint32), ("arrival_s", C.c_double),
                 ("admit_s", C.c_double), ("finish_s", C.c_double), ("footprint_bytes", C.c_uint64),
                 ("particles", C.c_uint64), ("particle_substeps", C.c_uint64),
-                ("pcg_iters", C.c_uint64), ("checksum", C.c_uint64)]
+                ("pcg_iters", C.c_uint64), ("checksum", C.c_uint64), ("device_allocs", C.c_uint64)]
 
 
 class Diagnostics(C.Structure):
@@ -227,6 +232,11 @@ DEVICE_ONLY = {
     "frame_writer_submit": [P, P, C.c_uint64],
     "frame_writer_flush": [P, c_u64_p],
     "frame_writer_free": [P],
+    "pool_new": [C.c_int32, C.POINTER(P)],
+    "pool_bind": [P],
+    "pool_get_stats": [P, C.POINTER(PoolStats)],
+    "pool_trim": [P],
+    "pool_free": [P],
 }
 
 

@@ -29,7 +29,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["kernels.hip", "mic_pipe.hip", "redist.hip", "p2g.hip", "sort.hip", "metrics.hip", "mesh_sdf.hip",
-           "surface.hip", "output.cpp",
+           "surface.hip", "output.cpp", "pool.cpp",
            "solver.cpp",
            "serve.cpp"]
 

@@ -5,9 +5,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
+#include <utility>
 
 #include "../../include/sdr_amd.h"
 
@@ -26,25 +30,56 @@ struct Error : std::runtime_error {
                   std::string(#x) + ": " + hipGetErrorString(e_));                           \
   } while (0)
 
+// ------------------------------------------------------------------ memory pools
+// A pool keeps the device and pinned host blocks its buffers release and hands them to
+// later allocations of a similar size, so a serving slot runs request after request
+// without hipMalloc / hipFree (hipFree waits for the device).  A thread allocates from
+// the pool bound to it (sdr_pool_bind); a released block returns to the pool it came
+// from.  One solver at a time per pool: a block is reused only by work queued after the
+// stream that used it last was synchronised or on that same stream.
+struct DevicePool {
+  int device = 0;
+  std::mutex mu;
+  std::multimap<size_t, void*> free_dev, free_host;  // by capacity
+  std::unordered_map<void*, size_t> dev_cap, host_cap;  // every block, by address
+  uint64_t hits = 0, misses = 0;
+  size_t cached = 0, live = 0;
+  void* get(size_t bytes, bool host);
+  void put(void* p, bool host);
+  void trim();  // free every cached block
+  ~DevicePool();
+};
+extern thread_local DevicePool* g_pool;  // the calling thread's bound pool (null: none)
+
 // ------------------------------------------------------------------ device buffer
 struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
   bool owned = true;  // false: a view of memory the buffer does not own (never freed)
+  DevicePool* pool = nullptr;  // where an owned block goes back to (null: hipFree)
   Buf() = default;
   Buf(const Buf&) = delete;
   Buf& operator=(const Buf&) = delete;
   ~Buf() { release(); }
   void release() {
-    if (p && owned) (void)hipFree(p);
+    if (p && owned) {
+      if (pool) pool->put(p, false);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
     owned = true;
+    pool = nullptr;
   }
   void alloc(size_t b) {
     release();
     if (b == 0) b = 256;
-    HIPCHK(hipMalloc(&p, b));
+    if (g_pool) {
+      p = g_pool->get(b, false);
+      pool = g_pool;
+    } else {
+      HIPCHK(hipMalloc(&p, b));
+    }
     bytes = b;
   }
   void view(void* q, size_t b) {
@@ -56,6 +91,10 @@ struct Buf {
   void ensure(size_t b) {
     if (b > bytes) alloc(b);
   }
+  // exchange the blocks of two buffers (growth keeps the old one alive for a copy)
+  void swap(Buf& o) {
+    std::swap(p, o.p), std::swap(bytes, o.bytes), std::swap(owned, o.owned), std::swap(pool, o.pool);
+  }
   template <class U>
   U* as() const {
     return (U*)p;
@@ -65,14 +104,26 @@ struct Buf {
 struct PinnedBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  ~PinnedBuf() {
-    if (p) (void)hipHostFree(p);
+  DevicePool* pool = nullptr;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) {
+      if (pool) pool->put(p, true);
+      else (void)hipHostFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
+    pool = nullptr;
   }
   void ensure(size_t b) {
     if (b <= bytes) return;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    release();
+    if (g_pool) {
+      p = g_pool->get(b, true);
+      pool = g_pool;
+    } else {
+      HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    }
     bytes = b;
   }
 };

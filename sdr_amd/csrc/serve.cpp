@@ -11,7 +11,10 @@
 //     the device; the thread sets the scenario up (SDF upload, cavity mask, seeded
 //     fill), runs its substeps and reads back its particles;
 //   * release: when a request finishes, its slot and footprint return to the pool and
-//     the queue is re-examined at once.
+//     the queue is re-examined at once;
+//   * resident pools (cfg->resident_pools): each slot keeps a memory pool (sdr_pool_*)
+//     across the requests it serves, so after the first requests a solver's buffers come
+//     from blocks the slot already holds instead of hipMalloc / hipFree.
 // The loop only uses the public C ABI, so every scenario runs exactly the code a
 // standalone caller would run: its result is the same alone and in the mix.
 #include <algorithm>
@@ -49,7 +52,27 @@ uint64_t particle_bound(const sdr_scenario& s, uint64_t nx, uint64_t ny, uint64_
   return (uint64_t)nx * ny * nz * s.fill_per_cell + (uint64_t)emit + 1024;
 }
 
-int run_one(const sdr_scenario& s, sdr_scenario_result* r) {
+int run_one(const sdr_scenario& s, sdr_scenario_result* r, sdr_pool* pool) {
+  // the slot's pool, if any, serves every allocation of the request on this thread
+  struct Bind {
+    sdr_pool* p;
+    uint64_t misses0 = 0;
+    sdr_scenario_result* r;
+    Bind(sdr_pool* p_, sdr_scenario_result* r_) : p(p_), r(r_) {
+      if (!p) return;
+      sdr_pool_stats ps{};
+      sdr_pool_get_stats(p, &ps);
+      misses0 = ps.misses;
+      sdr_pool_bind(p);
+    }
+    ~Bind() {
+      if (!p) return;
+      sdr_pool_bind(nullptr);
+      sdr_pool_stats ps{};
+      sdr_pool_get_stats(p, &ps);
+      r->device_allocs = ps.misses - misses0;
+    }
+  } bind{pool, r};
   sdr_solver* h = nullptr;
   int rc = sdr_solver_new(&s.solid, s.sim_dx, &s.fluid, &s.needle, &s.opts, &h);
   if (rc != SDR_OK) return rc;
@@ -115,6 +138,20 @@ extern "C" int sdr_serve(const sdr_scenario* sc, uint32_t n, const sdr_serve_con
   std::vector<uint32_t> free_slots;
   for (uint32_t k = cfg->max_slots; k > 0; --k) free_slots.push_back(k - 1);
   std::vector<std::thread> workers;
+  // one pool per slot, kept across the requests the slot serves
+  std::vector<sdr_pool*> pools(cfg->max_slots, nullptr);
+  struct FreePools {
+    std::vector<sdr_pool*>& v;
+    ~FreePools() {
+      for (sdr_pool* p : v)
+        if (p) sdr_pool_free(p);
+    }
+  } free_pools{pools};
+  if (cfg->resident_pools) {
+    int32_t dev = n > 0 ? sc[0].opts.device : 0;
+    for (auto& p : pools)
+      if (sdr_pool_new(dev, &p) != SDR_OK) p = nullptr;
+  }
   const Clock::time_point t0 = Clock::now();
   auto now_s = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   size_t head = 0;
@@ -142,7 +179,7 @@ extern "C" int sdr_serve(const sdr_scenario* sc, uint32_t n, const sdr_serve_con
       ++head;
       workers.emplace_back([&, i, slot] {
         sdr_scenario_result r = out[i];
-        const int rc = run_one(sc[i], &r);
+        const int rc = run_one(sc[i], &r, pools[slot]);
         std::lock_guard<std::mutex> g(mu);
         r.status = rc;
         r.finish_s = now_s();

@@ -908,7 +908,11 @@ struct Solver : SolverBase {
   CFacesD<T> csaved() const { return CFacesD<T>{su.as<T>(), sv.as<T>(), sw.as<T>()}; }
 
   ~Solver() override {
-    if (st) (void)hipStreamDestroy(st);
+    // the buffers go back to their pool (or hipFree) after this: nothing may still use them
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   }
 
   // Solver::new (solver.rs:184-222).
@@ -988,8 +992,7 @@ struct Solver : SolverBase {
         Buf nb;
         nb.alloc((size_t)nc * es);
         if (w == cur && n > 0) HIPCHK(hipMemcpyAsync(nb.p, part[w][a].p, (size_t)n * es, hipMemcpyDeviceToDevice, st));
-        std::swap(part[w][a].p, nb.p);
-        std::swap(part[w][a].bytes, nb.bytes);
+        part[w][a].swap(nb);
       }
     for (Buf* b : {&keys, &keys_alt, &vals, &vals_alt, &keep, &scanpos, &cullflag, &culled_ranks,
                    &culled_sorted, &cellkey, &act_rank, &act_slot, &act_rank_sorted,

@@ -250,6 +250,7 @@ pub struct sdr_scenario {
 pub struct sdr_serve_config {
     pub hbm_budget_bytes: f64,
     pub max_slots: u32,
+    pub resident_pools: u32,
 }
 
 #[repr(C)]
@@ -265,6 +266,18 @@ pub struct sdr_scenario_result {
     pub particle_substeps: u64,
     pub pcg_iters: u64,
     pub checksum: u64,
+    pub device_allocs: u64,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default)]
+pub struct sdr_pool_stats {
+    pub hits: u64,
+    pub misses: u64,
+    pub cached_bytes: u64,
+    pub live_bytes: u64,
+    pub cached_blocks: u64,
+    pub live_blocks: u64,
 }
 
 #[repr(C)]
@@ -337,6 +350,11 @@ extern "C" {
     pub fn sdr_solver_seed_fill(s: *mut sdr_solver, per_cell: u32, y_lo: f64, y_hi: f64, seed: u64, vel: *const sdr_vec3, n_out: *mut u64) -> c_int;
     pub fn sdr_project_capturing_with_source(grid: *const sdr_dims, u: *mut f64, v: *mut f64, w: *mut f64, cells: *const u8, rho: f64, dt: f64, params: *const sdr_solve_params, source: *const f64, pressure_out: *mut f64, report: *mut sdr_solve_report, opts: *const sdr_options) -> c_int;
     pub fn sdr_project_device_noalloc(grid: *const sdr_dims, u: *mut f64, v: *mut f64, w: *mut f64, cells: *const u8, rho: f64, dt: f64, params: *const sdr_solve_params, source: *const f64, pressure_out: *mut f64, report: *mut sdr_solve_report, opts: *const sdr_options, stream: *mut c_void) -> c_int;
+    pub fn sdr_pool_new(device: i32, out: *mut *mut sdr_pool) -> c_int;
+    pub fn sdr_pool_bind(p: *mut sdr_pool) -> c_int;
+    pub fn sdr_pool_get_stats(p: *const sdr_pool, out: *mut sdr_pool_stats) -> c_int;
+    pub fn sdr_pool_trim(p: *mut sdr_pool) -> c_int;
+    pub fn sdr_pool_free(p: *mut sdr_pool) -> c_int;
     pub fn sdr_solver_footprint(nx: u64, ny: u64, nz: u64, particles: u64, precision: i32, bytes: *mut u64) -> c_int;
     pub fn sdr_serve(scenarios: *const sdr_scenario, n: u32, cfg: *const sdr_serve_config, out: *mut sdr_scenario_result) -> c_int;
 }

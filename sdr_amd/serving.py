@@ -117,6 +117,7 @@ class ScenarioRecord:
     particle_substeps: int
     pcg_iters: int
     checksum: int
+    device_allocs: int = 0
 
     @property
     def latency_s(self) -> float:
@@ -124,8 +125,9 @@ class ScenarioRecord:
 
 
 def serve(specs: list[ScenarioSpec], budget_bytes: float = 200e9, max_slots: int = 8,
-          lib: A.Lib | None = None) -> list[ScenarioRecord]:
-    """Run the requests through the native serving loop (blocks until all finish)."""
+          lib: A.Lib | None = None, resident_pools: bool = True) -> list[ScenarioRecord]:
+    """Run the requests through the native serving loop (blocks until all finish);
+    resident_pools keeps each slot's device memory across the requests it serves."""
     if lib is None:
         from . import load_library
 
@@ -133,10 +135,10 @@ def serve(specs: list[ScenarioSpec], budget_bytes: float = 200e9, max_slots: int
     built = [_Built(s) for s in specs]
     arr = (A.Scenario * len(specs))(*[b.c for b in built])
     res = (A.ScenarioResult * len(specs))()
-    cfg = A.ServeConfig(float(budget_bytes), int(max_slots))
+    cfg = A.ServeConfig(float(budget_bytes), int(max_slots), 1 if resident_pools else 0)
     lib.serve(arr, len(specs), C.byref(cfg), res)
     return [ScenarioRecord(s, r.status, r.slot, r.arrival_s, r.admit_s, r.finish_s, r.footprint_bytes,
-                           r.particles, r.particle_substeps, r.pcg_iters, r.checksum)
+                           r.particles, r.particle_substeps, r.pcg_iters, r.checksum, r.device_allocs)
             for s, r in zip(specs, res)]
 
 
@@ -158,6 +160,7 @@ def summarize(recs: list[ScenarioRecord]) -> dict:
         "particle_substeps": int(ps),
         "aggregate_particle_substeps_per_s": round(ps / max(t1 - t0, 1e-9), 1),
         "max_in_flight": max_in_flight(recs),
+        "device_allocs": int(sum(r.device_allocs for r in recs)),
     }
 
 

@@ -96,3 +96,24 @@ def test_admission_respects_slots_budget_and_rejects_oversized(dev):
     assert sum(r.status == A.SDR_OK for r in rej) >= 1
     s = serving.summarize(rej)
     assert s["completed"] + s["rejected"] == len(specs)
+
+
+@pytest.mark.gpu
+def test_resident_pools_reuse_memory_and_keep_results(dev):
+    """With resident pools a slot's later requests take their buffers from the blocks its
+    earlier ones released (few or no new device allocations), and every result is the
+    same as with a fresh allocation per request."""
+    specs = _small_mix()
+    pooled = serving.serve(specs, max_slots=2, resident_pools=True)
+    fresh = serving.serve(specs, max_slots=2, resident_pools=False)
+    assert all(r.status == A.SDR_OK for r in pooled + fresh)
+    for a, b in zip(pooled, fresh):
+        assert (a.checksum, a.particles, a.pcg_iters) == (b.checksum, b.particles, b.pcg_iters)
+        assert b.device_allocs == 0  # no pool: nothing is counted
+    by_slot = {}
+    for r in sorted(pooled, key=lambda r: r.admit_s):
+        by_slot.setdefault(r.slot, []).append(r)
+    firsts = [rs[0].device_allocs for rs in by_slot.values()]
+    laters = [r.device_allocs for rs in by_slot.values() for r in rs[1:]]
+    assert min(firsts) > 20 and laters
+    assert max(laters) < min(firsts) / 2
