@@ -555,46 +555,33 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
 }
 
 // ------------------------------------------------------------------ dot (pressure.rs:588-603)
-// Exact mode: one wavefront per 4096-element chunk of the reference cell order; each
-// lane forms one product and lane order is folded sequentially into one accumulator,
-// so acc = ((0 + x0*y0) + x1*y1) + ... exactly as the reference chunk loop.
-__device__ inline double readlane_f64(double v, int lane) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_readlane(lo, lane);
-  hi = __builtin_amdgcn_readlane(hi, lane);
-  return __hiloint2double(hi, lo);
-}
-
+// Exact mode: one workgroup per 4096-element chunk of the reference cell order.  The
+// products are formed in parallel into LDS, then one lane folds them left to right,
+// acc = ((0 + x0*y0) + x1*y1) + ..., exactly as the reference chunk loop; the fold is a
+// chain of dependent adds, so the LDS loads run ahead of it.
 template <class T>
-__global__ void __launch_bounds__(64) k_dot_exact(Shear sh, const T* __restrict__ a,
-                                                  const T* __restrict__ b, double* partials,
-                                                  const PcgScalars* sc) {
+__global__ void __launch_bounds__(256) k_dot_exact(Shear sh, const T* __restrict__ a,
+                                                   const T* __restrict__ b, double* partials,
+                                                   const PcgScalars* sc) {
   if (sc->done) return;
+  __shared__ double prod[4096];
   const int64_t n = (int64_t)sh.nx * sh.ny * sh.nz;
   const int64_t c0 = (int64_t)blockIdx.x * 4096;
-  const int lane = threadIdx.x;
-  double acc = 0.0;
-  for (int m = 0; m < 64; ++m) {
-    int64_t r = c0 + m * 64 + lane;
-    double prod = 0.0;
-    bool ok = r < n;
-    if (ok) {
-      int i = (int)(r % sh.nx);
-      int j = (int)((r / sh.nx) % sh.ny);
-      int k = (int)(r / ((int64_t)sh.nx * sh.ny));
-      int64_t s = sh.at(i, j, k);
-      prod = (double)a[s] * (double)b[s];
-    }
-    int live = (int)((n - (c0 + m * 64)) < 64 ? (n - (c0 + m * 64)) : 64);
-    if (live <= 0) break;
-    if (live == 64) {
-#pragma unroll
-      for (int l = 0; l < 64; ++l) acc += readlane_f64(prod, l);
-    } else {
-      for (int l = 0; l < live; ++l) acc += readlane_f64(prod, l);
-    }
+  const int live = (int)(n - c0 < 4096 ? n - c0 : 4096);
+  for (int l = threadIdx.x; l < live; l += 256) {
+    const int64_t r = c0 + l;
+    const int i = (int)(r % sh.nx);
+    const int j = (int)((r / sh.nx) % sh.ny);
+    const int k = (int)(r / ((int64_t)sh.nx * sh.ny));
+    const int64_t s = sh.at(i, j, k);
+    prod[l] = (double)a[s] * (double)b[s];
   }
-  if (lane == 0) partials[blockIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double acc = 0.0;
+#pragma unroll 16
+  for (int l = 0; l < live; ++l) acc += prod[l];
+  partials[blockIdx.x] = acc;
 }
 
 // Fast mode: deterministic fixed tree in f64 (per-thread strided sum, then a wavefront
@@ -1511,7 +1498,7 @@ template <class T>
 void launch_dot_partials(Shear sh, const T* a, const T* b, double* partials, int nblocks, bool exact,
                          const PcgScalars* sc, hipStream_t st) {
   if (exact)
-    k_dot_exact<T><<<nblocks, 64, 0, st>>>(sh, a, b, partials, sc);
+    k_dot_exact<T><<<nblocks, 256, 0, st>>>(sh, a, b, partials, sc);
   else
     k_dot_fast<T><<<nblocks, 256, 0, st>>>(sh, a, b, partials, sc);
 }

@@ -67,10 +67,11 @@ def serving_workload(count: int = 64, seed: int = 2026, rate: float = 8.0, n_min
 
 
 def scenario_dt(spec: ScenarioSpec) -> float:
-    """The fixed substep of a request: spec.dt, at most one cell of jet travel."""
-    dx = 32 * MM / spec.n
-    speed = spec.flow_ml_s * ML_PER_S / (math.pi * (0.25 * MM) ** 2)
-    return min(spec.dt, dx / speed)
+    """The fixed substep of a request: spec.dt, at most one cell of jet travel (the CFL
+    bound of solver.rs:262-281)."""
+    needle = Needle(tip=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0), radius=0.25 * MM,
+                    flow_rate=spec.flow_ml_s * ML_PER_S)
+    return min(spec.dt, (32 * MM / spec.n) / needle.jet_speed())
 
 
 class _Built:
@@ -89,9 +90,7 @@ class _Built:
         axis = (0.0, math.sin(a), math.cos(a))
         tip = (0.0, -0.25 * half, -0.5 * half)
         needle = Needle(tip=tip, axis=axis, radius=0.25 * MM, flow_rate=spec.flow_ml_s * ML_PER_S)
-        # fixed substep, at most one cell of jet travel (the CFL bound of solver.rs:262-281)
-        self.dt = min(spec.dt, dx / needle.jet_speed())
-        assert self.dt == scenario_dt(spec)
+        self.dt = scenario_dt(spec)
         outlet = _socket_outlet(origin[1], 2 * MM, dx)
         c = A.Scenario()
         c.solid = self.sdf.c()
