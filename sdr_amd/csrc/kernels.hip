@@ -1441,14 +1441,6 @@ void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
     if (comp == 2) k_p2g<T, 2, false><<<gr, kBlock, 0, st>>>(g, p, cstart, cend, out);
   }
 }
-bool p2g_perface() {
-  // SDR_P2G_PERFACE=1 (read once per process) selects the face-by-face gather
-  static const bool perface = [] {
-    const char* pf = getenv("SDR_P2G_PERFACE");
-    return pf && atoi(pf) != 0;
-  }();
-  return perface;
-}
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st) {
   k_add_const<T><<<grid_for(n), kBlock, 0, st>>>(a, n, c);

@@ -70,7 +70,6 @@ template <class T>
 bool launch_p2g_fused(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend, FacesD<T> f,
                       hipStream_t st);
 // SDR_P2G_PERFACE=1: the face-by-face gather kernels instead of the fused one.
-bool p2g_perface();
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st);
 template <class T>

@@ -853,6 +853,7 @@ struct Solver : SolverBase {
   uint32_t rd_epoch = 0;
   int64_t n = 0, cap = 0;
   bool sorted_valid = false, have_saved = false;
+  bool p2g_perface = false;
   int cur = 0;
   // -- device
   hipStream_t st = nullptr;
@@ -975,6 +976,8 @@ struct Solver : SolverBase {
     }
     launch_static_cls<double>(grid64(), mask.as<uint8_t>(), outlet_box64(), static_cls.as<uint8_t>(), st);
     if (const char* e = getenv("SDR_REDIST_SEQUENTIAL")) redist_sequential = atoi(e) != 0;
+    // SDR_P2G_PERFACE=1 (read once per solver) selects the face-by-face gather
+    if (const char* e = getenv("SDR_P2G_PERFACE")) p2g_perface = atoi(e) != 0;
     proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
     proj.init(nx, ny, nz, dx, st, &timer);
     reserve(1024);
@@ -1103,7 +1106,7 @@ struct Solver : SolverBase {
   void p2g() {
     ensure_counts();
     Scope sc_(timer, "p2g");
-    if (!exact() && !p2g_perface() &&
+    if (!exact() && !p2g_perface &&
         launch_p2g_fused<T>(grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), faces(), st))
       return;
     launch_p2g<T>(0, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fu.as<T>(), exact(), st);

@@ -123,15 +123,16 @@ def test_fast_mode_within_tolerance(dev, orc):
 
 
 @pytest.mark.parametrize("precision", [A.PRECISION_F32, A.PRECISION_F64])
-def test_p2g_tiled_matches_face_gather(dev, precision, monkeypatch):
-    """The LDS-staged P2G (slabs staged once, several z-levels accumulated at a time)
-    sums every face in the same order as the face-by-face gather, so the two agree bit
-    for bit; a 5000-particle clump makes some slabs overflow the staging area (several
-    passes)."""
+def test_p2g_fused_matches_face_gather(dev, precision, monkeypatch):
+    """The one-pass LDS-staged P2G (slabs staged once for all three components, several
+    z-levels accumulated at a time) sums every face in the same order as the
+    face-by-face gather, so the two agree bit for bit; a 5000-particle clump makes some
+    slabs overflow the staging area (several passes)."""
     sc = scenes.cavity_scene("c2")
     g = np.random.default_rng(5)
     out = []
     for perface in ("0", "1"):
+        monkeypatch.setenv("SDR_P2G_PERFACE", perface)  # read when the solver is made
         d = scenes.make_solver(sc, precision=precision, **FAST)
         if not out:
             pos = scenes.fill_particles(d, sc, d.cells().reshape(d.nz, d.ny, d.nx) != 0, stride=1)
@@ -140,7 +141,6 @@ def test_p2g_tiled_matches_face_gather(dev, precision, monkeypatch):
             pos = np.concatenate([pos, clump])
             vel = g.standard_normal(pos.shape)
         d.set_particles(pos, vel)
-        monkeypatch.setenv("SDR_P2G_PERFACE", perface)
         d.run_stage("classify", sc.dt)
         d.run_stage("p2g", sc.dt)
         out.append(d.grid())
