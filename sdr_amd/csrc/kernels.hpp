@@ -165,7 +165,8 @@ struct RedistDirs {
   double d[6];
 };
 // Bucket-based decision pass (redist.hip).  ctr (8 words): [0] sources, [1] bucket
-// entries, [2]/[3] pending-list lengths, [4] large buckets, [6..7] moves (u64).
+// entries, [2]/[3] pending-list lengths, [4] large buckets, [5] the largest source count
+// above what one workgroup sorts (else 0), [6..7] moves (u64).  max_count is ctr[5].
 void launch_rd_prepare(const uint32_t* count, const uint32_t* sidx, const uint32_t* boff, int64_t nc,
                        uint32_t cap, uint32_t* src_cell, uint32_t* src_off, uint32_t* src_head,
                        uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend, uint32_t* ctr,
@@ -174,7 +175,7 @@ void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n,
                        const uint32_t* sidx, int64_t nsrc, const uint32_t* src_cell,
                        const uint32_t* src_off, uint32_t* src_fill, const uint32_t* rank,
                        uint32_t* brank, uint32_t* bslot, uint32_t* trank, uint32_t* tslot,
-                       uint32_t* big, uint32_t* ctr, hipStream_t st);
+                       uint32_t* big, uint32_t* ctr, uint32_t max_count, hipStream_t st);
 // One reservation round; parity selects which pending list is read (0: a, 1: b).
 void launch_rd_round2(int64_t max_pending, int parity, bool first, uint32_t* pend_a, uint32_t* pend_b,
                       const uint32_t* src_cell, const uint32_t* src_head, const uint32_t* src_lim,

@@ -38,7 +38,9 @@ namespace sdr {
 namespace {
 
 constexpr int kRdBlock = 256;
-constexpr uint32_t kBig = 64;  // buckets above this size are ordered by a whole wave
+constexpr uint32_t kBig = 64;         // buckets above this size are sorted by a workgroup
+constexpr uint32_t kSortRun = 8192;   // entries a workgroup sorts in LDS (64 KB of keys)
+constexpr int kSortThreads = 1024;
 constexpr int kRun = 8;        // bucket ranks a committing source loads at once
 
 inline unsigned rd_grid(int64_t n, int block = kRdBlock) {
@@ -127,7 +129,7 @@ __global__ void k_rd_totals(const uint32_t* __restrict__ count, const uint32_t* 
 __global__ void k_rd_src_init(const uint32_t* __restrict__ count, const uint32_t* __restrict__ sidx,
                               const uint32_t* __restrict__ boff, int64_t nc, uint32_t cap,
                               uint32_t* src_cell, uint32_t* src_off, uint32_t* src_head,
-                              uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend) {
+                              uint32_t* src_lim, uint32_t* src_fill, uint32_t* pend, uint32_t* ctr5) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const uint32_t cnt = count[c];
@@ -139,6 +141,7 @@ __global__ void k_rd_src_init(const uint32_t* __restrict__ count, const uint32_t
   src_lim[s] = off + (cnt - cap);
   src_fill[s] = 0;
   pend[s] = s;
+  if (cnt > kSortRun) atomicMax(&ctr5[0], cnt);  // buckets that need merge passes (rare)
 }
 
 // Every particle of a source cell takes a slot of its cell's bucket.  Particles are
@@ -176,7 +179,8 @@ __global__ void k_rd_bucket(const int32_t* __restrict__ cellkey, const uint32_t*
 }
 
 // The first count - cap entries of each bucket in reference order (partial selection;
-// the order of the rest does not matter).  Buckets over kBig go to a wave each.
+// the order of the rest does not matter).  Buckets over kBig are sorted in full
+// afterwards.
 __global__ void k_rd_select(int64_t nsrc, const uint32_t* __restrict__ src_cell,
                             const uint32_t* __restrict__ src_off, const uint32_t* __restrict__ count,
                             uint32_t cap, uint32_t* brank, uint32_t* bslot, uint32_t* big,
@@ -205,30 +209,84 @@ __global__ void k_rd_select(int64_t nsrc, const uint32_t* __restrict__ src_cell,
   }
 }
 
-// Large buckets: one wave sorts the whole bucket by rank (each entry's position is the
-// number of smaller ranks), through a scratch copy.
-__global__ void __launch_bounds__(64) k_rd_sort_big(const uint32_t* __restrict__ nbig,
-                                                    const uint32_t* __restrict__ big,
-                                                    const uint32_t* __restrict__ src_cell,
-                                                    const uint32_t* __restrict__ src_off,
-                                                    const uint32_t* __restrict__ count, uint32_t* brank,
-                                                    uint32_t* bslot, uint32_t* trank, uint32_t* tslot) {
+// Large buckets are sorted by rank in full (the selection needs only the first
+// count - cap entries, which in a clump is nearly all of them).  Ranks are unique, so the
+// sorted order is fixed whatever the network.
+//   * first, runs of up to kSortRun entries of every large bucket are sorted in LDS: a
+//     bitonic network over (rank << 32 | slot), padded to a power of two;
+//   * buckets longer than a run are then merged pass by pass, runs of width w into runs
+//     of 2w, between the bucket arrays and a scratch copy: an entry's place in the merged
+//     run is its index in its own run plus the number of smaller ranks in the partner run
+//     (a binary search).  An odd number of passes ends with a copy back.
+__global__ void __launch_bounds__(kSortThreads) k_rd_sort_runs(const uint32_t* __restrict__ nbig,
+                                                               const uint32_t* __restrict__ big,
+                                                               const uint32_t* __restrict__ src_cell,
+                                                               const uint32_t* __restrict__ src_off,
+                                                               const uint32_t* __restrict__ count,
+                                                               uint32_t* brank, uint32_t* bslot) {
+  __shared__ unsigned long long key[kSortRun];
   const uint32_t nb = *nbig;
   for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const uint32_t s = big[b], off = src_off[s], cnt = count[src_cell[s]];
-    for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
-      const uint32_t r = brank[off + e];
-      uint32_t p = 0;
-      for (uint32_t f = 0; f < cnt; ++f) p += brank[off + f] < r ? 1u : 0u;
-      trank[off + p] = r;
-      tslot[off + p] = bslot[off + e];
+    for (uint32_t r0 = 0; r0 < cnt; r0 += kSortRun) {
+      const uint32_t len = cnt - r0 < kSortRun ? cnt - r0 : kSortRun;
+      uint32_t P = 64;
+      while (P < len) P <<= 1;
+      for (uint32_t i = threadIdx.x; i < P; i += kSortThreads)
+        key[i] = i < len ? ((unsigned long long)brank[off + r0 + i] << 32) | bslot[off + r0 + i] : ~0ull;
+      __syncthreads();
+      for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < P; i += kSortThreads) {
+            const uint32_t l = i ^ j;
+            if (l > i) {
+              const unsigned long long a = key[i], c = key[l];
+              if ((a > c) == ((i & k) == 0)) {  // ascending where bit k of i is clear
+                key[i] = c;
+                key[l] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t i = threadIdx.x; i < len; i += kSortThreads) {
+        const unsigned long long v = key[i];
+        brank[off + r0 + i] = (uint32_t)(v >> 32);
+        bslot[off + r0 + i] = (uint32_t)v;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < cnt; e += 64) {
-      brank[off + e] = trank[off + e];
-      bslot[off + e] = tslot[off + e];
+  }
+}
+
+// One merge pass over the buckets longer than kSortRun: sorted runs of width w become
+// sorted runs of width 2w in the output arrays (w = 0: a plain copy).
+__global__ void __launch_bounds__(kSortThreads) k_rd_merge_runs(
+    const uint32_t* __restrict__ nbig, const uint32_t* __restrict__ big, const uint32_t* __restrict__ src_cell,
+    const uint32_t* __restrict__ src_off, const uint32_t* __restrict__ count, const uint32_t* __restrict__ rin,
+    const uint32_t* __restrict__ sin, uint32_t* __restrict__ rout, uint32_t* __restrict__ sout, uint32_t w) {
+  const uint32_t nb = *nbig;
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const uint32_t s = big[b], off = src_off[s], cnt = count[src_cell[s]];
+    if (cnt <= kSortRun) continue;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kSortThreads) {
+      const uint32_t r = rin[off + i];
+      uint32_t dst = i;
+      if (w) {
+        const uint32_t run = i / w, pair = (run & ~1u) * w;
+        const uint32_t pb = (run ^ 1u) * w;  // the partner run
+        uint32_t lo = pb, hi = pb < cnt ? (pb + w < cnt ? pb + w : cnt) : pb;
+        while (lo < hi) {  // first entry of the partner run with a rank above r
+          const uint32_t m = (lo + hi) >> 1;
+          if (rin[off + m] < r) lo = m + 1;
+          else hi = m;
+        }
+        dst = pair + (i - run * w) + (lo - pb);
+      }
+      rout[off + dst] = r;
+      sout[off + dst] = sin[off + i];
     }
-    __syncthreads();
   }
 }
 
@@ -407,21 +465,31 @@ void launch_rd_prepare(const uint32_t* count, const uint32_t* sidx, const uint32
                        hipStream_t st) {
   k_rd_totals<<<1, 1, 0, st>>>(count, sidx, boff, nc, cap, ctr);
   k_rd_src_init<<<rd_grid(nc), kRdBlock, 0, st>>>(count, sidx, boff, nc, cap, src_cell, src_off, src_head,
-                                                  src_lim, src_fill, pend);
+                                                  src_lim, src_fill, pend, ctr + 5);
 }
 
 void launch_rd_buckets(const int32_t* cellkey, const uint32_t* count, int64_t n, uint32_t cap,
                        const uint32_t* sidx, int64_t nsrc, const uint32_t* src_cell,
                        const uint32_t* src_off, uint32_t* src_fill, const uint32_t* rank,
                        uint32_t* brank, uint32_t* bslot, uint32_t* trank, uint32_t* tslot,
-                       uint32_t* big, uint32_t* ctr, hipStream_t st) {
+                       uint32_t* big, uint32_t* ctr, uint32_t max_count, hipStream_t st) {
   if (n > 0)
     k_rd_bucket<<<rd_grid(n), kRdBlock, 0, st>>>(cellkey, count, n, cap, sidx, src_off, src_fill, rank,
                                                  brank, bslot);
   if (nsrc <= 0) return;
   k_rd_select<<<rd_grid(nsrc), kRdBlock, 0, st>>>(nsrc, src_cell, src_off, count, cap, brank, bslot, big,
                                                   ctr + 4);
-  k_rd_sort_big<<<256, 64, 0, st>>>(ctr + 4, big, src_cell, src_off, count, brank, bslot, trank, tslot);
+  k_rd_sort_runs<<<512, kSortThreads, 0, st>>>(ctr + 4, big, src_cell, src_off, count, brank, bslot);
+  int passes = 0;
+  for (uint64_t w = kSortRun; w < max_count; w <<= 1, ++passes) {
+    const bool fwd = (passes & 1) == 0;
+    k_rd_merge_runs<<<512, kSortThreads, 0, st>>>(ctr + 4, big, src_cell, src_off, count, fwd ? brank : trank,
+                                                  fwd ? bslot : tslot, fwd ? trank : brank, fwd ? tslot : bslot,
+                                                  (uint32_t)w);
+  }
+  if (passes & 1)
+    k_rd_merge_runs<<<512, kSortThreads, 0, st>>>(ctr + 4, big, src_cell, src_off, count, trank, tslot, brank,
+                                                  bslot, 0u);
 }
 
 void launch_rd_round2(int64_t max_pending, int parity, bool first, uint32_t* pend_a, uint32_t* pend_b,

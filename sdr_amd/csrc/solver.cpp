@@ -1225,16 +1225,17 @@ struct Solver : SolverBase {
                       rd_src_cell.as<uint32_t>(), rd_src_off.as<uint32_t>(), rd_src_head.as<uint32_t>(),
                       rd_src_lim.as<uint32_t>(), rd_src_fill.as<uint32_t>(), rd_pend_a.as<uint32_t>(), ctr, st);
     uint32_t* hc = (uint32_t*)pin.p;
-    HIPCHK(hipMemcpyAsync(hc, ctr, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hc, ctr, 24, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const int64_t nsrc = hc[0], nact = hc[1];
+    const uint32_t max_count = hc[5];  // of the sources too large to sort in one workgroup, else 0
     if (nsrc == 0) return {0, 0};
     uint32_t* brank = act_rank.as<uint32_t>();
     uint32_t* bslot = act_slot.as<uint32_t>();
     launch_rd_buckets(cellkey.as<int32_t>(), cnt, n, capc, rd_sidx.as<uint32_t>(), nsrc,
                       rd_src_cell.as<uint32_t>(), rd_src_off.as<uint32_t>(), rd_src_fill.as<uint32_t>(),
                       part[cur][6].as<uint32_t>(), brank, bslot, act_rank_sorted.as<uint32_t>(),
-                      act_slot_sorted.as<uint32_t>(), rd_big.as<uint32_t>(), ctr, st);
+                      act_slot_sorted.as<uint32_t>(), rd_big.as<uint32_t>(), ctr, max_count, st);
     HIPCHK(hipMemsetAsync(dest.p, 0xff, (size_t)nact * 4, st));
     // Each round commits at least the lowest pending candidate, so the candidates (at
     // most nact) bound the rounds; the pending count is read every few rounds and sizes

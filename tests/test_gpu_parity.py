@@ -370,10 +370,12 @@ def _random_clumps(solver, seed, lo=0, hi=16):
 @pytest.mark.parametrize("sequential", ["0", "1"])
 def test_redistribute_decision_paths_bit_exact(dev, orc, sequential, monkeypatch):
     """The order-dependent redistribution pass gives the oracle's bits both by the
-    parallel reservation rounds and by the single-thread reference loop: one packed cell,
-    random clumps on the box and on the 36x42x34 cavity, and four full substeps."""
+    parallel reservation rounds and by the single-thread reference loop: one packed cell
+    (12 particles; 100 and 3000, whose buckets are sorted in one workgroup; 12000 and
+    20000, which also take one and two merge passes), random clumps on the box and on
+    the 36x42x34 cavity, and four full substeps."""
     monkeypatch.setenv("SDR_REDIST_SEQUENTIAL", sequential)
-    for n in (12, 100):
+    for n in (12, 100, 3000, 12000, 20000):
         sc = scenes.box_scene()
         d = scenes.make_solver(sc)
         o = scenes.make_solver(sc, lib=orc)
