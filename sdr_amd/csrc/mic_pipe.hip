@@ -620,12 +620,14 @@ __device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
 
 // ------------------------------------------------------------------ lean solve sweeps with an I/O wave
 // The solves again, with the per-step work of the compute waves cut to the cell itself:
-//   * TK compute waves (one plane each) plus one I/O wave per workgroup.  The I/O wave
-//     does all the cross-tile traffic: it fetches (a step ahead, from its prefetch ring)
-//     the z halo row of the tile's boundary plane and the x halo value of every plane,
-//     and publishes (a step late, from LDS) the tile's last plane and the edge lane of
-//     every plane.  Compute waves never wait on halo words and never issue
-//     write-through stores (which would hold back their own prefetch waits).
+//   * TK compute waves (one plane each) plus an I/O wave and a publish wave per
+//     workgroup, which do all the cross-tile traffic.  The I/O wave fetches (a step
+//     ahead, from its prefetch ring) the z halo row of the tile's boundary plane and the
+//     x halo value of every plane; the publish wave publishes (a step late, from LDS)
+//     the tile's last plane and the edge lane of every plane.  A wave's wait for a load
+//     also waits for the stores it issued before that load, so the halo loads and the
+//     write-through halo stores are kept in different waves, and compute waves never
+//     wait on halo words and never issue write-through stores.
 //   * Cell loads and stores address a buffer descriptor per plane (the plane's sheared
 //     slots, NJ * nx values): a lane's offset is (J * nx + i) * size, so rows before the
 //     plane (J < 0) wrap to huge unsigned offsets and rows past it exceed the record
@@ -637,7 +639,7 @@ __device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
 // (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
 // workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
 template <class T, bool REV, int TK, bool DOT, int PFH>
-__global__ void __launch_bounds__(64 * (TK + 1))
+__global__ void __launch_bounds__(64 * (TK + 2))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
                  MicPipeArgs<T> pa) {
@@ -661,7 +663,6 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   const int lane = threadIdx.x & 63;
   // wave-uniform (scalar) so the per-plane buffer descriptors live in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const bool io = wave >= TK;
   const int nti = pa.ntile_i, ntk = pa.ntile_k;
   int itl, ktl;
   tile_of(blockIdx.x, nti, ntk, &itl, &ktl);
@@ -680,7 +681,7 @@ __global__ void __launch_bounds__(64 * (TK + 1))
   const int lwave = REV ? 0 : TK - 1;       // last wave: the tile after takes its z neighbour from it
   const int edge_in = REV ? 63 : 0, edge_out = REV ? 0 : 63;
 
-  if (!io) {
+  if (wave < TK) {
     // ------------------------------------------------------------ compute wave
     const int wk = wave;
     const int k = kt * TK + wk;
@@ -790,30 +791,71 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     return;
   }
 
-  // -------------------------------------------------------------- I/O wave(s)
-  // one wave serves both halos (DZ: the z halo, DX: the x halo; splitting them over two
-  // waves measured slower)
-  {
-  constexpr bool DZ = true, DX = true;
-  // the I/O wave's step is the longest of the workgroup's: let its instructions issue
-  // first on the SIMD it shares with a compute wave
-  __builtin_amdgcn_s_setprio(3);
   const rsrc_t rhx = make_rsrc(pa.halo_x, pa.hx_bytes), rhz = make_rsrc(pa.halo_z, pa.hz_bytes);
-  const bool z_in = DZ && (REV ? kt + 1 < ntk : kt > 0);
-  const bool z_out = DZ && (REV ? kt > 0 : kt + 1 < ntk);
-  const bool x_in = DX && (REV ? it + 1 < nti : it > 0);
-  const bool x_out = DX && (REV ? it > 0 : it + 1 < nti);
-  const int zb_in = REV ? kt : kt - 1, zb_out = REV ? kt - 1 : kt;
-  const int xb_in = REV ? it : it - 1, xb_out = REV ? it - 1 : it;
   const int zrow = W * nxp;
-  const int zh_in = zb_in * NJ * zrow + i, zh_out = zb_out * NJ * zrow + i;
   // lane l < TK serves plane kt * TK + l for the x halo
   const int kl = kt * TK + lane;
   const bool xl = lane < TK && kl < nz;
+  if (wave == TK + 1) {
+    // ------------------------------------------------------------ publish wave
+    // Publishes the values compute waves wrote at step s - 1 (read from LDS at step s):
+    // the last plane's row for the tile after in k, the edge lane of every plane for the
+    // tile after in i.  Its stores are kept out of the I/O wave, whose waits for
+    // prefetched halo words would otherwise also wait for these stores to complete.
+    const bool z_out = REV ? kt > 0 : kt + 1 < ntk;
+    const bool x_out = REV ? it > 0 : it + 1 < nti;
+    const int zb_out = REV ? kt - 1 : kt, xb_out = REV ? it - 1 : it;
+    const int zh_out = zb_out * NJ * zrow + i;
+    const int xh_out = (xb_out * nz + (xl ? kl : 0)) * NJ * W;
+    const bool zl_ok = kt * TK + lwave < nz;
+    auto xpub_off = [&](int R) -> uint32_t {
+      return (x_out & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_out + R * W) * 8u : kOob;
+    };
+    auto publish_at = [&](uint32_t oz, uint32_t ox, T vz, T vx) {
+      uint32_t wz[W], wx[W];
+      __builtin_memcpy(wz, &vz, sizeof(T));
+      __builtin_memcpy(wx, &vx, sizeof(T));
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        bst64<kWriteThrough>(rhz, oz == kOob ? kOob : oz + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wz[q]);
+        bst64<kWriteThrough>(rhx, ox == kOob ? kOob : ox + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wx[q]);
+      }
+    };
+    // rows of the step's publish (step s - 1), advanced by dJ per step
+    int rzp = row(lwave, -1), rxp = row(lane < TK ? lane : 0, -1);
+    const uint32_t dz8 = (uint32_t)(dJ * zrow) * 8u;
+    uint32_t ozp = (uint32_t)(zh_out + rzp * zrow) * 8u;
+    __syncthreads();  // the prologue barrier
+    for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int pb = (u & 1) ^ 1;
+        const T vz = zbuf[pb][lwave][lane];
+        const T vx = zbuf[pb][lane < TK ? lane : 0][edge_out];
+        publish_at((z_out & zl_ok & ((unsigned)rzp < (unsigned)NJ)) ? ozp : kOob, xpub_off(rxp), vz, vx);
+        rzp += dJ, rxp += dJ;
+        ozp += dz8;
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();  // matches the compute waves' final barrier
+    return;
+  }
+
+  // -------------------------------------------------------------- I/O wave
+  // Fetches the halo inputs: the z halo row of the boundary wave and the x halo value of
+  // every plane, PFH steps ahead, into LDS a step ahead of their use.
+  {
+  // the I/O wave's step is the longest of the workgroup's: let its instructions issue
+  // first on the SIMD it shares with a compute wave
+  __builtin_amdgcn_s_setprio(3);
+  const bool z_in = REV ? kt + 1 < ntk : kt > 0;
+  const bool x_in = REV ? it + 1 < nti : it > 0;
+  const int zb_in = REV ? kt : kt - 1;
+  const int xb_in = REV ? it : it - 1;
+  const int zh_in = zb_in * NJ * zrow + i;
   const int xh_in = (xb_in * nz + (xl ? kl : 0)) * NJ * W;
-  const int xh_out = (xb_out * nz + (xl ? kl : 0)) * NJ * W;
-  const int klast = kt * TK + lwave;
-  const bool zl_ok = klast < nz;
   // Inputs of target step t: the z halo row of the boundary wave at step t, and the x
   // neighbour row of plane `lane` at step t (its row at step t, minus dJ).
   auto zrow_of = [&](int t) { return row(bwave, t); };
@@ -838,8 +880,8 @@ __global__ void __launch_bounds__(64 * (TK + 1))
     oxr[u] = ox;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-      if (DZ) hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
-      if (DX) hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
+      hz[u][q] = bld64<kAgent>(rhz, oz + (uint32_t)(q * nxp) * 8u);
+      hx[u][q] = bld64<kAgent>(rhx, ox + (uint32_t)q * 8u);
     }
   };
   auto fill = [&](int u, int t) { fill_at(u, zoff(zrow_of(t)), xoff(xrow_of(t))); };
@@ -879,86 +921,38 @@ __global__ void __launch_bounds__(64 * (TK + 1))
       settle(hx[u], rhx, ox, 8u);
       __builtin_amdgcn_s_waitcnt(0x0F70);
     }
-    if (DZ) zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
-    if (DX && lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
+    zbuf[b][TK][lane] = oz == kOob ? T(0) : words_value<T>(hz[u]);
+    if (lane < TK) xlds[b][lane] = ox == kOob ? T(0) : words_value<T>(hx[u]);
   };
-  // Publish the values compute waves wrote at step t (vz: the last plane's row, vx: the
-  // edge lane of plane `lane`, both read from LDS at the top of the step).  The stores
-  // are issued on every step (to an out-of-range offset when there is nothing to
-  // publish): with a fixed number of memory operations per step, the wait for a
-  // prefetched halo word counts exactly the operations issued after it.
-  auto zpub_off = [&](int R) -> uint32_t {
-    return (z_out & zl_ok & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(zh_out + R * zrow) * 8u : kOob;
-  };
-  auto xpub_off = [&](int R) -> uint32_t {
-    return (x_out & xl & ((unsigned)R < (unsigned)NJ)) ? (uint32_t)(xh_out + R * W) * 8u : kOob;
-  };
-  auto publish_at = [&](uint32_t oz, uint32_t ox, T vz, T vx) {
-    if (DZ) {
-      const T v = vz;
-      uint32_t wv[W];
-      __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = oz;
+  zbuf[0][TK][lane] = T(0);
+  if (lane < TK) xlds[0][lane] = T(0);
+  // The prologue issues the loads of steps 0 .. PFH in the order of the step loop, so a
+  // wait at the top of the loop holds for every way into it.
 #pragma unroll
-      for (int q = 0; q < W; ++q)
-        bst64<kWriteThrough>(rhz, o == kOob ? kOob : o + (uint32_t)(q * nxp) * 8u, ((uint64_t)tag << 32) | wv[q]);
-    }
-    if (DX) {
-      const T v = vx;
-      uint32_t wv[W];
-      __builtin_memcpy(wv, &v, sizeof(T));
-      const uint32_t o = ox;
-#pragma unroll
-      for (int q = 0; q < W; ++q)
-        bst64<kWriteThrough>(rhx, o == kOob ? kOob : o + (uint32_t)q * 8u, ((uint64_t)tag << 32) | wv[q]);
-    }
-  };
-  auto publish = [&](int t, T vz, T vx) {
-    publish_at(zpub_off(row(lwave, t)), xpub_off(row(lane < TK ? lane : 0, t)), vz, vx);
-  };
-  if (DZ) zbuf[0][TK][lane] = T(0);
-  if (DX && lane < TK) xlds[0][lane] = T(0);
-  // The prologue issues the memory operations of steps -PFH-1 .. -1 in the order of the
-  // step loop (the stores of those steps have no row, so nothing is stored): a wait
-  // at the top of the loop, which holds for every way into it, is then the same as in
-  // the steady state.
-  fill(0, 0);
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int u = 1; u < PFH; ++u) {
-    publish(-1, T(0), T(0));
+  for (int u = 0; u < PFH; ++u) {
     fill(u, u);
     __builtin_amdgcn_sched_barrier(0);
   }
   take(0, 0, 1);  // inputs of step 0
-  publish(-1, T(0), T(0));
   fill(0, PFH);
   __syncthreads();
   __builtin_amdgcn_sched_barrier(0);
-  // rows of the step's publish (step s - 1) and prefetch (target s + 1 + PFH), advanced
-  // by dJ per step: the offsets then cost a compare and a select each
-  int rzp = row(lwave, -1), rxp = row(lane < TK ? lane : 0, -1);
+  // the row of the step's prefetch (target s + 1 + PFH), advanced by dJ per step; the z
+  // offset's row term is advanced with it (no multiply in the step)
   int rzf = zrow_of(1 + PFH), rxf = xrow_of(1 + PFH);
-  // the z offsets' row terms, advanced with the rows (no multiply in the step)
   const uint32_t dz8 = (uint32_t)(dJ * zrow) * 8u;
-  uint32_t ozp = (uint32_t)(zh_out + rzp * zrow) * 8u, ozf = (uint32_t)(zh_in + rzf * zrow) * 8u;
+  uint32_t ozf = (uint32_t)(zh_in + rzf * zrow) * 8u;
   for (int s0 = 0; s0 < nsteps_pad; s0 += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      const int cb = u & 1, pb = cb ^ 1;
-      // values of step s - 1 (LDS buffer pb; step -1 has no row in the grid, so nothing
-      // is stored), inputs of step s + 1 (into buffer cb).  The two LDS reads go out
-      // first and complete while the inputs are checked and written to LDS.
-      const T vz = DZ ? zbuf[pb][lwave][lane] : T(0);
-      const T vx = DX ? zbuf[pb][lane < TK ? lane : 0][edge_out] : T(0);
-      __builtin_amdgcn_sched_barrier(0);
+      const int cb = u & 1;
+      // inputs of step s + 1 (into buffer cb)
       const int un = (u + 1) % PFH;
       take(un, s + 1, cb);
-      publish_at((z_out & zl_ok & ((unsigned)rzp < (unsigned)NJ)) ? ozp : kOob, xpub_off(rxp), vz, vx);
       fill_at(un, (z_in & ((unsigned)rzf < (unsigned)NJ)) ? ozf : kOob, xoff(rxf));
-      rzp += dJ, rxp += dJ, rzf += dJ, rxf += dJ;
-      ozp += dz8, ozf += dz8;
+      rzf += dJ, rxf += dJ;
+      ozf += dz8;
       __syncthreads();
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -973,11 +967,11 @@ void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, con
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
   if (op == kMicFwd)
-    k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
-    k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 1), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+    k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }
 
 template <class T, int OP, int TI, int TK>
@@ -1005,8 +999,8 @@ void dispatch_op(int op, Shear sh, const uint16_t* flags, T* precon, const T* in
 // Tile shapes.  Variants 0 and 1: k_mic_pipe (factor) / k_mic_solve tiles of 64 values
 // of i by 16 and 8 planes, one plane per wave (the factor's shapes: 16 planes for f32,
 // 8 for f64, whose 8-deep f64 ring needs the registers).  Variant 2 (the default solve):
-// k_mic_sweep3 tiles of 64 values of i by 7 planes plus the I/O wave, halo prefetch
-// depth 4.  Other tile shapes (multi-plane waves, 3-15 planes per tile, deeper rings,
+// k_mic_sweep3 tiles of 64 values of i by 7 planes plus the I/O and publish waves, halo
+// prefetch depth 4 (2 and 8 measured slower).  Other tile shapes (multi-plane waves, 3-15 planes per tile, deeper rings,
 // two I/O waves, XCD-placed tiles) were measured slower on 256^3 and 512x512x384 and
 // are not built.
 namespace {
