@@ -207,6 +207,11 @@ int sdr_solver_get_cells(sdr_solver* s, uint8_t* cells);
 int sdr_solver_set_cells(sdr_solver* s, const uint8_t* cells);
 /* `solver.pressure` (solver.rs:163). */
 int sdr_solver_get_pressure(sdr_solver* s, double* p);
+/* The device's own particle order, for parity tests of its cell sort: the reference
+ * index (`rank`) of each particle slot, in slot order (after `classify`: by clamped cell,
+ * then reference index in exact mode, previous slot otherwise).  No counterpart in the
+ * reference, which keeps its particles in reference order. */
+int sdr_solver_get_slot_ranks(sdr_solver* s, uint32_t* ranks, uint64_t n);
 int sdr_solver_get_state(const sdr_solver* s, sdr_state* st);
 int sdr_solver_set_state(sdr_solver* s, const sdr_state* st);
 

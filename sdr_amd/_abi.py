@@ -216,6 +216,7 @@ DEVICE_ONLY = {
     "solver_get_timing": [P, C.c_char_p, C.c_size_t, c_double_p, c_u64_p, C.c_uint64, c_u64_p],
     "solver_seed_fill": [P, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Vec3),
                          c_u64_p],
+    "solver_get_slot_ranks": [P, C.POINTER(C.c_uint32), C.c_uint64],
     "solver_footprint": [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, c_u64_p],
     "serve": [C.POINTER(Scenario), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(ScenarioResult)],
     # device pointers (u, v, w, cells, source, pressure) and a hipStream_t

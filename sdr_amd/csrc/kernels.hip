@@ -30,22 +30,7 @@ __device__ inline V3<T> load_vel(const Parts<T>& p, int64_t t) {
   return mk3(p.vx[t], p.vy[t], p.vz[t]);
 }
 
-// ------------------------------------------------------------------ cell keys + sort glue
-// Key = the clamped cell (every particle scatters through it, grid.rs:52-88); the exact
-// mode appends the reference index so each cell's segment is in reference order.
-template <class T>
-__global__ void k_keys(Parts<T> p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
-                       uint32_t* vals, int exact) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t c = g.clamped_cell(load_pos(p, t));
-  if (exact)
-    keys64[t] = ((uint64_t)c << 32) | (uint64_t)p.rank[t];
-  else
-    keys[t] = (uint32_t)c;
-  vals[t] = (uint32_t)t;
-}
-
+// ------------------------------------------------------------------ sort glue
 template <class T>
 __global__ void k_gather_parts(Parts<T> s, Parts<T> d, const uint32_t* __restrict__ perm,
                                int64_t n) {
@@ -59,18 +44,6 @@ __global__ void k_gather_parts(Parts<T> s, Parts<T> d, const uint32_t* __restric
   d.vy[t] = s.vy[q];
   d.vz[t] = s.vz[q];
   d.rank[t] = s.rank[q];
-}
-
-__global__ void k_cell_ranges(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ keys64,
-                              int64_t n, uint32_t* cstart, uint32_t* cend) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  auto cell = [&](int64_t q) -> uint32_t {
-    return keys64 ? (uint32_t)(keys64[q] >> 32) : keys[q];
-  };
-  uint32_t c = cell(t);
-  if (t == 0 || cell(t - 1) != c) cstart[c] = (uint32_t)t;
-  if (t == n - 1 || cell(t + 1) != c) cend[c] = (uint32_t)(t + 1);
 }
 
 // ------------------------------------------------------------------ classify (solver.rs:342-371)
@@ -1401,22 +1374,10 @@ __global__ void k_cavity_mask(GridD<T> g, SdfD<T> sdf, uint8_t* mask) {
 
 // ================================================================== launch wrappers
 template <class T>
-void launch_keys(const Parts<T>& p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
-                 uint32_t* vals, bool exact, hipStream_t st) {
-  if (n <= 0) return;
-  k_keys<T><<<grid_for(n), kBlock, 0, st>>>(p, n, g, keys, keys64, vals, exact ? 1 : 0);
-}
-template <class T>
 void launch_gather_parts(const Parts<T>& s, const Parts<T>& d, const uint32_t* perm, int64_t n,
                          hipStream_t st) {
   if (n <= 0) return;
   k_gather_parts<T><<<grid_for(n), kBlock, 0, st>>>(s, d, perm, n);
-}
-void launch_cell_ranges(const uint32_t* keys, const uint64_t* keys64, int64_t n, int64_t ncells,
-                        uint32_t* cstart, uint32_t* cend, hipStream_t st) {
-  (void)ncells;
-  if (n <= 0) return;
-  k_cell_ranges<<<grid_for(n), kBlock, 0, st>>>(keys, keys64, n, cstart, cend);
 }
 template <class T>
 void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
@@ -1734,8 +1695,6 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
 }
 
 #define SDR_INST(T)                                                                                \
-  template void launch_keys<T>(const Parts<T>&, int64_t, GridD<T>, uint32_t*, uint64_t*, uint32_t*, \
-                               bool, hipStream_t);                                                 \
   template void launch_gather_parts<T>(const Parts<T>&, const Parts<T>&, const uint32_t*, int64_t,  \
                                        hipStream_t);                                               \
   template void launch_classify<T>(const uint8_t*, const uint32_t*, const uint32_t*, const Parts<T>&, \

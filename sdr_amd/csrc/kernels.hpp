@@ -49,13 +49,8 @@ struct SolveCfg {
 
 // ---- launches (all asynchronous on `st`)
 template <class T>
-void launch_keys(const Parts<T>& p, int64_t n, GridD<T> g, uint32_t* keys, uint64_t* keys64,
-                 uint32_t* vals, bool exact, hipStream_t st);
-template <class T>
 void launch_gather_parts(const Parts<T>& src, const Parts<T>& dst, const uint32_t* perm, int64_t n,
                          hipStream_t st);
-void launch_cell_ranges(const uint32_t* keys, const uint64_t* keys64, int64_t n, int64_t ncells,
-                        uint32_t* cstart, uint32_t* cend, hipStream_t st);
 template <class T>
 void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
                      const Parts<T>& p, GridD<T> g, BoxD<T> outlet, uint8_t* cells,

@@ -859,8 +859,8 @@ struct Solver : SolverBase {
   hipStream_t st = nullptr;
   Timer timer;
   Buf part[2][7];
-  Buf keys, keys_alt, vals, vals_alt, keys64, keys64_alt;
-  Buf cstart, cend, count, mask, static_cls, cells;
+  Buf keys, keys_alt, vals, vals_alt, skey_t, sort_big, sort_ctr;
+  Buf cstart, cend, count, sort_cursor, mask, static_cls, cells;
   Buf fu, fv, fw, su, sv, sw, ftmp, known0, known1, pressure, sdf_data;
   Buf temp;
   Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
@@ -947,7 +947,7 @@ struct Solver : SolverBase {
       for (size_t t = 0; t < nsdf; ++t) tmp[t] = (T)s->data[t];
       HIPCHK(hipMemcpy(sdf_data.p, tmp.data(), nsdf * sizeof(T), hipMemcpyHostToDevice));
     }
-    for (Buf* b : {&cstart, &cend, &count}) b->alloc((size_t)nc * 4);
+    for (Buf* b : {&cstart, &cend, &count, &sort_cursor}) b->alloc((size_t)nc * 4);
     for (Buf* b : {&mask, &static_cls, &cells}) b->alloc((size_t)nc);
     for (Buf* b : {&fu, &su}) b->alloc((size_t)nu() * sizeof(T));
     for (Buf* b : {&fv, &sv}) b->alloc((size_t)nv() * sizeof(T));
@@ -1001,13 +1001,11 @@ struct Solver : SolverBase {
                    &culled_sorted, &cellkey, &act_rank, &act_slot, &act_rank_sorted,
                    &act_slot_sorted, &dest})
       b->alloc((size_t)nc * 4);
-    if (opts.dot_mode == SDR_DOT_EXACT) {
-      keys64.alloc((size_t)nc * 8);
-      keys64_alt.alloc((size_t)nc * 8);
-    }
-    size_t tb = std::max({sort_pairs_u32_temp(nc), sort_pairs_u64_temp(nc), sort_keys_u32_temp(nc),
-                          scan_u32_temp(nc), scan_over_cap_temp(ncells())});
-    temp.alloc(tb);
+    if (opts.dot_mode == SDR_DOT_EXACT) skey_t.alloc((size_t)nc * 4);
+    // segments longer than a thread's register sort: each holds more than 16 particles
+    sort_big.alloc((size_t)(nc / 16 + 1) * 4);
+    if (!sort_ctr.p) sort_ctr.alloc(64);
+    temp.alloc(scan_u32_temp(std::max<int64_t>(nc, ncells())));
     cap = nc;
   }
 
@@ -1055,31 +1053,20 @@ struct Solver : SolverBase {
     sorted_valid = false;
   }
 
-  // Stable sort of the particles by (clamped) cell; cell segments [cstart, cend).
+  // Sort of the particles by (clamped) cell; cell segments [cstart, cend).  Inside a
+  // segment: reference order in exact mode, the previous order otherwise (stable).
   void sort_particles() {
     Scope sc_(timer, "sort");
-    const int64_t nc = ncells();
-    HIPCHK(hipMemsetAsync(cstart.p, 0, (size_t)nc * 4, st));
-    HIPCHK(hipMemsetAsync(cend.p, 0, (size_t)nc * 4, st));
+    const CellSortBufs b{keys.as<uint32_t>(),   vals.as<uint32_t>(),   keys_alt.as<uint32_t>(),
+                         vals_alt.as<uint32_t>(), skey_t.as<uint32_t>(), cstart.as<uint32_t>(),
+                         cend.as<uint32_t>(),   sort_cursor.as<uint32_t>(), sort_big.as<uint32_t>(),
+                         sort_ctr.as<uint32_t>()};
+    cell_sort<T>(parts(cur), n, grid(), exact(), b, temp.p, temp.bytes, st);
     if (n == 0) {
       sorted_valid = true;
       return;
     }
-    int cbits = bits_for((uint64_t)(nc - 1));
-    if (exact()) {
-      launch_keys<T>(parts(cur), n, grid(), nullptr, keys64.as<uint64_t>(), vals.as<uint32_t>(), true, st);
-      HIPCHK(sort_pairs_u64(temp.p, temp.bytes, keys64.as<uint64_t>(), keys64_alt.as<uint64_t>(),
-                            vals.as<uint32_t>(), vals_alt.as<uint32_t>(), n, 32 + cbits, st));
-      launch_cell_ranges(nullptr, keys64_alt.as<uint64_t>(), n, nc, cstart.as<uint32_t>(),
-                         cend.as<uint32_t>(), st);
-    } else {
-      launch_keys<T>(parts(cur), n, grid(), keys.as<uint32_t>(), nullptr, vals.as<uint32_t>(), false, st);
-      HIPCHK(sort_pairs_u32(temp.p, temp.bytes, keys.as<uint32_t>(), keys_alt.as<uint32_t>(),
-                            vals.as<uint32_t>(), vals_alt.as<uint32_t>(), n, cbits, st));
-      launch_cell_ranges(keys_alt.as<uint32_t>(), nullptr, n, nc, cstart.as<uint32_t>(),
-                         cend.as<uint32_t>(), st);
-    }
-    launch_gather_parts<T>(parts(cur), parts(1 - cur), vals_alt.as<uint32_t>(), n, st);
+    launch_gather_parts<T>(parts(cur), parts(1 - cur), vals.as<uint32_t>(), n, st);
     cur = 1 - cur;
     sorted_valid = true;
   }
@@ -1190,7 +1177,7 @@ struct Solver : SolverBase {
                       culled_ranks.as<uint32_t>(), st);
     cur = 1 - cur;
     int64_t kept = n - removed;
-    HIPCHK(sort_keys_u32(temp.p, temp.bytes, culled_ranks.as<uint32_t>(), culled_sorted.as<uint32_t>(),
+    HIPCHK(sort_keys_u32(culled_ranks.as<uint32_t>(), culled_sorted.as<uint32_t>(), cullflag.as<uint32_t>(),
                          removed, st));
     launch_rank_fix(part[cur][6].as<uint32_t>(), kept, culled_sorted.as<uint32_t>(), removed, st);
     n = kept;
@@ -1308,8 +1295,8 @@ struct Solver : SolverBase {
       if (nact == 0) break;
       launch_active_compact(keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, part[cur][6].as<uint32_t>(),
                             act_rank.as<uint32_t>(), act_slot.as<uint32_t>(), st);
-      HIPCHK(sort_pairs_u32(temp.p, temp.bytes, act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(),
-                            act_slot.as<uint32_t>(), act_slot_sorted.as<uint32_t>(), nact, 32, st));
+      HIPCHK(sort_pairs_u32(act_rank.as<uint32_t>(), act_rank_sorted.as<uint32_t>(), act_slot.as<uint32_t>(),
+                            act_slot_sorted.as<uint32_t>(), keys.as<uint32_t>(), keys_alt.as<uint32_t>(), nact, st));
       redist_active += (uint64_t)nact;
       unsigned long long* mv = counters.as<unsigned long long>() + 3;
       launch_redist_decide(act_slot_sorted.as<uint32_t>(), nact, cellkey.as<int32_t>(), count.as<uint32_t>(),
@@ -1765,12 +1752,13 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     const double S = precision == SDR_PRECISION_F32 ? 4.0 : 8.0;
     const double nc = (double)nx * ny * nz;
     const double nsh = (double)nx * (nx + ny - 1) * nz;  // sheared solve layout
-    // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch
-    const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 12.0 + (precision == SDR_PRECISION_F64 ? 16.0 : 0.0);
-    // per cell: cell ranges and classes, faces (current, saved, scratch), pressure,
-    // redistribution tables; per sheared slot: 7 solve vectors, flags, f64 refinement
-    // vectors (f32 path), halo words of the pipelined sweeps
-    const double pcell = 12.0 + 3.0 + 7.0 * S + 2.0 + 89.0;
+    // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch (the
+    // exact mode's merge keys, the long-segment list)
+    const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 5.0;
+    // per cell: cell ranges, counts, sort cursor and classes, faces (current, saved,
+    // scratch), pressure, redistribution tables; per sheared slot: 7 solve vectors,
+    // flags, f64 refinement vectors (f32 path), halo words of the pipelined sweeps
+    const double pcell = 16.0 + 3.0 + 7.0 * S + 2.0 + 89.0;
     const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;
@@ -1870,6 +1858,12 @@ int sdr_solver_set_cells(sdr_solver* s, const uint8_t* c) {
 }
 int sdr_solver_get_pressure(sdr_solver* s, double* p) {
   return WITH(s, S.copy_out(S.pressure, S.ncells(), p));
+}
+int sdr_solver_get_slot_ranks(sdr_solver* s, uint32_t* r, uint64_t m) {
+  return WITH(s, if ((int64_t)m != S.n) throw Error(SDR_E_INVALID, "particle count mismatch");
+              if (m == 0) return;
+              HIPCHK(hipMemcpyAsync(r, S.part[S.cur][6].p, (size_t)m * 4, hipMemcpyDeviceToHost, S.st));
+              HIPCHK(hipStreamSynchronize(S.st)));
 }
 int sdr_solver_get_state(const sdr_solver* s, sdr_state* st) {
   return WITH(const_cast<sdr_solver*>(s), memset(st, 0, sizeof *st); st->time = S.time;

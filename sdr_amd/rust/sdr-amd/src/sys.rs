@@ -319,6 +319,7 @@ extern "C" {
     pub fn sdr_solver_get_cells(s: *mut sdr_solver, cells: *mut u8) -> c_int;
     pub fn sdr_solver_set_cells(s: *mut sdr_solver, cells: *const u8) -> c_int;
     pub fn sdr_solver_get_pressure(s: *mut sdr_solver, p: *mut f64) -> c_int;
+    pub fn sdr_solver_get_slot_ranks(s: *mut sdr_solver, ranks: *mut u32, n: u64) -> c_int;
     pub fn sdr_solver_get_state(s: *const sdr_solver, st: *mut sdr_state) -> c_int;
     pub fn sdr_solver_set_state(s: *mut sdr_solver, st: *const sdr_state) -> c_int;
     pub fn sdr_solver_diagnostics(s: *mut sdr_solver, out: *mut sdr_diagnostics) -> c_int;
