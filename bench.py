@@ -39,6 +39,7 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
         "update_s": nc * 3 * s,                 # s = z + b s
         "mic_build": nc * (s + flags),          # read flags; write precon
         "spmv_dot": nc * (2 * s + flags),       # read s, flags; write As (dot(s, As) in the same pass)
+        "spmv_update": nc * (4 * s + flags),    # read z, old s, flags; write new s, As (and dot(s, As))
         "axpy_norm": nc * 6 * s,                # p += a s; r -= a As; max|r|
         "dot_update": nc * 5 * s,               # dot(z, r); s = z + b s
         "p2g": np_ * 6 * s + 3 * nc * s,        # read x, v; write u, v, w
@@ -259,6 +260,7 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
 
 
 SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot", "spmv_dot",
+                 "spmv_update",
                  "axpy_norm", "dot_update", "update_s", "gradient", "refine"]
 
 

@@ -527,6 +527,160 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
   if (partials) block_partial<0>(acc, partials);
 }
 
+// The direction update s = z + beta s (pressure.rs:386-388; s = z on the first step,
+// pressure.rs:367) fused with apply_a of the new s and the partials of dot(s, As)
+// (pressure.rs:372-373).  A workgroup owns a tile of 64 values of i by kR rows J over a
+// chunk of planes and marches up the planes: each plane's tile plus a one-slot halo of
+// the new s (formed from z and the old s as it is staged) goes into a ring of three LDS
+// planes, so z, the old s and the flags are read once per cell (plus the halo) instead
+// of once per neighbour, and the new s is stored once, by the tile that owns the cell.
+// The old and new s live in different buffers, since tiles read the old s of their
+// neighbours' cells.  The arithmetic and its order are those of k_update_s and k_spmv.
+constexpr int kR = 16;
+struct Tile25 {
+  int nti, ntj, ntk, kc;
+};
+__host__ __device__ inline Tile25 tile25(Shear sh) {
+  Tile25 t;
+  t.kc = sh.nz >= 128 ? 16 : 8;
+  t.nti = (sh.nx + 63) / 64;
+  t.ntj = (sh.NJ + kR - 1) / kR;
+  t.ntk = (sh.nz + t.kc - 1) / t.kc;
+  return t;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* __restrict__ flags,
+                                                     const T* __restrict__ z, T* s0, T* s1, T* __restrict__ out,
+                                                     PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  constexpr int NS = ((kR + 2) * 66 + 255) / 256;  // staged slots per thread
+  constexpr int NC = kR / 4;                       // owned cells per thread and plane
+  // four plane slots: the staging of plane k + 1 writes the slot plane k - 3 used, which no
+  // thread reads after the barrier of plane k - 1, so one barrier per plane suffices
+  __shared__ T ring[4][kR + 2][66];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters += 1;  // one more iteration of a running solve
+  const int par = sc->spar;
+  const bool copy = sc->started == 0;
+  const T* __restrict__ sin = par ? s1 : s0;
+  T* __restrict__ sout = par ? s0 : s1;
+  const T beta = (T)sc->beta;
+  const int nx = sh.nx, ny = sh.ny, nz = sh.nz, NJ = sh.NJ;
+  const int64_t plane = (int64_t)NJ * nx;
+  const Tile25 tl = tile25(sh);
+  // workgroup b runs on XCD b % 8: each XCD gets a contiguous range of tiles (i fastest,
+  // then J, then the plane chunk), so the halo rows and planes it re-reads are its own
+  const int G = gridDim.x, b = blockIdx.x;
+  const int lb = (b & 7) * (G >> 3) + (b >> 3);
+  const int ti = lb % tl.nti, tj = (lb / tl.nti) % tl.ntj, tk = lb / (tl.nti * tl.ntj);
+  const int i0 = ti * 64, J0 = tj * kR, k0 = tk * tl.kc;
+  const int k1 = k0 + tl.kc < nz ? k0 + tl.kc : nz;
+  const int ilast = i0 + 63 < nx - 1 ? i0 + 63 : nx - 1;
+  // a tile with no in-grid slot (j = J - i outside [0, ny) on every row) does nothing
+  const bool live = tk < tl.ntk && i0 < nx && J0 + kR - 1 >= i0 && J0 - ilast <= ny - 1;
+  double acc = 0.0;
+  if (live) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = i0 + lane;
+    // staged slots of this thread: in-plane offset (or -1 outside the grid), ring
+    // position, and whether the tile owns (stores) the cell
+    int soff[NS], spos[NS];
+    bool sown[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      const int rr = e / 66, cc = e - rr * 66;
+      const int J = J0 - 1 + rr, ii = i0 - 1 + cc, jj = J - ii;
+      const bool in = e < (kR + 2) * 66 && (unsigned)ii < (unsigned)nx && (unsigned)jj < (unsigned)ny;
+      soff[q] = in ? J * nx + ii : -1;
+      spos[q] = e < (kR + 2) * 66 ? e : -1;
+      sown[q] = rr >= 1 && rr <= kR && cc >= 1 && cc <= 64;
+    }
+    // owned cells: in-plane offset (or -1 outside the grid) and j
+    int coff[NC], cj[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      const int J = J0 + w + 4 * m, jj = J - i;
+      cj[m] = jj;
+      coff[m] = i < nx && (unsigned)jj < (unsigned)ny ? J * nx + i : -1;
+    }
+    T zr[NS], sr[NS];
+    uint16_t fr[NC];
+    // issue the loads of plane k (z, old s, and the owned cells' flags)
+    auto load_plane = [&](int k) {
+      const bool kin = (unsigned)k < (unsigned)nz;
+      const int64_t base = (int64_t)k * plane;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        zr[q] = T(0), sr[q] = T(0);
+        if (kin && soff[q] >= 0) {
+          zr[q] = z[base + soff[q]];
+          if (!copy) sr[q] = sin[base + soff[q]];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) fr[m] = kin && k < k1 && coff[m] >= 0 ? flags[base + coff[m]] : (uint16_t)0;
+    };
+    // the new s of plane k into ring slot sl; the cells the tile owns are stored
+    auto stage = [&](int k, int sl) {
+      const bool own = k >= k0 && k < k1;
+      const int64_t base = (int64_t)k * plane;
+      T* rs = &ring[sl][0][0];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const T v = copy ? zr[q] : zr[q] + beta * sr[q];
+        if (spos[q] >= 0) rs[spos[q]] = v;
+        if (own && sown[q] && soff[q] >= 0) sout[base + soff[q]] = v;
+      }
+    };
+    uint16_t fcur[NC];
+    load_plane(k0 - 1);
+    stage(k0 - 1, 0);
+    load_plane(k0);
+    stage(k0, 1);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) fcur[m] = fr[m];
+    load_plane(k0 + 1);
+    for (int k = k0; k < k1; ++k) {
+      const int sm = (k - k0) & 3, s0_ = (k - k0 + 1) & 3, sp = (k - k0 + 2) & 3;
+      stage(k + 1, sp);
+      uint16_t fn[NC];
+#pragma unroll
+      for (int m = 0; m < NC; ++m) fn[m] = fr[m];
+      if (k + 1 < k1) load_plane(k + 2);  // in flight under this plane's product
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        if (coff[m] < 0) continue;
+        const int rr = 1 + w + 4 * m, j = cj[m];
+        const uint16_t fu = fcur[m];
+        const int cc = lane + 1;
+        const T xc = ring[s0_][rr][cc];
+        T val = T(fl_diag(fu)) * xc;
+        // a neighbour inside the grid adds (-1 or +0) * s, one outside it adds nothing
+        const T a0 = val + ((fu & kFlPx) ? T(-1) : T(0)) * ring[s0_][rr + 1][cc + 1];
+        val = i + 1 < nx ? a0 : val;
+        const T a1 = val + ((fu & kFlMx) ? T(-1) : T(0)) * ring[s0_][rr - 1][cc - 1];
+        val = i >= 1 ? a1 : val;
+        const T a2 = val + ((fu & kFlPy) ? T(-1) : T(0)) * ring[s0_][rr + 1][cc];
+        val = j + 1 < ny ? a2 : val;
+        const T a3 = val + ((fu & kFlMy) ? T(-1) : T(0)) * ring[s0_][rr - 1][cc];
+        val = j >= 1 ? a3 : val;
+        const T a4 = val + ((fu & kFlPz) ? T(-1) : T(0)) * ring[sp][rr][cc];
+        val = k + 1 < nz ? a4 : val;
+        const T a5 = val + ((fu & kFlMz) ? T(-1) : T(0)) * ring[sm][rr][cc];
+        val = k >= 1 ? a5 : val;
+        const bool fl = (fu & kFlFluid) != 0;
+        const T res = fl ? val : T(0);
+        acc += fl ? (double)xc * (double)res : 0.0;
+        out[(int64_t)k * plane + coff[m]] = res;
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) fcur[m] = fn[m];
+    }
+  }
+  block_partial<0>(acc, partials);
+}
+
 // ------------------------------------------------------------------ dot (pressure.rs:588-603)
 // Exact mode: one workgroup per 4096-element chunk of the reference cell order.  The
 // products are formed in parallel into LDS, then one lane folds them left to right,
@@ -620,6 +774,9 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
       break;
     case kOpResidual:
       sc->residual = v;
+      // the step's direction (written by the fused product) is the current one now
+      sc->spar ^= 1;
+      sc->started = 1;
       if (v <= sc->tol_eff) sc->done = 1;
       break;
     case kOpRhsNorm:
@@ -661,9 +818,12 @@ __global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, c
 // wavefront step (the loads of both cells are issued before either is stored).
 template <class T>
 __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T* __restrict__ r,
-                                                  const T* __restrict__ s, const T* __restrict__ as,
-                                                  const PcgScalars* sc, double* partials) {
+                                                  const T* s0, const T* __restrict__ as,
+                                                  const PcgScalars* sc, double* partials,
+                                                  const T* s1 = nullptr) {
   if (sc->done) return;
+  // s1: s is the direction the fused product just wrote (sbuf[1 - spar])
+  const T* __restrict__ s = s1 && !sc->spar ? s1 : s0;
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
@@ -720,6 +880,43 @@ __global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__
   if (sc->done) return;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
+               rq = make_rsrc(as, n * SZ);
+  const T alpha = (T)sc->alpha;
+  double m = 0.0;
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    T pv[U], sv[U], rv[U], qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      pv[u] = bld<T>(rp, t[u] * SZ);
+      sv[u] = bld<T>(rs, t[u] * SZ);
+      rv[u] = bld<T>(rr, t[u] * SZ);
+      qv[u] = bld<T>(rq, t[u] * SZ);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bst<T>(rp, t[u] * SZ, pv[u] + alpha * sv[u]);
+      const T nr = rv[u] - alpha * qv[u];
+      bst<T>(rr, t[u] * SZ, nr);
+      m = nmax(m, fabs((double)nr));  // 0 outside the grid
+    }
+  }
+  block_partial<1>(m, partials);
+}
+
+// k_axpy_pr_units with s the direction the fused product just wrote (sbuf[1 - spar]).
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_axpy_pr_par(Shear sh, T* __restrict__ p, T* __restrict__ r,
+                                                     const T* s0, const T* s1, const T* __restrict__ as,
+                                                     const PcgScalars* sc, double* partials, int span) {
+  if (sc->done) return;
+  constexpr uint32_t SZ = sizeof(T);
+  const uint32_t n = (uint32_t)sh.size();
+  const T* s = sc->spar ? s0 : s1;
   const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
                rq = make_rsrc(as, n * SZ);
   const T alpha = (T)sc->alpha;
@@ -1482,6 +1679,25 @@ void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream
   }
   k_update_s<T><<<nb, 256, 0, st>>>(sh, s, z, sc);
 }
+int spmv_update_blocks(Shear sh) {
+  const Tile25 t = tile25(sh);
+  const int64_t g = (int64_t)t.nti * t.ntj * t.ntk;
+  return (int)((g + 7) / 8 * 8);
+}
+template <class T>
+void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
+                        double* partials, hipStream_t st) {
+  k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials);
+}
+template <class T>
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
+                        double* partials, int nblocks, hipStream_t st) {
+  if (units_fit<T>(sh)) {
+    k_axpy_pr_par<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s0, s1, as, sc, partials, unit_span_len(sh, nblocks, 4));
+    return;
+  }
+  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s0, as, sc, partials, s1);
+}
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
                       int sigma, const PcgScalars* sc, hipStream_t st) {
@@ -1715,6 +1931,10 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_axpy_pr<T>(Shear, T*, T*, const T*, const T*, const PcgScalars*, double*,   \
                                   int, hipStream_t);                                               \
   template void launch_update_s<T>(Shear, T*, const T*, const PcgScalars*, hipStream_t);           \
+  template void launch_spmv_update<T>(Shear, const uint16_t*, const T*, T*, T*, T*, PcgScalars*,     \
+                                      double*, hipStream_t);                                       \
+  template void launch_axpy_pr_par<T>(Shear, T*, T*, const T*, const T*, const T*, const PcgScalars*, \
+                                      double*, int, hipStream_t);                                  \
   template void launch_mic_plane<T>(int, Shear, const uint16_t*, T*, const T*, T*, int,            \
                                     const PcgScalars*, hipStream_t);                               \
   template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \

@@ -30,6 +30,10 @@ struct PcgScalars {
   int32_t nan_seen;
   int32_t pad;
   double tol_eff;  // the residual bound in use: tol, or tol * max|b| in relative mode
+  // The fused product's two direction buffers (launch_spmv_update): spar names the one
+  // holding the current s; started is 0 until the first step, whose product takes s = z.
+  int32_t spar;
+  int32_t started;
 };
 
 enum DotOp : int32_t {
@@ -96,6 +100,17 @@ void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScal
                     double* partials, int nblocks, hipStream_t st);
 template <class T>
 void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st);
+// The direction update s = z + beta s (s = z on a solve's first step) fused with the
+// product As = A s and the partials of dot(s, As): s is read from sbuf[spar] and written
+// to sbuf[1 - spar] (the residual's finish flips spar).  partials: spmv_update_blocks.
+template <class T>
+void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
+                        double* partials, hipStream_t st);
+int spmv_update_blocks(Shear sh);
+// p += alpha s; r -= alpha As with s the direction just written by launch_spmv_update.
+template <class T>
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
+                        double* partials, int nblocks, hipStream_t st);
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
                       int sigma, const PcgScalars* sc, hipStream_t st);

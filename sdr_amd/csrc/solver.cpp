@@ -160,7 +160,7 @@ struct Projector {
   bool exact_dot = true;
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
-  Buf b, z, s, p, as, q, precon, flags, partials, scal, units;
+  Buf b, z, s, s2, p, as, q, precon, flags, partials, scal, units;  // s2: the fused product's second direction buffer
   Buf halo_x, halo_z, pipe_err, sink;
   Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
@@ -252,6 +252,7 @@ struct Projector {
     }
     if (hx * sizeof(uint64_t) >= lim || hz * sizeof(uint64_t) >= lim) mic_variant = -1;
     for (int v = 0; v < mic_pipe_variants(); ++v) nb = std::max(nb, mic_pipe_dot_partials(mic_pipe_shape(sh, v, ncu)));
+    nb = std::max(nb, spmv_update_blocks(sh));
     if (mic_variant < 0) hx = hz = 1;
     pl.nb = (size_t)nb, pl.hx = hx, pl.hz = hz;
     return pl;
@@ -279,6 +280,7 @@ struct Projector {
     halo_z.alloc(pl.hz * sizeof(uint64_t));
     pipe_err.alloc(sizeof(int));
     sink.alloc(1024 * sizeof(T));
+    s2.alloc(pl.n * sizeof(T));
     HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
     HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
     HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
@@ -297,7 +299,7 @@ struct Projector {
   // Bytes of caller memory init_noalloc carves for a plan (256-byte aligned pieces).
   static size_t up256(size_t b) { return (b + 255) / 256 * 256; }
   static size_t noalloc_bytes(const Plan& pl) {
-    return 7 * up256(pl.n * sizeof(T)) + up256(pl.n * 2) + up256(pl.nb * 8) + up256(sizeof(PcgScalars)) +
+    return 8 * up256(pl.n * sizeof(T)) + up256(pl.n * 2) + up256(pl.nb * 8) + up256(sizeof(PcgScalars)) +
            up256(pl.nunits * 4) + up256(pl.hx * 8) + up256(pl.hz * 8) + up256(sizeof(int)) + up256(1024 * sizeof(T));
   }
   // The same working set as init, as views of [base, base + cap): no allocation, and the
@@ -316,7 +318,7 @@ struct Projector {
       bf.view(base + off, up256(bytes));
       off += up256(bytes);
     };
-    for (Buf* bb : {&b, &z, &s, &p, &as, &q, &precon}) take(*bb, pl.n * sizeof(T));
+    for (Buf* bb : {&b, &z, &s, &s2, &p, &as, &q, &precon}) take(*bb, pl.n * sizeof(T));
     take(flags, pl.n * 2);
     take(partials, pl.nb * 8);
     take(scal, sizeof(PcgScalars));
@@ -392,6 +394,17 @@ struct Projector {
     launch_dot_finish(partials.as<double>(), nb, exact_dot, op, sc(), tol, st);
   }
 
+  // The fused step head of the fast mode: s = z + beta s (s = z on the first step) and
+  // As = A s with the partials of dot(s, As), then denom and alpha.  The new s goes to
+  // the other direction buffer (sbuf = {s, s2}, PcgScalars::spar).
+  void fused_product() {
+    launch_spmv_update<T>(sh, flags.as<uint16_t>(), z.as<T>(), s.as<T>(), s2.as<T>(), as.as<T>(), sc(),
+                          partials.as<double>(), st);
+    launch_dot_finish(partials.as<double>(), spmv_update_blocks(sh), false, kOpDenom, sc(), 0.0, st);
+  }
+  // The current direction buffer (fused: by the polled scalars).
+  Buf& cur_s(const PcgScalars& h) { return fused() && h.spar ? s2 : s; }
+
   PcgScalars read_scalars() {
     if (noalloc) {  // no pinned staging: a pageable copy
       PcgScalars r;
@@ -412,7 +425,7 @@ struct Projector {
     const size_t n = (size_t)sh.size();
     {
       Scope sc_(*timer, "pcg_setup");
-      for (Buf* bb : {&z, &s, &p, &as, &q}) zero(*bb, n * sizeof(T));
+      for (Buf* bb : {&z, &s, &s2, &p, &as, &q}) zero(*bb, n * sizeof(T));
       if (build) zero(precon, n * sizeof(T));
       zero(scal, sizeof(PcgScalars));
       int mb = maxabs_blocks(sh);
@@ -432,23 +445,24 @@ struct Projector {
       Scope sc_(*timer, "mic_bwd");
       mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
     }
-    HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    const bool fz = fused();
+    // s = z (pressure.rs:367); the fused product takes it on the first step itself
+    if (!fz) HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
     dot(z.as<T>(), r, kOpSigma0, tol);
     const int mb = maxabs_blocks(sh);
     const uint64_t check_every = 8;
-    const bool fz = fused();
     const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
     // one PCG iteration (pressure.rs:372-396); dev: sweep tags from the device word
     auto iteration = [&](bool dev) {
       if (fz) {
         {
-          Scope sc1(*timer, "spmv_dot");
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), partials.as<double>(), st);
-          launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), tol, st);
+          Scope sc1(*timer, "spmv_update");
+          fused_product();
         }
         {
           Scope sc1(*timer, "axpy_norm");
-          launch_axpy_pr<T>(sh, p.as<T>(), r, s.as<T>(), as.as<T>(), sc(), partials.as<double>(), mb, st);
+          launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
+                                mb, st);
           launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
         }
         {
@@ -459,10 +473,6 @@ struct Projector {
           Scope sc1(*timer, "mic_bwd_dot");
           mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r, dev);
           launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
-        }
-        {
-          Scope sc1(*timer, "update_s");
-          launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
         }
         return;
       }
@@ -623,7 +633,7 @@ struct Projector {
     }
     {
       Scope sc_(*timer, "pcg_setup");
-      for (Buf* bb : {&z, &s, &p, &as, &q, &precon}) zero(*bb, n * sizeof(T));
+      for (Buf* bb : {&z, &s, &s2, &p, &as, &q, &precon}) zero(*bb, n * sizeof(T));
       zero(scal, sizeof(PcgScalars));
       launch_maxabs_partials<T>(sh, r, partials.as<double>(), mb, sc(), st);
       launch_dot_finish(partials.as<double>(), mb, false, kOpRhsNorm, sc(), target / scale, st, 0);
@@ -640,9 +650,9 @@ struct Projector {
       Scope sc_(*timer, "mic_bwd");
       mic_sweep(kMicBwd, q.as<T>(), z.as<T>());
     }
-    HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
-    dot(z.as<T>(), r, kOpSigma0, 0.0);
     const bool fz = fused();
+    if (!fz) HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    dot(z.as<T>(), r, kOpSigma0, 0.0);
     const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
     const uint64_t check_every = 8;
     uint64_t it = 0;
@@ -651,15 +661,21 @@ struct Projector {
     // An iteration is split where the scalars are polled: head = the product, step and
     // residual norm; tail = the preconditioner solve and the direction update.
     auto head = [&] {
+      if (fz) {
+        {
+          Scope sc1(*timer, "spmv_update");
+          fused_product();
+        }
+        Scope sc1(*timer, "axpy_norm");
+        launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
+                              mb, st);
+        launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), 0.0, st);
+        return;
+      }
       {
         Scope sc1(*timer, "spmv_dot");
-        if (fz) {
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), partials.as<double>(), st);
-          launch_dot_finish(partials.as<double>(), cell_blocks(sh), false, kOpDenom, sc(), 0.0, st);
-        } else {
-          launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), nullptr, st);
-          dot(s.as<T>(), as.as<T>(), kOpDenom, 0.0);
-        }
+        launch_spmv<T>(sh, flags.as<uint16_t>(), s.as<T>(), as.as<T>(), sc(), nullptr, st);
+        dot(s.as<T>(), as.as<T>(), kOpDenom, 0.0);
       }
       {
         Scope sc1(*timer, "axpy_norm");
@@ -683,7 +699,7 @@ struct Projector {
         }
         dot(z.as<T>(), r, kOpSigmaNew, 0.0);
       }
-      {
+      if (!fz) {
         Scope sc1(*timer, "update_s");
         launch_update_s<T>(sh, s.as<T>(), z.as<T>(), sc(), st);
       }
@@ -749,11 +765,12 @@ struct Projector {
           const double f = scale / rn;
           scale = rn;
           launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / scale, r, st);
+          Buf& sd = cur_s(sc_h);
           if (restart) {
-            zero(s, n * sizeof(T));
+            zero(sd, n * sizeof(T));
             ++restarts;
           } else {
-            launch_scale_inplace<T>((int64_t)n, s.as<T>(), f, st);
+            launch_scale_inplace<T>((int64_t)n, sd.as<T>(), f, st);
           }
           launch_pcg_rescale(sc(), f * f, target / scale, restart ? 1 : 0, st);
         } else if (sc_h.done) {
