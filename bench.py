@@ -32,9 +32,9 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
     flags = the u16 coupling flags of a cell."""
     flags = 2
     table = {
-        "mic_fwd": nc * (3 * s + flags),        # read r, precon, flags; write q
+        "mic_fwd": nc * 3 * s,                  # read r, precon (Fluid = precon != 0); write q
+        "mic_fwd_dot": nc * 3 * s,              # ... and dot(q, q) = dot(z, r) from registers
         "mic_bwd": nc * (3 * s + flags),        # read q, precon, flags; write z
-        "mic_bwd_dot": nc * (4 * s + flags),    # ... and read r for dot(z, r)
         "mic_fwd_axpy": nc * (8 * s + flags),   # read r, As, p, s, precon, flags; write r, p, q
         "update_s": nc * 3 * s,                 # s = z + b s
         "mic_build": nc * (s + flags),          # read flags; write precon
@@ -259,7 +259,8 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
     return s, elapsed, iters, particle_steps, timing, breakdown
 
 
-SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot", "spmv_dot",
+SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_dot", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot",
+                 "spmv_dot",
                  "spmv_update",
                  "axpy_norm", "dot_update", "update_s", "gradient", "refine"]
 

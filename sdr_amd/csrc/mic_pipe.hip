@@ -635,20 +635,25 @@ __device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
 //     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) are
 //     loaded from a far offset (not fetched: half the sheared slots), read as flags 0,
 //     and compute and store 0; that select is the only bounds test in the step.
-// DOT (backward solve): each compute lane also accumulates z * r over its cells in f64
-// (fast reduction mode of the dot(z, r) that follows, pressure.rs:383), and the
-// workgroup leaves one partial: the dot costs one more load per cell instead of a pass.
+// DOT (forward solve): each compute lane also accumulates q * q over its cells in f64,
+// and the workgroup leaves one partial.  The preconditioner is applied as M = L L^T (the
+// forward solve L q = r, the backward L^T z = q), so dot(z, r) = r^T L^-T L^-1 r = |q|^2:
+// the fast reduction mode takes the dot(z, r) of pressure.rs:383 from values the forward
+// solve holds in registers, and the backward solve reads no residual.
+// The forward solve needs no flags: a cell is Fluid exactly when its precon is nonzero
+// (the factor writes 1/sqrt(e) to Fluid cells only; the others stay +0).
 template <class T, bool REV, int TK, bool DOT, int PFH>
 __global__ void __launch_bounds__(64 * (TK + 2))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
                  MicPipeArgs<T> pa) {
+  static_assert(!(REV && DOT), "the dot is taken in the forward solve");
   if (sc->done) return;
   constexpr int W = Words<T>::n;
   constexpr uint32_t SZ = sizeof(T);
-  // the cell ring, PF steps deep: 3 loads and a store per step (4 and a store with
-  // DOT); a step waits for the loads of PF steps before (deeper rings, 12 and 16 steps,
-  // measured no faster)
+  // the cell ring, PF steps deep: 2 loads and a store per step forward, 3 and a store
+  // backward; a step waits for the loads of PF steps before (deeper rings, 12 and 16
+  // steps, measured no faster)
   constexpr int PF = 8;
   // the step loop is unrolled by U: ring slots are compile-time (u % PF for the cells,
   // (u + 1) % PFH for the halo words) and the LDS parity of step s0 + u is u & 1
@@ -691,8 +696,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
     const rsrc_t rf = make_rsrc(flags + (int64_t)plane * NJ * nx, recs * 2u),
                  rp = make_rsrc(pbase, recs * SZ),
                  ri = make_rsrc(in + (int64_t)plane * NJ * nx, recs * SZ),
-                 ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ),
-                 rr = make_rsrc(DOT ? pa.dot_r + (int64_t)plane * NJ * nx : nullptr, DOT ? recs * SZ : 0u);
+                 ro = make_rsrc(out + (int64_t)plane * NJ * nx, recs * SZ);
     const int zsrc = wk == bwave ? TK : (REV ? wk + 1 : wk - 1);
     // element offset of (row J, lane i); lanes past the grid start out of range for good
     // (2^28 * size stays below 2^32 and above every plane's record count)
@@ -708,20 +712,19 @@ __global__ void __launch_bounds__(64 * (TK + 2))
     // sheared slots) are not loaded
     int jn = J0 + PF * dJ - i;
     uint32_t cf[PF];
-    T cp[PF], ci[PF], cr[PF];
+    T cp[PF], ci[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const bool in = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny;
       const uint32_t e = in ? lbase + (uint32_t)((J0 + u * dJ) * nx) : kFarElem;
-      cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0);
+      cf[u] = REV ? __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0) : 0u;
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
-      if (DOT) cr[u] = bld<T>(rr, e * SZ);
       // issue slot by slot, in the order of the step loop: the wait for a slot's loads
       // at the top of the loop then also holds for the ones issued here
       __builtin_amdgcn_sched_barrier(0);
     }
-    double dacc = 0.0;  // dot(z, r) (DOT)
+    double dacc = 0.0;  // |q|^2 (DOT)
     zbuf[1][wk][lane] = T(0);
     T prev = T(0);
     __syncthreads();  // the I/O wave's prologue has filled zbuf[1][TK] and xlds[1]
@@ -735,7 +738,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         const uint32_t f = cf[u];
         const T inu = ci[u];
         const T pcu = cp[u];
-        const bool live = (f & kFlFluid) != 0;
+        const bool live = REV ? (f & kFlFluid) != 0 : pcu != T(0);
         T res, pub;
         if (!REV) {
           // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
@@ -761,13 +764,12 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         // are never stored to
         const bool inc = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny;
         bst<T>(ro, inc ? oc : kFarElem * SZ, res);
-        if (DOT) dacc += (double)res * (double)cr[u];  // 0 outside the grid and off-fluid
+        if (DOT) dacc += (double)res * (double)res;  // 0 outside the grid and off-fluid
         const bool inn = (unsigned)jn < (unsigned)sh.ny;
         const uint32_t en2 = inn ? of2 : kFarElem * 2u, enS = inn ? ofn : kFarElem * SZ;
-        cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
+        if (REV) cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
         cp[u] = bld<T>(rp, enS);
         ci[u] = bld<T>(ri, enS);
-        if (DOT) cr[u] = bld<T>(rr, enS);
         oc += dS;
         of2 += d2;
         ofn += dS;
@@ -966,10 +968,10 @@ template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd)
+  if (op == kMicFwd && pa.dot)
+    k_mic_sweep3<T, false, TK, true, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else if (op == kMicFwd)
     k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
-  else if (pa.dot_r)
-    k_mic_sweep3<T, true, TK, true, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
     k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }

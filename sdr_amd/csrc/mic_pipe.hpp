@@ -24,10 +24,10 @@ struct MicPipeArgs {
   uint32_t tag;
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
   int ncu;       // compute units of the device
-  // Backward solves of the I/O-wave kernel (kind 4) only: with dot_r set, the sweep also
-  // leaves one partial of dot(z, dot_r) per workgroup (mic_pipe_dot_partials of them) in
-  // dot_partials.
-  const T* dot_r;
+  // Forward solves of the I/O-wave kernel (kind 4) only: with dot set, the sweep also
+  // leaves one partial of dot(q, q) = dot(z, r) per workgroup (mic_pipe_dot_partials of
+  // them) in dot_partials.
+  int dot;
   double* dot_partials;
   // When set, the sweep's tag is read from device memory instead of `tag`: sweeps
   // replayed from a captured graph keep their launch arguments, so launch_tag_next

@@ -352,7 +352,9 @@ struct Projector {
   // The fast reduction mode fuses dot(z, r) into the backward solve (I/O-wave sweeps only).
   bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 4; }
 
-  void mic_sweep(int op, const T* in, T* out, const T* dot_r = nullptr, bool dev_tag = false) {
+  // dot (forward solves of the fused mode): the sweep leaves the partials of
+  // dot(q, q) = dot(z, r) in `partials`.
+  void mic_sweep(int op, const T* in, T* out, bool dot = false, bool dev_tag = false) {
     if (mic_variant < 0) {
       const int planes = sh.NJ + sh.nz - 1;
       for (int sgm = 0; sgm < planes; ++sgm)
@@ -372,7 +374,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, dot_r, partials.as<double>(),
+                      (uint32_t)halo_z.bytes, ncu, dot ? 1 : 0, partials.as<double>(),
                       dev_tag ? tagbuf.as<uint32_t>() : nullptr};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
@@ -466,13 +468,13 @@ struct Projector {
           launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
         }
         {
-          Scope sc1(*timer, "mic_fwd");
-          mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
+          Scope sc1(*timer, "mic_fwd_dot");
+          mic_sweep(kMicFwd, r, q.as<T>(), true, dev);
+          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
         }
         {
-          Scope sc1(*timer, "mic_bwd_dot");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r, dev);
-          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
+          Scope sc1(*timer, "mic_bwd");
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
         }
         return;
       }
@@ -493,11 +495,11 @@ struct Projector {
       }
       {
         Scope sc_(*timer, "mic_fwd");
-        mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
+        mic_sweep(kMicFwd, r, q.as<T>(), false, dev);
       }
       {
         Scope sc_(*timer, "mic_bwd");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), nullptr, dev);
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
       }
       {
         Scope sc_(*timer, "dot_update");
@@ -684,18 +686,22 @@ struct Projector {
       }
     };
     auto tail = [&](bool dev) {
-      {
-        Scope sc1(*timer, "mic_fwd");
-        mic_sweep(kMicFwd, r, q.as<T>(), nullptr, dev);
-      }
       if (fz) {
-        Scope sc1(*timer, "mic_bwd_dot");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), r, dev);
-        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+        {
+          Scope sc1(*timer, "mic_fwd_dot");
+          mic_sweep(kMicFwd, r, q.as<T>(), true, dev);
+          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+        }
+        Scope sc1(*timer, "mic_bwd");
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
       } else {
         {
+          Scope sc1(*timer, "mic_fwd");
+          mic_sweep(kMicFwd, r, q.as<T>(), false, dev);
+        }
+        {
           Scope sc1(*timer, "mic_bwd");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), nullptr, dev);
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
         }
         dot(z.as<T>(), r, kOpSigmaNew, 0.0);
       }
