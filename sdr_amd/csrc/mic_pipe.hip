@@ -641,8 +641,12 @@ __device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
 // the fast reduction mode takes the dot(z, r) of pressure.rs:383 from values the forward
 // solve holds in registers, and the backward solve reads no residual.
 // The forward solve needs no flags: a cell is Fluid exactly when its precon is nonzero
-// (the factor writes 1/sqrt(e) to Fluid cells only; the others stay +0).
-template <class T, bool REV, int TK, bool DOT, int PFH>
+// (the factor writes 1/sqrt(e) to Fluid cells only; the others stay +0).  The backward
+// solve of the fast mode (FL = false) needs none either: a neighbour that does not
+// couple (Air or Solid, or outside the grid) publishes +0, so subtracting its term
+// unconditionally gives the same values up to the sign of a zero; the reference mode
+// (FL) keeps the coupling flags and the reference bits.
+template <class T, bool REV, int TK, bool DOT, int PFH, bool FL = true>
 __global__ void __launch_bounds__(64 * (TK + 2))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
@@ -717,7 +721,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
     for (int u = 0; u < PF; ++u) {
       const bool in = (unsigned)(J0 + u * dJ - i) < (unsigned)sh.ny;
       const uint32_t e = in ? lbase + (uint32_t)((J0 + u * dJ) * nx) : kFarElem;
-      cf[u] = REV ? __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0) : 0u;
+      cf[u] = REV && FL ? __builtin_amdgcn_raw_buffer_load_b16(rf, e * 2u, 0, 0) : 0u;
       cp[u] = bld<T>(rp, e * SZ);
       ci[u] = bld<T>(ri, e * SZ);
       // issue slot by slot, in the order of the step loop: the wait for a slot's loads
@@ -738,7 +742,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         const uint32_t f = cf[u];
         const T inu = ci[u];
         const T pcu = cp[u];
-        const bool live = REV ? (f & kFlFluid) != 0 : pcu != T(0);
+        const bool live = REV && FL ? (f & kFlFluid) != 0 : pcu != T(0);
         T res, pub;
         if (!REV) {
           // a neighbour that does not couple publishes +0, and x - (+0) == x for every x
@@ -748,9 +752,15 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         } else {
           const T npc = -pcu;
           T x = inu;
-          x = x - ((f & kFlPx) ? npc * xin : T(0));
-          x = x - ((f & kFlPy) ? npc * prev : T(0));
-          x = x - ((f & kFlPz) ? npc * zin : T(0));
+          if (FL) {
+            x = x - ((f & kFlPx) ? npc * xin : T(0));
+            x = x - ((f & kFlPy) ? npc * prev : T(0));
+            x = x - ((f & kFlPz) ? npc * zin : T(0));
+          } else {
+            x = x - npc * xin;
+            x = x - npc * prev;
+            x = x - npc * zin;
+          }
           const T z = x * pcu;
           res = live ? z : T(0);
           pub = res;
@@ -767,7 +777,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         if (DOT) dacc += (double)res * (double)res;  // 0 outside the grid and off-fluid
         const bool inn = (unsigned)jn < (unsigned)sh.ny;
         const uint32_t en2 = inn ? of2 : kFarElem * 2u, enS = inn ? ofn : kFarElem * SZ;
-        if (REV) cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
+        if (REV && FL) cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
         cp[u] = bld<T>(rp, enS);
         ci[u] = bld<T>(ri, enS);
         oc += dS;
@@ -972,6 +982,8 @@ void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, con
     k_mic_sweep3<T, false, TK, true, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else if (op == kMicFwd)
     k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else if (pa.flagfree)
+    k_mic_sweep3<T, true, TK, false, PFH, false><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else
     k_mic_sweep3<T, true, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
 }

@@ -29,6 +29,9 @@ struct MicPipeArgs {
   // them) in dot_partials.
   int dot;
   double* dot_partials;
+  // Backward solves of kind 4 in the fast mode: subtract every neighbour's term without
+  // the coupling flags (the same values up to the sign of a zero).
+  int flagfree;
   // When set, the sweep's tag is read from device memory instead of `tag`: sweeps
   // replayed from a captured graph keep their launch arguments, so launch_tag_next
   // advances this word before each of them.  Device tags lie in [2^31, 2^32), host tags

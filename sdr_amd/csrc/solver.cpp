@@ -374,7 +374,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, dot ? 1 : 0, partials.as<double>(),
+                      (uint32_t)halo_z.bytes, ncu, dot ? 1 : 0, partials.as<double>(), fused() ? 1 : 0,
                       dev_tag ? tagbuf.as<uint32_t>() : nullptr};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
