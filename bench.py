@@ -33,8 +33,8 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
     flags = 2
     table = {
         "mic_fwd": nc * 3 * s,                  # read r, precon (Fluid = precon != 0); write q
-        "mic_fwd_dot": nc * 3 * s,              # ... and dot(q, q) = dot(z, r) from registers
         "mic_bwd": nc * (3 * s + flags),        # read q, precon, flags; write z
+        "mic_bwd_dot": nc * 3 * s,              # fast mode: read q, precon; write z; dot(q, q) = dot(z, r)
         "mic_fwd_axpy": nc * (8 * s + flags),   # read r, As, p, s, precon, flags; write r, p, q
         "update_s": nc * 3 * s,                 # s = z + b s
         "mic_build": nc * (s + flags),          # read flags; write precon

@@ -635,11 +635,11 @@ __device__ inline void tile_of(int b, int nti, int ntk, int* itl, int* ktl) {
 //     always out of range.  Slots of a row outside the grid (j < 0 or j >= ny) are
 //     loaded from a far offset (not fetched: half the sheared slots), read as flags 0,
 //     and compute and store 0; that select is the only bounds test in the step.
-// DOT (forward solve): each compute lane also accumulates q * q over its cells in f64,
+// DOT (backward solve): each compute lane also accumulates q * q over its cells in f64,
 // and the workgroup leaves one partial.  The preconditioner is applied as M = L L^T (the
 // forward solve L q = r, the backward L^T z = q), so dot(z, r) = r^T L^-T L^-1 r = |q|^2:
-// the fast reduction mode takes the dot(z, r) of pressure.rs:383 from values the forward
-// solve holds in registers, and the backward solve reads no residual.
+// the fast reduction mode takes the dot(z, r) of pressure.rs:383 from the backward
+// solve's own input, and reads no residual.
 // The forward solve needs no flags: a cell is Fluid exactly when its precon is nonzero
 // (the factor writes 1/sqrt(e) to Fluid cells only; the others stay +0).  The backward
 // solve of the fast mode (FL = false) needs none either: a neighbour that does not
@@ -651,7 +651,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
     k_mic_sweep3(Shear sh, const uint16_t* __restrict__ flags, const T* __restrict__ precon,
                  const T* __restrict__ in, T* __restrict__ out, const PcgScalars* sc,
                  MicPipeArgs<T> pa) {
-  static_assert(!(REV && DOT), "the dot is taken in the forward solve");
+  static_assert(REV || !DOT, "the dot is taken in the backward solve");
   if (sc->done) return;
   constexpr int W = Words<T>::n;
   constexpr uint32_t SZ = sizeof(T);
@@ -774,7 +774,7 @@ __global__ void __launch_bounds__(64 * (TK + 2))
         // are never stored to
         const bool inc = (unsigned)(jn - PF * dJ) < (unsigned)sh.ny;
         bst<T>(ro, inc ? oc : kFarElem * SZ, res);
-        if (DOT) dacc += (double)res * (double)res;  // 0 outside the grid and off-fluid
+        if (DOT) dacc += (double)inu * (double)inu;  // q: 0 outside the grid and off-fluid
         const bool inn = (unsigned)jn < (unsigned)sh.ny;
         const uint32_t en2 = inn ? of2 : kFarElem * 2u, enS = inn ? ofn : kFarElem * SZ;
         if (REV && FL) cf[u] = __builtin_amdgcn_raw_buffer_load_b16(rf, en2, 0, 0);
@@ -978,10 +978,10 @@ template <class T, int TK, int PFH>
 void launch_sweep3(int op, Shear sh, const uint16_t* flags, const T* precon, const T* in, T* out,
                    const PcgScalars* sc, MicPipeArgs<T> pa, hipStream_t st) {
   const int grid = pa.ntile_i * pa.ntile_k;
-  if (op == kMicFwd && pa.dot)
-    k_mic_sweep3<T, false, TK, true, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
-  else if (op == kMicFwd)
+  if (op == kMicFwd)
     k_mic_sweep3<T, false, TK, false, PFH><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
+  else if (pa.flagfree && pa.dot)
+    k_mic_sweep3<T, true, TK, true, PFH, false><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else if (pa.flagfree)
     k_mic_sweep3<T, true, TK, false, PFH, false><<<grid, 64 * (TK + 2), 0, st>>>(sh, flags, precon, in, out, sc, pa);
   else

@@ -24,9 +24,9 @@ struct MicPipeArgs {
   uint32_t tag;
   uint32_t hx_bytes, hz_bytes;  // sizes of halo_x / halo_z (below 2^31)
   int ncu;       // compute units of the device
-  // Forward solves of the I/O-wave kernel (kind 4) only: with dot set, the sweep also
-  // leaves one partial of dot(q, q) = dot(z, r) per workgroup (mic_pipe_dot_partials of
-  // them) in dot_partials.
+  // Backward solves of the I/O-wave kernel (kind 4) in the fast mode only: with dot set,
+  // the sweep also leaves one partial of dot(q, q) = dot(z, r) per workgroup
+  // (mic_pipe_dot_partials of them) in dot_partials.
   int dot;
   double* dot_partials;
   // Backward solves of kind 4 in the fast mode: subtract every neighbour's term without

@@ -352,7 +352,7 @@ struct Projector {
   // The fast reduction mode fuses dot(z, r) into the backward solve (I/O-wave sweeps only).
   bool fused() const { return !exact_dot && mic_variant >= 0 && mic_pipe_shape(sh, mic_variant, ncu).kind == 4; }
 
-  // dot (forward solves of the fused mode): the sweep leaves the partials of
+  // dot (backward solves of the fused mode): the sweep leaves the partials of
   // dot(q, q) = dot(z, r) in `partials`.
   void mic_sweep(int op, const T* in, T* out, bool dot = false, bool dev_tag = false) {
     if (mic_variant < 0) {
@@ -468,13 +468,13 @@ struct Projector {
           launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
         }
         {
-          Scope sc1(*timer, "mic_fwd_dot");
-          mic_sweep(kMicFwd, r, q.as<T>(), true, dev);
-          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
+          Scope sc1(*timer, "mic_fwd");
+          mic_sweep(kMicFwd, r, q.as<T>(), false, dev);
         }
         {
-          Scope sc1(*timer, "mic_bwd");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
+          Scope sc1(*timer, "mic_bwd_dot");
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);
+          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
         }
         return;
       }
@@ -688,12 +688,12 @@ struct Projector {
     auto tail = [&](bool dev) {
       if (fz) {
         {
-          Scope sc1(*timer, "mic_fwd_dot");
-          mic_sweep(kMicFwd, r, q.as<T>(), true, dev);
-          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+          Scope sc1(*timer, "mic_fwd");
+          mic_sweep(kMicFwd, r, q.as<T>(), false, dev);
         }
-        Scope sc1(*timer, "mic_bwd");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), false, dev);
+        Scope sc1(*timer, "mic_bwd_dot");
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);
+        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
       } else {
         {
           Scope sc1(*timer, "mic_fwd");

@@ -91,7 +91,7 @@ def test_timing_modes(dev, orc):
     assert d1.last_pcg_iters == d2.last_pcg_iters > 16
     assert any(not k.startswith("mic_") for k in t1)
     sampled = {k for k in t1 if k.startswith("mic_") and t1[k][1] >= 16}
-    assert set(t2) == sampled and "mic_fwd_dot" in t2
+    assert set(t2) == sampled and "mic_bwd_dot" in t2
     for k, (ms, calls) in t2.items():
         assert calls == t1[k][1] // 16 and ms > 0.0, (k, calls, t1[k])
 
