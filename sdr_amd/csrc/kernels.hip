@@ -531,16 +531,20 @@ __global__ void __launch_bounds__(256) k_spmv_units(Shear sh, const uint16_t* __
 // pressure.rs:367) fused with apply_a of the new s and the partials of dot(s, As)
 // (pressure.rs:372-373).  A workgroup owns a tile of 64 values of i by kR rows J over a
 // chunk of planes and marches up the planes: each plane's tile plus a one-slot halo of
-// the new s (formed from z and the old s as it is staged) goes into a ring of three LDS
-// planes, so z, the old s and the flags are read once per cell (plus the halo) instead
-// of once per neighbour, and the new s is stored once, by the tile that owns the cell.
-// The old and new s live in different buffers, since tiles read the old s of their
-// neighbours' cells.  The arithmetic and its order are those of k_update_s and k_spmv.
+// the new s (formed from z and the old s as it is staged) goes into a ring of LDS planes,
+// so z, the old s and the flags are read once per cell (plus the halo) instead of once
+// per neighbour, and the new s is stored once, by the tile that owns the cell.  The old
+// and new s live in different buffers, since tiles read the old s of their neighbours'
+// cells.  The arithmetic and its order are those of k_update_s and k_spmv.
+//
+// Every access goes through a descriptor of one plane of its buffer (no records for a
+// plane outside the grid) at a 32-bit offset, kOob where the slot is outside the grid or
+// not the thread's to store: no bounds branches and no 64-bit address arithmetic.
 constexpr int kR = 16;
 struct Tile25 {
   int nti, ntj, ntk, kc;
 };
-__host__ __device__ inline Tile25 tile25(Shear sh) {
+inline Tile25 tile25(Shear sh) {
   Tile25 t;
   t.kc = sh.nz >= 128 ? 16 : 8;
   t.nti = (sh.nx + 63) / 64;
@@ -551,22 +555,27 @@ __host__ __device__ inline Tile25 tile25(Shear sh) {
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* __restrict__ flags,
                                                      const T* __restrict__ z, T* s0, T* s1, T* __restrict__ out,
-                                                     PcgScalars* sc, double* partials) {
+                                                     PcgScalars* sc, double* partials, Tile25 tl) {
   if (sc->done) return;
   constexpr int NS = ((kR + 2) * 66 + 255) / 256;  // staged slots per thread
   constexpr int NC = kR / 4;                       // owned cells per thread and plane
+  constexpr uint32_t SZ = sizeof(T);
   // four plane slots: the staging of plane k + 1 writes the slot plane k - 3 used, which no
   // thread reads after the barrier of plane k - 1, so one barrier per plane suffices
   __shared__ T ring[4][kR + 2][66];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters += 1;  // one more iteration of a running solve
   const int par = sc->spar;
   const bool copy = sc->started == 0;
-  const T* __restrict__ sin = par ? s1 : s0;
-  T* __restrict__ sout = par ? s0 : s1;
+  const T* sin = par ? s1 : s0;
+  T* sout = par ? s0 : s1;
   const T beta = (T)sc->beta;
   const int nx = sh.nx, ny = sh.ny, nz = sh.nz, NJ = sh.NJ;
   const int64_t plane = (int64_t)NJ * nx;
-  const Tile25 tl = tile25(sh);
+  // plane k of a buffer of esz-byte elements; no records unless k is in [klo, khi)
+  auto prs = [&](const void* base, int k, int klo, int khi, uint32_t esz) {
+    const bool in = k >= klo && k < khi;
+    return make_rsrc((const char*)base + (in ? (int64_t)k * plane * esz : 0), in ? (uint32_t)(plane * esz) : 0u);
+  };
   // workgroup b runs on XCD b % 8: each XCD gets a contiguous range of tiles (i fastest,
   // then J, then the plane chunk), so the halo rows and planes it re-reads are its own
   const int G = gridDim.x, b = blockIdx.x;
@@ -581,55 +590,51 @@ __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* _
   if (live) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = i0 + lane;
-    // staged slots of this thread: in-plane offset (or -1 outside the grid), ring
-    // position, and whether the tile owns (stores) the cell
-    int soff[NS], spos[NS];
-    bool sown[NS];
+    // staged slots of this thread: load offset (kOob outside the grid) and store offset
+    // (kOob unless the tile owns the cell); the ring position is threadIdx.x + 256 q
+    uint32_t lo[NS], so[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const int e = threadIdx.x + 256 * q;
       const int rr = e / 66, cc = e - rr * 66;
       const int J = J0 - 1 + rr, ii = i0 - 1 + cc, jj = J - ii;
       const bool in = e < (kR + 2) * 66 && (unsigned)ii < (unsigned)nx && (unsigned)jj < (unsigned)ny;
-      soff[q] = in ? J * nx + ii : -1;
-      spos[q] = e < (kR + 2) * 66 ? e : -1;
-      sown[q] = rr >= 1 && rr <= kR && cc >= 1 && cc <= 64;
+      const bool own = rr >= 1 && rr <= kR && cc >= 1 && cc <= 64;
+      lo[q] = in ? (uint32_t)(J * nx + ii) * SZ : kOob;
+      so[q] = in && own ? lo[q] : kOob;
     }
-    // owned cells: in-plane offset (or -1 outside the grid) and j
-    int coff[NC], cj[NC];
+    // owned cells: in-plane index (kFarElem outside the grid) and j
+    uint32_t ci[NC];
+    int cj[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
       const int J = J0 + w + 4 * m, jj = J - i;
       cj[m] = jj;
-      coff[m] = i < nx && (unsigned)jj < (unsigned)ny ? J * nx + i : -1;
+      ci[m] = i < nx && (unsigned)jj < (unsigned)ny ? (uint32_t)(J * nx + i) : kFarElem;
     }
     T zr[NS], sr[NS];
     uint16_t fr[NC];
-    // issue the loads of plane k (z, old s, and the owned cells' flags)
+    // issue the loads of plane k: z, the old s, and the owned cells' flags
     auto load_plane = [&](int k) {
-      const bool kin = (unsigned)k < (unsigned)nz;
-      const int64_t base = (int64_t)k * plane;
+      const rsrc_t rz = prs(z, k, 0, nz, SZ), rs = prs(sin, k, 0, copy ? 0 : nz, SZ);
+      const rsrc_t rf = prs(flags, k, k0, k1, 2);
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
-        zr[q] = T(0), sr[q] = T(0);
-        if (kin && soff[q] >= 0) {
-          zr[q] = z[base + soff[q]];
-          if (!copy) sr[q] = sin[base + soff[q]];
-        }
+        zr[q] = bld<T>(rz, lo[q]);
+        sr[q] = copy ? T(0) : bld<T>(rs, lo[q]);
       }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) fr[m] = kin && k < k1 && coff[m] >= 0 ? flags[base + coff[m]] : (uint16_t)0;
+      for (int m = 0; m < NC; ++m) fr[m] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, ci[m] * 2u, 0, 0);
     };
     // the new s of plane k into ring slot sl; the cells the tile owns are stored
     auto stage = [&](int k, int sl) {
-      const bool own = k >= k0 && k < k1;
-      const int64_t base = (int64_t)k * plane;
+      const rsrc_t rso = prs(sout, k, k0, k1, SZ);
       T* rs = &ring[sl][0][0];
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const T v = copy ? zr[q] : zr[q] + beta * sr[q];
-        if (spos[q] >= 0) rs[spos[q]] = v;
-        if (own && sown[q] && soff[q] >= 0) sout[base + soff[q]] = v;
+        if (q < NS - 1 || threadIdx.x + 256 * q < (kR + 2) * 66) rs[threadIdx.x + 256 * q] = v;
+        bst<T>(rso, so[q], v);
       }
     };
     uint16_t fcur[NC];
@@ -648,9 +653,10 @@ __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* _
       for (int m = 0; m < NC; ++m) fn[m] = fr[m];
       if (k + 1 < k1) load_plane(k + 2);  // in flight under this plane's product
       __syncthreads();
+      const rsrc_t ro = prs(out, k, 0, nz, SZ);
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        if (coff[m] < 0) continue;
+        // cells outside the grid have flags 0 (no fluid): they add nothing and store nothing
         const int rr = 1 + w + 4 * m, j = cj[m];
         const uint16_t fu = fcur[m];
         const int cc = lane + 1;
@@ -672,7 +678,7 @@ __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* _
         const bool fl = (fu & kFlFluid) != 0;
         const T res = fl ? val : T(0);
         acc += fl ? (double)xc * (double)res : 0.0;
-        out[(int64_t)k * plane + coff[m]] = res;
+        bst<T>(ro, ci[m] * SZ, res);
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) fcur[m] = fn[m];
@@ -774,9 +780,11 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
       break;
     case kOpResidual:
       sc->residual = v;
-      // the step's direction (written by the fused product) is the current one now
+      // the step's direction (written by the fused product) is the current one now, and
+      // its p += alpha s is pending (launch_flush_p)
       sc->spar ^= 1;
       sc->started = 1;
+      sc->ppend = 1;
       if (v <= sc->tol_eff) sc->done = 1;
       break;
     case kOpRhsNorm:
@@ -818,12 +826,9 @@ __global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, c
 // wavefront step (the loads of both cells are issued before either is stored).
 template <class T>
 __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T* __restrict__ r,
-                                                  const T* s0, const T* __restrict__ as,
-                                                  const PcgScalars* sc, double* partials,
-                                                  const T* s1 = nullptr) {
+                                                  const T* __restrict__ s, const T* __restrict__ as,
+                                                  const PcgScalars* sc, double* partials) {
   if (sc->done) return;
-  // s1: s is the direction the fused product just wrote (sbuf[1 - spar])
-  const T* __restrict__ s = s1 && !sc->spar ? s1 : s0;
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
@@ -908,41 +913,74 @@ __global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__
   block_partial<1>(m, partials);
 }
 
-// k_axpy_pr_units with s the direction the fused product just wrote (sbuf[1 - spar]).
+// r -= alpha As (pressure.rs:375) with the partials of max|r|: the fast mode's axpy; its
+// p update (launch_flush_p) is off the critical path.
 template <class T, int U>
-__global__ void __launch_bounds__(256) k_axpy_pr_par(Shear sh, T* __restrict__ p, T* __restrict__ r,
-                                                     const T* s0, const T* s1, const T* __restrict__ as,
-                                                     const PcgScalars* sc, double* partials, int span) {
+__global__ void __launch_bounds__(256) k_axpy_r_units(Shear sh, T* __restrict__ r, const T* __restrict__ as,
+                                                      const PcgScalars* sc, double* partials, int span) {
   if (sc->done) return;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
-  const T* s = sc->spar ? s0 : s1;
-  const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
-               rq = make_rsrc(as, n * SZ);
+  const rsrc_t rr = make_rsrc(r, n * SZ), rq = make_rsrc(as, n * SZ);
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   int q0, q1;
   unit_span(sh, span, &q0, &q1);
   for (int q = q0; q < q1; q += U) {
     uint32_t t[U];
-    T pv[U], sv[U], rv[U], qv[U];
+    T rv[U], qv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       t[u] = unit_slot(sh, q + u, q1).t;
-      pv[u] = bld<T>(rp, t[u] * SZ);
-      sv[u] = bld<T>(rs, t[u] * SZ);
       rv[u] = bld<T>(rr, t[u] * SZ);
       qv[u] = bld<T>(rq, t[u] * SZ);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      bst<T>(rp, t[u] * SZ, pv[u] + alpha * sv[u]);
       const T nr = rv[u] - alpha * qv[u];
       bst<T>(rr, t[u] * SZ, nr);
       m = nmax(m, fabs((double)nr));  // 0 outside the grid
     }
   }
   block_partial<1>(m, partials);
+}
+// The same over row pairs (grids too large for the units' 32-bit offsets).
+template <class T>
+__global__ void __launch_bounds__(256) k_axpy_r2(Shear sh, T* __restrict__ r, const T* __restrict__ as,
+                                                 const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  const T alpha = (T)sc->alpha;
+  double m = 0.0;
+  for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
+    const int64_t ta = a.base + ai, tb = b.base + bi;
+    T ra = T(0), qa = T(0), rb = T(0), qb = T(0);
+    if (ao) ra = r[ta], qa = as[ta];
+    if (bo) rb = r[tb], qb = as[tb];
+    if (ao) {
+      const T rv = ra - alpha * qa;
+      r[ta] = rv;
+      m = nmax(m, fabs((double)rv));
+    }
+    if (bo) {
+      const T rv = rb - alpha * qb;
+      r[tb] = rv;
+      m = nmax(m, fabs((double)rv));
+    }
+  });
+  block_partial<1>(m, partials);
+}
+// p += alpha s (pressure.rs:374) of the step whose residual update set the pending flag
+// (s = sbuf[spar]); k_ppend_clear then clears the flag.
+template <class T>
+__global__ void __launch_bounds__(256) k_flush_p(Shear sh, T* __restrict__ p, const T* s0, const T* s1,
+                                                 const PcgScalars* sc) {
+  if (!sc->ppend) return;
+  const T* __restrict__ s = sc->spar ? s1 : s0;
+  const T alpha = (T)sc->alpha;
+  for_cells(sh, [&](int64_t t, int, int, int) { p[t] = p[t] + alpha * s[t]; });
+}
+__global__ void k_ppend_clear(PcgScalars* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->ppend = 0;
 }
 
 // ------------------------------------------------------------------ MIC(0) (pressure.rs:442-570)
@@ -1687,16 +1725,21 @@ int spmv_update_blocks(Shear sh) {
 template <class T>
 void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
                         double* partials, hipStream_t st) {
-  k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials);
+  k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials, tile25(sh));
 }
 template <class T>
-void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
-                        double* partials, int nblocks, hipStream_t st) {
+void launch_axpy_r(Shear sh, T* r, const T* as, const PcgScalars* sc, double* partials, int nblocks,
+                   hipStream_t st) {
   if (units_fit<T>(sh)) {
-    k_axpy_pr_par<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s0, s1, as, sc, partials, unit_span_len(sh, nblocks, 4));
+    k_axpy_r_units<T, 4><<<nblocks, 256, 0, st>>>(sh, r, as, sc, partials, unit_span_len(sh, nblocks, 4));
     return;
   }
-  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s0, as, sc, partials, s1);
+  k_axpy_r2<T><<<nblocks, 256, 0, st>>>(sh, r, as, sc, partials);
+}
+template <class T>
+void launch_flush_p(Shear sh, T* p, const T* s0, const T* s1, PcgScalars* sc, hipStream_t st) {
+  k_flush_p<T><<<cell_blocks(sh), 256, 0, st>>>(sh, p, s0, s1, sc);
+  k_ppend_clear<<<1, 64, 0, st>>>(sc);
 }
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
@@ -1933,8 +1976,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_update_s<T>(Shear, T*, const T*, const PcgScalars*, hipStream_t);           \
   template void launch_spmv_update<T>(Shear, const uint16_t*, const T*, T*, T*, T*, PcgScalars*,     \
                                       double*, hipStream_t);                                       \
-  template void launch_axpy_pr_par<T>(Shear, T*, T*, const T*, const T*, const T*, const PcgScalars*, \
-                                      double*, int, hipStream_t);                                  \
+  template void launch_axpy_r<T>(Shear, T*, const T*, const PcgScalars*, double*, int, hipStream_t);  \
+  template void launch_flush_p<T>(Shear, T*, const T*, const T*, PcgScalars*, hipStream_t);          \
   template void launch_mic_plane<T>(int, Shear, const uint16_t*, T*, const T*, T*, int,            \
                                     const PcgScalars*, hipStream_t);                               \
   template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \
