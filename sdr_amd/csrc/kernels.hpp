@@ -67,12 +67,29 @@ void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const ui
 template <class T>
 void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
                 const uint32_t* cend, T* out, bool exact, hipStream_t st);
+// Heavy cells of the fused P2G: a cell holding at least `thresh` particles is reduced
+// once, by a workgroup of its own (per target face a fixed-order sum over the cell's
+// particles), and enters each of its faces' sums as one term after the other particles
+// of its slab, so no face walks it particle by particle.  thresh 0: none.
+struct P2gHeavy {
+  uint32_t* hid;         // per cell: its entry (read only for cells at or over thresh)
+  uint32_t* hcell;       // [cap] the cell of each entry
+  uint32_t* ctr;         // entries made (may pass cap: the cells past it stay ordinary)
+  void* part;            // [cap][kP2gHeavyVals] of T: per component 18 sums of w v, then 18 of w
+  uint32_t cap, thresh;
+};
+constexpr int kP2gHeavyVals = 108;
+// List the heavy cells and reduce each (before launch_p2g_fused with the same tables).
+template <class T>
+void launch_p2g_heavy(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend,
+                      P2gHeavy h, hipStream_t st);
 // All three P2G components as one LDS-staged gather over the sorted particles
-// (p2g.hip), the same sums in the same order as the face-by-face gather (fast mode);
-// false when the grid is too large for it.
+// (p2g.hip), the same sums in the same order as the face-by-face gather (fast mode),
+// except that heavy cells (h.thresh > 0) enter as one pre-reduced term each; false when
+// the grid is too large for it.
 template <class T>
 bool launch_p2g_fused(GridD<T> g, const Parts<T>& p, const uint32_t* cstart, const uint32_t* cend, FacesD<T> f,
-                      hipStream_t st);
+                      P2gHeavy h, hipStream_t st);
 // SDR_P2G_PERFACE=1: the face-by-face gather kernels instead of the fused one.
 template <class T>
 void launch_axpy_const(T* a, int64_t n, T c, hipStream_t st);

@@ -919,6 +919,11 @@ struct Solver : SolverBase {
   int64_t n = 0, cap = 0;
   bool sorted_valid = false, have_saved = false;
   bool p2g_perface = false;
+  // Heavy cells of the fused P2G (fast mode): cells holding at least this many particles
+  // are pre-reduced and enter their faces' sums as one term each (SDR_P2G_HEAVY, read
+  // once per solver; 0 walks every cell).  The table holds up to one cell in 64.
+  uint32_t p2g_heavy = 32;
+  Buf p2g_hcell, p2g_hpart;
   int cur = 0;
   // -- device
   hipStream_t st = nullptr;
@@ -1043,6 +1048,12 @@ struct Solver : SolverBase {
     if (const char* e = getenv("SDR_REDIST_SEQUENTIAL")) redist_sequential = atoi(e) != 0;
     // SDR_P2G_PERFACE=1 (read once per solver) selects the face-by-face gather
     if (const char* e = getenv("SDR_P2G_PERFACE")) p2g_perface = atoi(e) != 0;
+    if (const char* e = getenv("SDR_P2G_HEAVY")) p2g_heavy = (uint32_t)std::max(0, atoi(e));
+    if (opts.dot_mode != SDR_DOT_EXACT && p2g_heavy) {
+      const size_t hcap = (size_t)(nc / 64 + 1);
+      p2g_hcell.alloc((hcap + 1) * 4);  // entries, then the counter
+      p2g_hpart.alloc(hcap * kP2gHeavyVals * sizeof(T));
+    }
     proj.exact_dot = opts.dot_mode == SDR_DOT_EXACT;
     proj.init(nx, ny, nz, dx, st, &timer);
     reserve(1024);
@@ -1158,9 +1169,18 @@ struct Solver : SolverBase {
   void p2g() {
     ensure_counts();
     Scope sc_(timer, "p2g");
-    if (!exact() && !p2g_perface &&
-        launch_p2g_fused<T>(grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), faces(), st))
-      return;
+    if (!exact() && !p2g_perface) {
+      P2gHeavy h{};
+      if (p2g_heavy && p2g_hcell.p) {
+        const uint32_t hcap = (uint32_t)(p2g_hcell.bytes / 4 - 1);
+        // the sort's per-cell cursor is free once the particles are sorted: entry per cell
+        h = P2gHeavy{sort_cursor.as<uint32_t>(), p2g_hcell.as<uint32_t>(), p2g_hcell.as<uint32_t>() + hcap,
+                     p2g_hpart.p, hcap, p2g_heavy};
+        launch_p2g_heavy<T>(grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), h, st);
+      }
+      if (launch_p2g_fused<T>(grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), faces(), h, st))
+        return;
+    }
     launch_p2g<T>(0, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fu.as<T>(), exact(), st);
     launch_p2g<T>(1, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fv.as<T>(), exact(), st);
     launch_p2g<T>(2, grid(), parts(cur), cstart.as<uint32_t>(), cend.as<uint32_t>(), fw.as<T>(), exact(), st);
@@ -1821,9 +1841,10 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     // exact mode's merge keys, the long-segment list)
     const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 5.0;
     // per cell: cell ranges, counts, sort cursor and classes, faces (current, saved,
-    // scratch), pressure, redistribution tables; per sheared slot: 7 solve vectors,
+    // scratch), pressure, redistribution tables, the P2G heavy-cell table (one entry per
+    // 64 cells); per sheared slot: 7 solve vectors,
     // flags, f64 refinement vectors (f32 path), halo words of the pipelined sweeps
-    const double pcell = 16.0 + 3.0 + 7.0 * S + 2.0 + 89.0;
+    const double pcell = 16.0 + 3.0 + 7.0 * S + 2.0 + 89.0 + (sdr::kP2gHeavyVals * S + 4.0) / 64.0;
     const double pslot = 7.0 * S + 2.0 + (precision == SDR_PRECISION_F32 ? 24.0 : 0.0) + 12.0;
     const double est = 1.1 * ((double)particles * ppart + nc * pcell + nsh * pslot) + 64.0 * 1048576.0;
     *bytes = (uint64_t)est;

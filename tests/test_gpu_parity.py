@@ -127,10 +127,12 @@ def test_p2g_fused_matches_face_gather(dev, precision, monkeypatch):
     """The one-pass LDS-staged P2G (slabs staged once for all three components, several
     z-levels accumulated at a time) sums every face in the same order as the
     face-by-face gather, so the two agree bit for bit; a 5000-particle clump makes some
-    slabs overflow the staging area (several passes)."""
+    slabs overflow the staging area (several passes).  Heavy cells off: every cell is
+    walked."""
     sc = scenes.cavity_scene("c2")
     g = np.random.default_rng(5)
     out = []
+    monkeypatch.setenv("SDR_P2G_HEAVY", "0")
     for perface in ("0", "1"):
         monkeypatch.setenv("SDR_P2G_PERFACE", perface)  # read when the solver is made
         d = scenes.make_solver(sc, precision=precision, **FAST)
@@ -148,6 +150,81 @@ def test_p2g_fused_matches_face_gather(dev, precision, monkeypatch):
     for a, b in zip(*out):
         assert np.abs(a).max() > 0
         assert bits_equal(a, b)
+
+
+def _dilate(a, axis):
+    """a | a shifted by one cell either way along `axis`."""
+    out = a.copy()
+    sl = [slice(None)] * a.ndim
+    lo, hi = list(sl), list(sl)
+    lo[axis], hi[axis] = slice(1, None), slice(None, -1)
+    out[tuple(lo)] |= a[tuple(hi)]
+    out[tuple(hi)] |= a[tuple(lo)]
+    return out
+
+
+def _faces_reached(heavy):
+    """Faces whose P2G stencil covers a cell of `heavy` (z, y, x): along a component's axis
+    the cells f - 1 and f, across it f - 1 .. f + 1."""
+    nz, ny, nx = heavy.shape
+    out = []
+    for ax in (2, 1, 0):  # u (x), v (y), w (z)
+        h = heavy
+        for other in (0, 1, 2):
+            if other != ax:
+                h = _dilate(h, other)
+        shape = list(heavy.shape)
+        shape[ax] += 1
+        f = np.zeros(shape, bool)
+        lo, hi = [slice(None)] * 3, [slice(None)] * 3
+        lo[ax], hi[ax] = slice(0, -1), slice(1, None)
+        f[tuple(lo)] |= h  # face f = cell index (the cell after the face)
+        f[tuple(hi)] |= h  # face f = cell index + 1 (the cell before)
+        out.append(f)
+    return out
+
+
+@pytest.mark.parametrize("precision", [A.PRECISION_F32, A.PRECISION_F64])
+def test_p2g_heavy_cells_match_face_gather(dev, precision, monkeypatch):
+    """Cells holding at least SDR_P2G_HEAVY particles (32) are pre-reduced by a workgroup
+    of their own and enter each face's sums as one term: faces no such cell reaches keep
+    the face-by-face gather's bits, the others agree to rounding (the same terms summed in
+    another order).  Clumps of 31, 32, 40-600 and 5000 particles on top of the fill; the
+    faces counted as reached are those of every cell holding 30 or more (a superset, so
+    that a particle rounding into the next cell in f32 cannot move the boundary)."""
+    sc = scenes.cavity_scene("c2")
+    g = np.random.default_rng(7)
+    out = []
+    for perface, heavy in (("1", "0"), ("0", "32")):
+        monkeypatch.setenv("SDR_P2G_PERFACE", perface)
+        monkeypatch.setenv("SDR_P2G_HEAVY", heavy)
+        d = scenes.make_solver(sc, precision=precision, **FAST)
+        if not out:
+            fluid = d.cells().reshape(d.nz, d.ny, d.nx) != 0
+            pos = scenes.fill_particles(d, sc, fluid, stride=1)
+            kk, jj, ii = np.nonzero(fluid)
+            pick = g.choice(kk.size, size=40, replace=False)
+            sizes = [31, 32, 5000] + list(g.integers(40, 600, size=37))
+            clumps = [np.array(d.cell_center(ii[p], jj[p], kk[p])) + g.uniform(-0.45, 0.45, (n, 3)) * d.dx
+                      for p, n in zip(pick, sizes)]
+            pos = np.concatenate([pos] + clumps)
+            vel = g.standard_normal(pos.shape)
+            loc = np.clip(((pos - np.asarray(d.origin)) / d.dx).astype(np.int64), 0,
+                          [d.nx - 1, d.ny - 1, d.nz - 1])
+            cnt = np.zeros((d.nz, d.ny, d.nx), np.int64)
+            np.add.at(cnt, (loc[:, 2], loc[:, 1], loc[:, 0]), 1)
+            heavy_cells = cnt >= 30
+            assert (cnt >= 34).sum() >= 38
+        d.set_particles(pos, vel)
+        d.run_stage("classify", sc.dt)
+        d.run_stage("p2g", sc.dt)
+        out.append(d.grid())
+        d.close()
+    tol = FIELD_TOL if precision == A.PRECISION_F32 else 1e-12
+    for ref, got, reach in zip(out[0], out[1], _faces_reached(heavy_cells)):
+        assert reach.any() and not reach.all()
+        assert bits_equal(ref[~reach], got[~reach])
+        assert np.abs(got[reach] - ref[reach]).max() <= tol * np.abs(ref).max()
 
 
 def test_deterministic_run_to_run(dev):
