@@ -41,7 +41,6 @@ def region_bytes(name: str, s: int, nc: int, nsh: int, np_: int, nsdf: int) -> f
         "spmv_dot": nc * (2 * s + flags),       # read s, flags; write As (dot(s, As) in the same pass)
         "spmv_update": nc * (4 * s + flags),    # read z, old s, flags; write new s, As (and dot(s, As))
         "axpy_norm": nc * 6 * s,                # p += a s; r -= a As; max|r|
-        "axpy_r": nc * 3 * s,                   # fast mode: r -= a As; max|r| (p += a s on a side stream)
         "dot_update": nc * 5 * s,               # dot(z, r); s = z + b s
         "p2g": np_ * 6 * s + 3 * nc * s,        # read x, v; write u, v, w
         "g2p": np_ * 9 * s + 6 * nc * s,        # read x, v; write v; read u,v,w and saved
@@ -263,7 +262,7 @@ def run_path(scene, prec, dot_mode, solve_mode, tol, max_iters, steps, warmup, l
 SOLVE_REGIONS = ["rhs", "pcg_setup", "mic_build", "mic_fwd", "mic_fwd_dot", "mic_fwd_axpy", "mic_bwd", "mic_bwd_dot",
                  "spmv_dot",
                  "spmv_update",
-                 "axpy_norm", "axpy_r", "dot_update", "update_s", "gradient", "refine"]
+                 "axpy_norm", "dot_update", "update_s", "gradient", "refine"]
 
 
 def solve_error(s, scene, rel_ref, rel_tol, f32):

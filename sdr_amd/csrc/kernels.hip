@@ -780,11 +780,9 @@ __global__ void __launch_bounds__(256) k_dot_finish(double* partials, int nblock
       break;
     case kOpResidual:
       sc->residual = v;
-      // the step's direction (written by the fused product) is the current one now, and
-      // its p += alpha s is pending (launch_flush_p)
+      // the step's direction (written by the fused product) is the current one now
       sc->spar ^= 1;
       sc->started = 1;
-      sc->ppend = 1;
       if (v <= sc->tol_eff) sc->done = 1;
       break;
     case kOpRhsNorm:
@@ -826,9 +824,12 @@ __global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, c
 // wavefront step (the loads of both cells are issued before either is stored).
 template <class T>
 __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T* __restrict__ r,
-                                                  const T* __restrict__ s, const T* __restrict__ as,
-                                                  const PcgScalars* sc, double* partials) {
+                                                  const T* s0, const T* __restrict__ as,
+                                                  const PcgScalars* sc, double* partials,
+                                                  const T* s1 = nullptr) {
   if (sc->done) return;
+  // s1: s is the direction the fused product just wrote (sbuf[1 - spar])
+  const T* __restrict__ s = s1 && !sc->spar ? s1 : s0;
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
@@ -913,74 +914,41 @@ __global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__
   block_partial<1>(m, partials);
 }
 
-// r -= alpha As (pressure.rs:375) with the partials of max|r|: the fast mode's axpy; its
-// p update (launch_flush_p) is off the critical path.
+// k_axpy_pr_units with s the direction the fused product just wrote (sbuf[1 - spar]).
 template <class T, int U>
-__global__ void __launch_bounds__(256) k_axpy_r_units(Shear sh, T* __restrict__ r, const T* __restrict__ as,
-                                                      const PcgScalars* sc, double* partials, int span) {
+__global__ void __launch_bounds__(256) k_axpy_pr_par(Shear sh, T* __restrict__ p, T* __restrict__ r,
+                                                     const T* s0, const T* s1, const T* __restrict__ as,
+                                                     const PcgScalars* sc, double* partials, int span) {
   if (sc->done) return;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
-  const rsrc_t rr = make_rsrc(r, n * SZ), rq = make_rsrc(as, n * SZ);
+  const T* s = sc->spar ? s0 : s1;
+  const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
+               rq = make_rsrc(as, n * SZ);
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   int q0, q1;
   unit_span(sh, span, &q0, &q1);
   for (int q = q0; q < q1; q += U) {
     uint32_t t[U];
-    T rv[U], qv[U];
+    T pv[U], sv[U], rv[U], qv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       t[u] = unit_slot(sh, q + u, q1).t;
+      pv[u] = bld<T>(rp, t[u] * SZ);
+      sv[u] = bld<T>(rs, t[u] * SZ);
       rv[u] = bld<T>(rr, t[u] * SZ);
       qv[u] = bld<T>(rq, t[u] * SZ);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      bst<T>(rp, t[u] * SZ, pv[u] + alpha * sv[u]);
       const T nr = rv[u] - alpha * qv[u];
       bst<T>(rr, t[u] * SZ, nr);
       m = nmax(m, fabs((double)nr));  // 0 outside the grid
     }
   }
   block_partial<1>(m, partials);
-}
-// The same over row pairs (grids too large for the units' 32-bit offsets).
-template <class T>
-__global__ void __launch_bounds__(256) k_axpy_r2(Shear sh, T* __restrict__ r, const T* __restrict__ as,
-                                                 const PcgScalars* sc, double* partials) {
-  if (sc->done) return;
-  const T alpha = (T)sc->alpha;
-  double m = 0.0;
-  for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
-    const int64_t ta = a.base + ai, tb = b.base + bi;
-    T ra = T(0), qa = T(0), rb = T(0), qb = T(0);
-    if (ao) ra = r[ta], qa = as[ta];
-    if (bo) rb = r[tb], qb = as[tb];
-    if (ao) {
-      const T rv = ra - alpha * qa;
-      r[ta] = rv;
-      m = nmax(m, fabs((double)rv));
-    }
-    if (bo) {
-      const T rv = rb - alpha * qb;
-      r[tb] = rv;
-      m = nmax(m, fabs((double)rv));
-    }
-  });
-  block_partial<1>(m, partials);
-}
-// p += alpha s (pressure.rs:374) of the step whose residual update set the pending flag
-// (s = sbuf[spar]); k_ppend_clear then clears the flag.
-template <class T>
-__global__ void __launch_bounds__(256) k_flush_p(Shear sh, T* __restrict__ p, const T* s0, const T* s1,
-                                                 const PcgScalars* sc) {
-  if (!sc->ppend) return;
-  const T* __restrict__ s = sc->spar ? s1 : s0;
-  const T alpha = (T)sc->alpha;
-  for_cells(sh, [&](int64_t t, int, int, int) { p[t] = p[t] + alpha * s[t]; });
-}
-__global__ void k_ppend_clear(PcgScalars* sc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) sc->ppend = 0;
 }
 
 // ------------------------------------------------------------------ MIC(0) (pressure.rs:442-570)
@@ -1728,18 +1696,13 @@ void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s
   k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials, tile25(sh));
 }
 template <class T>
-void launch_axpy_r(Shear sh, T* r, const T* as, const PcgScalars* sc, double* partials, int nblocks,
-                   hipStream_t st) {
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
+                        double* partials, int nblocks, hipStream_t st) {
   if (units_fit<T>(sh)) {
-    k_axpy_r_units<T, 4><<<nblocks, 256, 0, st>>>(sh, r, as, sc, partials, unit_span_len(sh, nblocks, 4));
+    k_axpy_pr_par<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s0, s1, as, sc, partials, unit_span_len(sh, nblocks, 4));
     return;
   }
-  k_axpy_r2<T><<<nblocks, 256, 0, st>>>(sh, r, as, sc, partials);
-}
-template <class T>
-void launch_flush_p(Shear sh, T* p, const T* s0, const T* s1, PcgScalars* sc, hipStream_t st) {
-  k_flush_p<T><<<cell_blocks(sh), 256, 0, st>>>(sh, p, s0, s1, sc);
-  k_ppend_clear<<<1, 64, 0, st>>>(sc);
+  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s0, as, sc, partials, s1);
 }
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
@@ -1976,8 +1939,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_update_s<T>(Shear, T*, const T*, const PcgScalars*, hipStream_t);           \
   template void launch_spmv_update<T>(Shear, const uint16_t*, const T*, T*, T*, T*, PcgScalars*,     \
                                       double*, hipStream_t);                                       \
-  template void launch_axpy_r<T>(Shear, T*, const T*, const PcgScalars*, double*, int, hipStream_t);  \
-  template void launch_flush_p<T>(Shear, T*, const T*, const T*, PcgScalars*, hipStream_t);          \
+  template void launch_axpy_pr_par<T>(Shear, T*, T*, const T*, const T*, const T*, const PcgScalars*, \
+                                      double*, int, hipStream_t);                                  \
   template void launch_mic_plane<T>(int, Shear, const uint16_t*, T*, const T*, T*, int,            \
                                     const PcgScalars*, hipStream_t);                               \
   template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \

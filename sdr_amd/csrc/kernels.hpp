@@ -34,11 +34,6 @@ struct PcgScalars {
   // holding the current s; started is 0 until the first step, whose product takes s = z.
   int32_t spar;
   int32_t started;
-  // The fast mode applies p += alpha s (pressure.rs:374) in a pass of its own beside the
-  // next preconditioner solve: ppend is set once a step's residual update has run,
-  // cleared when that p update is done.
-  int32_t ppend;
-  int32_t pad2;
 };
 
 enum DotOp : int32_t {
@@ -129,14 +124,10 @@ template <class T>
 void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
                         double* partials, hipStream_t st);
 int spmv_update_blocks(Shear sh);
-// r -= alpha As with the partials of max|r| (the fast mode's p update is launch_flush_p).
+// p += alpha s; r -= alpha As with s the direction just written by launch_spmv_update.
 template <class T>
-void launch_axpy_r(Shear sh, T* r, const T* as, const PcgScalars* sc, double* partials, int nblocks,
-                   hipStream_t st);
-// p += alpha s of the last step (s = sbuf[spar]) if its update is pending; clears the
-// pending flag.
-template <class T>
-void launch_flush_p(Shear sh, T* p, const T* s0, const T* s1, PcgScalars* sc, hipStream_t st);
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
+                        double* partials, int nblocks, hipStream_t st);
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
                       int sigma, const PcgScalars* sc, hipStream_t st);

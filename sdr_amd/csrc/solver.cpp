@@ -187,22 +187,9 @@ struct Projector {
   double g_pcg_tol = 0.0;
   int64_t ncells() const { return (int64_t)nx * ny * nz; }
 
-  // The fast mode's p += alpha s (pressure.rs:374) runs on a stream of its own beside the
-  // preconditioner solve, off the critical path: only the next product waits for it.  The
-  // no-alloc projection creates no stream and runs it in line.
-  hipStream_t pst = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool p_side = false;  // a p update on pst that st has not waited for yet
-
   ~Projector() {
     for (hipGraphExec_t g : {g_pcg, g_rel_first, g_rel_block})
       if (g) (void)hipGraphExecDestroy(g);
-    if (pst) {
-      (void)hipStreamSynchronize(pst);
-      (void)hipEventDestroy(ev_fork);
-      (void)hipEventDestroy(ev_join);
-      (void)hipStreamDestroy(pst);
-    }
   }
   bool use_graphs() const { return graph_ok && !(timer && timer->on); }
   // Capture the launches `body` issues on the stream into an executable graph.
@@ -210,11 +197,9 @@ struct Projector {
   hipGraphExec_t capture(F&& body) {
     hipGraph_t g = nullptr;
     // relaxed: solvers on other threads keep allocating and copying meanwhile
-    join_p();  // the graph waits on no event recorded before it
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     try {
       body();
-      join_p();  // the side stream's last update joins the graph
     } catch (...) {
       (void)hipStreamEndCapture(st, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -296,11 +281,6 @@ struct Projector {
     pipe_err.alloc(sizeof(int));
     sink.alloc(1024 * sizeof(T));
     s2.alloc(pl.n * sizeof(T));
-    if (!pst) {
-      HIPCHK(hipStreamCreateWithFlags(&pst, hipStreamNonBlocking));
-      HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-      HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    }
     HIPCHK(hipMemsetAsync(halo_x.p, 0, halo_x.bytes, st));
     HIPCHK(hipMemsetAsync(halo_z.p, 0, halo_z.bytes, st));
     HIPCHK(hipMemsetAsync(pipe_err.p, 0, sizeof(int), st));
@@ -420,31 +400,12 @@ struct Projector {
   // As = A s with the partials of dot(s, As), then denom and alpha.  The new s goes to
   // the other direction buffer (sbuf = {s, s2}, PcgScalars::spar).
   void fused_product() {
-    join_p();  // the next product overwrites the direction the p update reads
     launch_spmv_update<T>(sh, flags.as<uint16_t>(), z.as<T>(), s.as<T>(), s2.as<T>(), as.as<T>(), sc(),
                           partials.as<double>(), st);
     launch_dot_finish(partials.as<double>(), spmv_update_blocks(sh), false, kOpDenom, sc(), 0.0, st);
   }
   // The current direction buffer (fused: by the polled scalars).
   Buf& cur_s(const PcgScalars& h) { return fused() && h.spar ? s2 : s; }
-  // p += alpha s of the step whose residual update was just launched (fused mode).
-  void p_update() {
-    if (!pst) {
-      launch_flush_p<T>(sh, p.as<T>(), s.as<T>(), s2.as<T>(), sc(), st);
-      return;
-    }
-    HIPCHK(hipEventRecord(ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(pst, ev_fork, 0));
-    launch_flush_p<T>(sh, p.as<T>(), s.as<T>(), s2.as<T>(), sc(), pst);
-    HIPCHK(hipEventRecord(ev_join, pst));
-    p_side = true;
-  }
-  // st waits for the last p update (before p or the direction it reads changes or is read).
-  void join_p() {
-    if (!p_side) return;
-    HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
-    p_side = false;
-  }
 
   PcgScalars read_scalars() {
     if (noalloc) {  // no pinned staging: a pageable copy
@@ -501,10 +462,10 @@ struct Projector {
           fused_product();
         }
         {
-          Scope sc1(*timer, "axpy_r");
-          launch_axpy_r<T>(sh, r, as.as<T>(), sc(), partials.as<double>(), mb, st);
+          Scope sc1(*timer, "axpy_norm");
+          launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
+                                mb, st);
           launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
-          p_update();
         }
         {
           Scope sc1(*timer, "mic_fwd");
@@ -568,7 +529,6 @@ struct Projector {
       it += nblk;
       if (it < max_iters && read_scalars().done) break;
     }
-    join_p();
     return read_scalars();
   }
 
@@ -708,10 +668,10 @@ struct Projector {
           Scope sc1(*timer, "spmv_update");
           fused_product();
         }
-        Scope sc1(*timer, "axpy_r");
-        launch_axpy_r<T>(sh, r, as.as<T>(), sc(), partials.as<double>(), mb, st);
+        Scope sc1(*timer, "axpy_norm");
+        launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
+                              mb, st);
         launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), 0.0, st);
-        p_update();
         return;
       }
       {
@@ -787,7 +747,6 @@ struct Projector {
         const bool reached = sc_h.residual <= sc_h.tol_eff;
         if (sc_h.done || sc_h.residual <= kReplace) {
           Scope sc1(*timer, "refine");
-          join_p();
           launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
           zero(p, n * sizeof(T));
           const double rn = true_residual();
@@ -826,7 +785,6 @@ struct Projector {
       }
     }
     if (!converged && !tot.nan_seen) {  // the cap hit: fold in what the f32 run has
-      join_p();
       launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
       zero(p, n * sizeof(T));
       rnorm = true_residual();
