@@ -1080,16 +1080,13 @@ __global__ void __launch_bounds__(256) k_residual64(Shear sh, const uint16_t* __
                                                     const double* __restrict__ x, double* __restrict__ r,
                                                     double* partials) {
   __shared__ double red[4];
-  const int64_t n = sh.size();
   const int64_t row = sh.nx, plane = (int64_t)sh.nx * sh.NJ;
   double m = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+  // in-grid slots only: r's slots outside the grid hold +0 from its allocation on
+  for_cells(sh, [&](int64_t t, int i, int J, int k) {
     const uint16_t f = flags[t];
     double v = 0.0;
     if (f & kFlFluid) {
-      const int i = (int)(t % sh.nx);
-      const int J = (int)((t / sh.nx) % sh.NJ);
-      const int k = (int)(t / plane);
       const int j = J - i;
       double ax = (double)fl_diag(f) * x[t];
       if (i + 1 < sh.nx) ax += ((f & kFlPx) ? -1.0 : 0.0) * x[t + row + 1];
@@ -1102,11 +1099,127 @@ __global__ void __launch_bounds__(256) k_residual64(Shear sh, const uint16_t* __
     }
     r[t] = v;
     m = nmax(m, fabs(v));
-  }
+  });
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) partials[blockIdx.x] = nmax(nmax(nmax(red[0], red[1]), red[2]), red[3]);
+}
+
+// k_residual64 over work units (the addressing of k_spmv_units: U units of 64 slots in
+// flight per wave, 32-bit byte offsets, lanes outside the grid at kFarElem, whose loads
+// read 0 and whose stores are dropped), the same arithmetic in the same order.
+template <int U>
+__global__ void __launch_bounds__(256) k_residual64_units(Shear sh, const uint16_t* __restrict__ flags,
+                                                          const double* __restrict__ b,
+                                                          const double* __restrict__ x, double* __restrict__ r,
+                                                          double* partials, int span) {
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rf = make_rsrc(flags, n * 2u), rx = make_rsrc(x, n * 8u), rb = make_rsrc(b, n * 8u),
+               rr = make_rsrc(r, n * 8u);
+  const int nx = sh.nx, ny = sh.ny, nz = sh.nz;
+  const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)sh.NJ;
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  double m = 0.0;
+  for (int q = q0; q < q1; q += U) {
+    UnitLane L[U];
+    uint16_t f[U];
+    double xc[U], bv[U], xn[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      L[u] = unit_slot(sh, q + u, q1);
+      const uint32_t t = L[u].t;
+      const uint32_t on[6] = {t + row + 1, t - row - 1, t + row, t - row, t + plane, t - plane};
+      f[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rf, t * 2u, 0, 0);
+      xc[u] = bld<double>(rx, t * 8u);
+      bv[u] = bld<double>(rb, t * 8u);
+#pragma unroll
+      for (int d = 0; d < 6; ++d) xn[u][d] = bld<double>(rx, on[d] * 8u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint16_t fu = f[u];
+      const int i = L[u].i, j = L[u].j, k = L[u].k;
+      double ax = (double)fl_diag(fu) * xc[u];
+      const double a0 = ax + ((fu & kFlPx) ? -1.0 : 0.0) * xn[u][0];
+      ax = i + 1 < nx ? a0 : ax;
+      const double a1 = ax + ((fu & kFlMx) ? -1.0 : 0.0) * xn[u][1];
+      ax = i >= 1 ? a1 : ax;
+      const double a2 = ax + ((fu & kFlPy) ? -1.0 : 0.0) * xn[u][2];
+      ax = j + 1 < ny ? a2 : ax;
+      const double a3 = ax + ((fu & kFlMy) ? -1.0 : 0.0) * xn[u][3];
+      ax = j >= 1 ? a3 : ax;
+      const double a4 = ax + ((fu & kFlPz) ? -1.0 : 0.0) * xn[u][4];
+      ax = k + 1 < nz ? a4 : ax;
+      const double a5 = ax + ((fu & kFlMz) ? -1.0 : 0.0) * xn[u][5];
+      ax = k >= 1 ? a5 : ax;
+      const double v = (fu & kFlFluid) ? bv[u] - ax : 0.0;
+      bst<double>(rr, L[u].t * 8u, v);
+      m = nmax(m, fabs(v));
+    }
+  }
+  block_partial<1>(m, partials);
+}
+
+// The elementwise steps of the reliable updates over work units (as above).
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_scale_cast_units(Shear sh, const double* __restrict__ r, double inv,
+                                                          T* __restrict__ b, int span) {
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rr = make_rsrc(r, n * 8u), rb = make_rsrc(b, n * (uint32_t)sizeof(T));
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      v[u] = bld<double>(rr, t[u] * 8u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) bst<T>(rb, t[u] * (uint32_t)sizeof(T), (T)(v[u] * inv));
+  }
+}
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_scale_inplace_units(Shear sh, T* __restrict__ x, double f, int span) {
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rx = make_rsrc(x, n * (uint32_t)sizeof(T));
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      v[u] = bld<T>(rx, t[u] * (uint32_t)sizeof(T));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) bst<T>(rx, t[u] * (uint32_t)sizeof(T), (T)((double)v[u] * f));
+  }
+}
+template <class T, int U>
+__global__ void __launch_bounds__(256) k_accum64_units(Shear sh, double* __restrict__ p, const T* __restrict__ e,
+                                                       double scale, int span) {
+  const uint32_t n = (uint32_t)sh.size();
+  const rsrc_t rp = make_rsrc(p, n * 8u), re = make_rsrc(e, n * (uint32_t)sizeof(T));
+  int q0, q1;
+  unit_span(sh, span, &q0, &q1);
+  for (int q = q0; q < q1; q += U) {
+    uint32_t t[U];
+    double pv[U];
+    T ev[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = unit_slot(sh, q + u, q1).t;
+      pv[u] = bld<double>(rp, t[u] * 8u);
+      ev[u] = bld<T>(re, t[u] * (uint32_t)sizeof(T));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) bst<double>(rp, t[u] * 8u, pv[u] + scale * (double)ev[u]);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_max_finish(const double* partials, int nblocks, double* out) {
@@ -1121,16 +1234,14 @@ __global__ void __launch_bounds__(256) k_max_finish(const double* partials, int 
 
 // rhs of a correction solve: b = r * inv (rounded to the storage type)
 template <class T>
-__global__ void k_scale_cast(int64_t n, const double* __restrict__ r, double inv, T* __restrict__ b) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) b[t] = (T)(r[t] * inv);
+__global__ void k_scale_cast(Shear sh, const double* __restrict__ r, double inv, T* __restrict__ b) {
+  for_cells(sh, [&](int64_t t, int, int, int) { b[t] = (T)(r[t] * inv); });
 }
 
 // x *= f (the search direction carried into new residual units)
 template <class T>
-__global__ void k_scale_inplace(int64_t n, T* __restrict__ x, double f) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) x[t] = (T)((double)x[t] * f);
+__global__ void k_scale_inplace(Shear sh, T* __restrict__ x, double f) {
+  for_cells(sh, [&](int64_t t, int, int, int) { x[t] = (T)((double)x[t] * f); });
 }
 
 // The PCG scalars after a residual replacement: sigma into the new residual units (or
@@ -1145,9 +1256,8 @@ __global__ void k_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int res
 
 // p += scale * e in f64
 template <class T>
-__global__ void k_accum64(int64_t n, double* __restrict__ p, const T* __restrict__ e, double scale) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] += scale * (double)e[t];
+__global__ void k_accum64(Shear sh, double* __restrict__ p, const T* __restrict__ e, double scale) {
+  for_cells(sh, [&](int64_t t, int, int, int) { p[t] += scale * (double)e[t]; });
 }
 
 // ------------------------------------------------------------------ extrapolate (solver.rs:608-647, 960-1009)
@@ -1740,23 +1850,38 @@ void launch_unshear64(Shear sh, const double* p, T* out, hipStream_t st) {
 }
 void launch_residual64(Shear sh, const uint16_t* flags, const double* b, const double* x, double* r,
                        double* partials, int nblocks, double* out, hipStream_t st) {
-  k_residual64<<<nblocks, 256, 0, st>>>(sh, flags, b, x, r, partials);
+  if (units_fit<double>(sh))
+    k_residual64_units<4><<<nblocks, 256, 0, st>>>(sh, flags, b, x, r, partials, unit_span_len(sh, nblocks, 4));
+  else
+    k_residual64<<<nblocks, 256, 0, st>>>(sh, flags, b, x, r, partials);
   k_max_finish<<<1, 256, 0, st>>>(partials, nblocks, out);
 }
 template <class T>
-void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t st) {
-  k_scale_cast<T><<<grid_for(n), kBlock, 0, st>>>(n, r, inv, b);
+void launch_scale_cast(Shear sh, const double* r, double inv, T* b, hipStream_t st) {
+  const int nb = cell_blocks(sh);
+  if (units_fit<double>(sh))
+    k_scale_cast_units<T, 4><<<nb, 256, 0, st>>>(sh, r, inv, b, unit_span_len(sh, nb, 4));
+  else
+    k_scale_cast<T><<<nb, kBlock, 0, st>>>(sh, r, inv, b);
 }
 template <class T>
-void launch_scale_inplace(int64_t n, T* x, double f, hipStream_t st) {
-  k_scale_inplace<T><<<grid_for(n), kBlock, 0, st>>>(n, x, f);
+void launch_scale_inplace(Shear sh, T* x, double f, hipStream_t st) {
+  const int nb = cell_blocks(sh);
+  if (units_fit<T>(sh))
+    k_scale_inplace_units<T, 4><<<nb, 256, 0, st>>>(sh, x, f, unit_span_len(sh, nb, 4));
+  else
+    k_scale_inplace<T><<<nb, kBlock, 0, st>>>(sh, x, f);
 }
 void launch_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int restart, hipStream_t st) {
   k_pcg_rescale<<<1, 64, 0, st>>>(sc, f2, tol_eff, restart);
 }
 template <class T>
-void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st) {
-  k_accum64<T><<<grid_for(n), kBlock, 0, st>>>(n, p, e, scale);
+void launch_accum64(Shear sh, double* p, const T* e, double scale, hipStream_t st) {
+  const int nb = cell_blocks(sh);
+  if (units_fit<double>(sh))
+    k_accum64_units<T, 4><<<nb, 256, 0, st>>>(sh, p, e, scale, unit_span_len(sh, nb, 4));
+  else
+    k_accum64<T><<<nb, kBlock, 0, st>>>(sh, p, e, scale);
 }
 template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st) {
@@ -1949,9 +2074,9 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_gradient64<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const double*,     \
                                      double, hipStream_t);                                         \
   template void launch_unshear64<T>(Shear, const double*, T*, hipStream_t);                        \
-  template void launch_scale_cast<T>(int64_t, const double*, double, T*, hipStream_t);             \
-  template void launch_accum64<T>(int64_t, double*, const T*, double, hipStream_t);                \
-  template void launch_scale_inplace<T>(int64_t, T*, double, hipStream_t);                         \
+  template void launch_scale_cast<T>(Shear, const double*, double, T*, hipStream_t);               \
+  template void launch_accum64<T>(Shear, double*, const T*, double, hipStream_t);                  \
+  template void launch_scale_inplace<T>(Shear, T*, double, hipStream_t);                           \
   template void launch_extrap_init<T>(GridD<T>, int, const uint8_t*, uint8_t*, hipStream_t);       \
   template void launch_extrap_iter<T>(int, int, int, const T*, T*, const uint8_t*, uint8_t*,       \
                                       hipStream_t);                                                \

@@ -145,12 +145,13 @@ void launch_unshear64(Shear sh, const double* p, T* out, hipStream_t st);
 // r = b - A x and *out = max|r| (nblocks partials)
 void launch_residual64(Shear sh, const uint16_t* flags, const double* b, const double* x, double* r,
                        double* partials, int nblocks, double* out, hipStream_t st);
+// Over the in-grid slots of the sheared layout (the others are left as they are: +0).
 template <class T>
-void launch_scale_cast(int64_t n, const double* r, double inv, T* b, hipStream_t st);
+void launch_scale_cast(Shear sh, const double* r, double inv, T* b, hipStream_t st);
 template <class T>
-void launch_accum64(int64_t n, double* p, const T* e, double scale, hipStream_t st);
+void launch_accum64(Shear sh, double* p, const T* e, double scale, hipStream_t st);
 template <class T>
-void launch_scale_inplace(int64_t n, T* x, double f, hipStream_t st);
+void launch_scale_inplace(Shear sh, T* x, double f, hipStream_t st);
 void launch_pcg_rescale(PcgScalars* sc, double f2, double tol_eff, int restart, hipStream_t st);
 template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st);

@@ -556,7 +556,7 @@ struct Projector {
     for (int outer = 0; outer < 16 && rnorm > rel_tol * bnorm && used < max_iters; ++outer) {
       {
         Scope sc_(*timer, "refine");
-        launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / rnorm, b.as<T>(), st);
+        launch_scale_cast<T>(sh, r64.as<double>(), 1.0 / rnorm, b.as<T>(), st);
       }
       const double inner = std::max(kInner, 0.5 * rel_tol * bnorm / rnorm);
       PcgScalars r = pcg(max_iters - used, inner, false, first);
@@ -566,7 +566,7 @@ struct Projector {
       tot.nan_seen |= r.nan_seen;
       {
         Scope sc_(*timer, "refine");
-        launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), rnorm, st);
+        launch_accum64<T>(sh, p64.as<double>(), p.as<T>(), rnorm, st);
         launch_residual64(sh, flags.as<uint16_t>(), b64.as<double>(), p64.as<double>(),
                           r64.as<double>(), partials.as<double>(), mb, norms.as<double>(), st);
         HIPCHK(hipMemcpyAsync(hn, norms.p, 8, hipMemcpyDeviceToHost, st));
@@ -631,7 +631,7 @@ struct Projector {
     T* r = b.as<T>();
     {
       Scope sc_(*timer, "refine");
-      launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / scale, r, st);
+      launch_scale_cast<T>(sh, r64.as<double>(), 1.0 / scale, r, st);
     }
     {
       Scope sc_(*timer, "pcg_setup");
@@ -747,7 +747,7 @@ struct Projector {
         const bool reached = sc_h.residual <= sc_h.tol_eff;
         if (sc_h.done || sc_h.residual <= kReplace) {
           Scope sc1(*timer, "refine");
-          launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
+          launch_accum64<T>(sh, p64.as<double>(), p.as<T>(), scale, st);
           zero(p, n * sizeof(T));
           const double rn = true_residual();
           ++replacements;
@@ -770,13 +770,13 @@ struct Projector {
           last = rn;
           const double f = scale / rn;
           scale = rn;
-          launch_scale_cast<T>((int64_t)n, r64.as<double>(), 1.0 / scale, r, st);
+          launch_scale_cast<T>(sh, r64.as<double>(), 1.0 / scale, r, st);
           Buf& sd = cur_s(sc_h);
           if (restart) {
             zero(sd, n * sizeof(T));
             ++restarts;
           } else {
-            launch_scale_inplace<T>((int64_t)n, sd.as<T>(), f, st);
+            launch_scale_inplace<T>(sh, sd.as<T>(), f, st);
           }
           launch_pcg_rescale(sc(), f * f, target / scale, restart ? 1 : 0, st);
         } else if (sc_h.done) {
@@ -785,7 +785,7 @@ struct Projector {
       }
     }
     if (!converged && !tot.nan_seen) {  // the cap hit: fold in what the f32 run has
-      launch_accum64<T>((int64_t)n, p64.as<double>(), p.as<T>(), scale, st);
+      launch_accum64<T>(sh, p64.as<double>(), p.as<T>(), scale, st);
       zero(p, n * sizeof(T));
       rnorm = true_residual();
     }
@@ -810,6 +810,8 @@ struct Projector {
     if (ref && !p64.p) {
       const size_t n = (size_t)sh.size();
       b64.alloc(n * 8), p64.alloc(n * 8), r64.alloc(n * 8), norms.alloc(64);
+      // the residual kernel writes in-grid slots only; the others stay +0
+      HIPCHK(hipMemsetAsync(r64.p, 0, n * 8, st));
     }
     {
       Scope sc_(*timer, "rhs");
