@@ -546,7 +546,7 @@ struct Tile25 {
 };
 inline Tile25 tile25(Shear sh) {
   Tile25 t;
-  t.kc = sh.nz >= 128 ? 16 : 8;
+  t.kc = sh.nz >= 256 ? 32 : sh.nz >= 128 ? 16 : 8;
   t.nti = (sh.nx + 63) / 64;
   t.ntj = (sh.NJ + kR - 1) / kR;
   t.ntk = (sh.nz + t.kc - 1) / t.kc;
