@@ -133,6 +133,42 @@ __device__ inline T trilinear_diff(const T* __restrict__ a, const T* __restrict_
   return c0 * (T(1) - fz) + c1 * fz;
 }
 
+// trilinear(a, ...) and trilinear_diff(a, b, ...) at one point from one set of corner
+// loads of a (the arithmetic of the two functions, value for value).
+template <class T>
+__device__ inline void trilinear_and_diff(const T* __restrict__ a, const T* __restrict__ b, int nx, int ny,
+                                          int nz, T gx, T gy, T gz, T* va, T* vd) {
+  int i, j, k;
+  T fx, fy, fz;
+  clamp_base(gx, nx, i, fx);
+  clamp_base(gy, ny, j, fy);
+  clamp_base(gz, nz, k, fz);
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const int64_t o = (int64_t)k * sz + (int64_t)j * sy + i;
+  const T* p = a + o;
+  const T* q = b + o;
+  const T a000 = p[0], a100 = p[1], a010 = p[sy], a110 = p[sy + 1];
+  const T a001 = p[sz], a101 = p[sz + 1], a011 = p[sz + sy], a111 = p[sz + sy + 1];
+  const T b000 = q[0], b100 = q[1], b010 = q[sy], b110 = q[sy + 1];
+  const T b001 = q[sz], b101 = q[sz + 1], b011 = q[sz + sy], b111 = q[sz + sy + 1];
+  {
+    T c00 = a000 * (T(1) - fx) + a100 * fx;
+    T c10 = a010 * (T(1) - fx) + a110 * fx;
+    T c01 = a001 * (T(1) - fx) + a101 * fx;
+    T c11 = a011 * (T(1) - fx) + a111 * fx;
+    T c0 = c00 * (T(1) - fy) + c10 * fy;
+    T c1 = c01 * (T(1) - fy) + c11 * fy;
+    *va = c0 * (T(1) - fz) + c1 * fz;
+  }
+  T c00 = (a000 - b000) * (T(1) - fx) + (a100 - b100) * fx;
+  T c10 = (a010 - b010) * (T(1) - fx) + (a110 - b110) * fx;
+  T c01 = (a001 - b001) * (T(1) - fx) + (a101 - b101) * fx;
+  T c11 = (a011 - b011) * (T(1) - fx) + (a111 - b111) * fx;
+  T c0 = c00 * (T(1) - fy) + c10 * fy;
+  T c1 = c01 * (T(1) - fy) + c11 * fy;
+  *vd = c0 * (T(1) - fz) + c1 * fz;
+}
+
 // Geometry of the MAC grid on the device.
 template <class T>
 struct GridD {
@@ -237,6 +273,22 @@ __device__ inline V3<T> project_inside(const SdfD<T>& s, T grid_dx, V3<T> p) {
     p = p - g * (phi + margin);
   }
   return p;
+}
+// advect's last step (solver.rs:706-708): project_inside, then keep the projected point
+// if the SDF there is <= 0, else the start.  A loop that stops on a sample has that
+// sample at the point it returns, so the SDF is sampled there again only after the
+// third move.
+template <class T>
+__device__ inline V3<T> project_inside_or_stay(const SdfD<T>& s, T grid_dx, V3<T> p, V3<T> p0) {
+  T margin = T(0.5) * grid_dx;
+  for (int it = 0; it < 3; ++it) {
+    T phi = s.sample(p);
+    if (phi <= -margin) return phi <= T(0) ? p : p0;
+    V3<T> g = normalize_or_zero(s.gradient(p));
+    if (len3(g) == T(0)) return phi <= T(0) ? p : p0;
+    p = p - g * (phi + margin);
+  }
+  return s.sample(p) <= T(0) ? p : p0;
 }
 
 // solver.rs:916-930 (splitmix64 finaliser -> [-0.3, 0.3)).

@@ -1312,11 +1312,13 @@ __global__ void k_g2p(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   V3<T> x = load_pos(p, t);
-  V3<T> pic = velocity_at(g, f, x);
+  // velocity_at(g, f, x) and the trilinear_diff samples from one set of corner loads
   V3<T> l = g.local(x);
-  T dvx = trilinear_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5));
-  T dvy = trilinear_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5));
-  T dvz = trilinear_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z);
+  T pa, pb, pc, dvx, dvy, dvz;
+  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx);
+  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy);
+  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz);
+  V3<T> pic = mk3(pa, pb, pc);
   V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
   V3<T> nv = fv * flip + pic * (T(1) - flip);
   p.vx[t] = nv.x;
@@ -1336,8 +1338,7 @@ __global__ void k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T
   V3<T> st = vm * dt;
   T len = len3(st);
   if (len > g.dx) st = st * (g.dx / len);
-  V3<T> q = project_inside(sdf, g.dx, p0 + st);
-  V3<T> r = sdf.sample(q) <= T(0) ? q : p0;
+  V3<T> r = project_inside_or_stay(sdf, g.dx, p0 + st, p0);
   p.px[t] = r.x;
   p.py[t] = r.y;
   p.pz[t] = r.z;
@@ -1352,11 +1353,13 @@ __global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CF
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const V3<T> p0 = load_pos(p, t);
-  const V3<T> pic = velocity_at(g, f, p0);
+  // velocity_at(g, f, p0) and the trilinear_diff samples from one set of corner loads
   const V3<T> l = g.local(p0);
-  T dvx = trilinear_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5));
-  T dvy = trilinear_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5));
-  T dvz = trilinear_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z);
+  T pa, pb, pc, dvx, dvy, dvz;
+  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx);
+  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy);
+  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz);
+  const V3<T> pic = mk3(pa, pb, pc);
   const V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
   const V3<T> nv = fv * flip + pic * (T(1) - flip);
   p.vx[t] = nv.x;
@@ -1367,8 +1370,7 @@ __global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CF
   V3<T> st = vm * dt;
   const T len = len3(st);
   if (len > g.dx) st = st * (g.dx / len);
-  const V3<T> q = project_inside(sdf, g.dx, p0 + st);
-  const V3<T> r = sdf.sample(q) <= T(0) ? q : p0;
+  const V3<T> r = project_inside_or_stay(sdf, g.dx, p0 + st, p0);
   p.px[t] = r.x;
   p.py[t] = r.y;
   p.pz[t] = r.z;
