@@ -133,11 +133,19 @@ __device__ inline T trilinear_diff(const T* __restrict__ a, const T* __restrict_
   return c0 * (T(1) - fz) + c1 * fz;
 }
 
+// The eight corner values of a trilinear stencil and the offset of its base corner.
+template <class T>
+struct Corners {
+  int64_t o;
+  T c[8];  // x fastest: (0,0,0), (1,0,0), (0,1,0), (1,1,0), (0,0,1), ...
+};
+
 // trilinear(a, ...) and trilinear_diff(a, b, ...) at one point from one set of corner
-// loads of a (the arithmetic of the two functions, value for value).
+// loads of a (the arithmetic of the two functions, value for value); a's corners are
+// left in *keep for trilinear_again.
 template <class T>
 __device__ inline void trilinear_and_diff(const T* __restrict__ a, const T* __restrict__ b, int nx, int ny,
-                                          int nz, T gx, T gy, T gz, T* va, T* vd) {
+                                          int nz, T gx, T gy, T gz, T* va, T* vd, Corners<T>* keep) {
   int i, j, k;
   T fx, fy, fz;
   clamp_base(gx, nx, i, fx);
@@ -149,6 +157,9 @@ __device__ inline void trilinear_and_diff(const T* __restrict__ a, const T* __re
   const T* q = b + o;
   const T a000 = p[0], a100 = p[1], a010 = p[sy], a110 = p[sy + 1];
   const T a001 = p[sz], a101 = p[sz + 1], a011 = p[sz + sy], a111 = p[sz + sy + 1];
+  keep->o = o;
+  keep->c[0] = a000, keep->c[1] = a100, keep->c[2] = a010, keep->c[3] = a110;
+  keep->c[4] = a001, keep->c[5] = a101, keep->c[6] = a011, keep->c[7] = a111;
   const T b000 = q[0], b100 = q[1], b010 = q[sy], b110 = q[sy + 1];
   const T b001 = q[sz], b101 = q[sz + 1], b011 = q[sz + sy], b111 = q[sz + sy + 1];
   {
@@ -167,6 +178,35 @@ __device__ inline void trilinear_and_diff(const T* __restrict__ a, const T* __re
   T c0 = c00 * (T(1) - fy) + c10 * fy;
   T c1 = c01 * (T(1) - fy) + c11 * fy;
   *vd = c0 * (T(1) - fz) + c1 * fz;
+}
+
+// trilinear(a, ...) (the same arithmetic), taking the corners from `kept` when the
+// stencil's base is the same and loading them otherwise.
+template <class T>
+__device__ inline T trilinear_again(const T* __restrict__ a, int nx, int ny, int nz, T gx, T gy, T gz,
+                                    const Corners<T>& kept) {
+  int i, j, k;
+  T fx, fy, fz;
+  clamp_base(gx, nx, i, fx);
+  clamp_base(gy, ny, j, fy);
+  clamp_base(gz, nz, k, fz);
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const int64_t o = (int64_t)k * sz + (int64_t)j * sy + i;
+  T v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = kept.c[q];
+  if (o != kept.o) {
+    const T* p = a + o;
+    v[0] = p[0], v[1] = p[1], v[2] = p[sy], v[3] = p[sy + 1];
+    v[4] = p[sz], v[5] = p[sz + 1], v[6] = p[sz + sy], v[7] = p[sz + sy + 1];
+  }
+  T c00 = v[0] * (T(1) - fx) + v[1] * fx;
+  T c10 = v[2] * (T(1) - fx) + v[3] * fx;
+  T c01 = v[4] * (T(1) - fx) + v[5] * fx;
+  T c11 = v[6] * (T(1) - fx) + v[7] * fx;
+  T c0 = c00 * (T(1) - fy) + c10 * fy;
+  T c1 = c01 * (T(1) - fy) + c11 * fy;
+  return c0 * (T(1) - fz) + c1 * fz;
 }
 
 // Geometry of the MAC grid on the device.

@@ -1308,16 +1308,17 @@ __global__ void k_extrap_iter(int nx, int ny, int nz, const T* __restrict__ src,
 
 // ------------------------------------------------------------------ G2P (solver.rs:650-677)
 template <class T>
-__global__ void k_g2p(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip) {
+__global__ void __launch_bounds__(256) k_g2p(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   V3<T> x = load_pos(p, t);
   // velocity_at(g, f, x) and the trilinear_diff samples from one set of corner loads
   V3<T> l = g.local(x);
   T pa, pb, pc, dvx, dvy, dvz;
-  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx);
-  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy);
-  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz);
+  Corners<T> ku, kv, kw;
+  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx, &ku);
+  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy, &kv);
+  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz, &kw);
   V3<T> pic = mk3(pa, pb, pc);
   V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
   V3<T> nv = fv * flip + pic * (T(1) - flip);
@@ -1328,7 +1329,7 @@ __global__ void k_g2p(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T
 
 // ------------------------------------------------------------------ advect (solver.rs:693-711)
 template <class T>
-__global__ void k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt) {
+__global__ void __launch_bounds__(256) k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, T dt) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   V3<T> p0 = load_pos(p, t);
@@ -1348,7 +1349,7 @@ __global__ void k_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, SdfD<T
 // of g2p is advect's first RK2 sample (same grid, same position), so the particle is
 // read once and that sample is taken once; every value is the one the two kernels give.
 template <class T>
-__global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip,
+__global__ void __launch_bounds__(256) k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> sv, T flip,
                              SdfD<T> sdf, T dt) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -1356,9 +1357,10 @@ __global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CF
   // velocity_at(g, f, p0) and the trilinear_diff samples from one set of corner loads
   const V3<T> l = g.local(p0);
   T pa, pb, pc, dvx, dvy, dvz;
-  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx);
-  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy);
-  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz);
+  Corners<T> ku, kv, kw;
+  trilinear_and_diff(f.u, sv.u, g.nx + 1, g.ny, g.nz, l.x, l.y - T(0.5), l.z - T(0.5), &pa, &dvx, &ku);
+  trilinear_and_diff(f.v, sv.v, g.nx, g.ny + 1, g.nz, l.x - T(0.5), l.y, l.z - T(0.5), &pb, &dvy, &kv);
+  trilinear_and_diff(f.w, sv.w, g.nx, g.ny, g.nz + 1, l.x - T(0.5), l.y - T(0.5), l.z, &pc, &dvz, &kw);
   const V3<T> pic = mk3(pa, pb, pc);
   const V3<T> fv = load_vel(p, t) + mk3(dvx, dvy, dvz);
   const V3<T> nv = fv * flip + pic * (T(1) - flip);
@@ -1366,7 +1368,17 @@ __global__ void k_g2p_advect(Parts<T> p, int64_t n, GridD<T> g, CFacesD<T> f, CF
   p.vy[t] = nv.y;
   p.vz[t] = nv.z;
   const V3<T> mid = p0 + pic * (T(0.5) * dt);
-  const V3<T> vm = velocity_at(g, f, mid);
+  // velocity_at(g, f, mid), in f32 reusing the corners of the first sample where the
+  // stencil base has not moved (in f64 the kept corners would spill)
+  V3<T> vm;
+  if constexpr (sizeof(T) == 4) {
+    const V3<T> lm = g.local(mid);
+    vm = mk3(trilinear_again(f.u, g.nx + 1, g.ny, g.nz, lm.x, lm.y - T(0.5), lm.z - T(0.5), ku),
+             trilinear_again(f.v, g.nx, g.ny + 1, g.nz, lm.x - T(0.5), lm.y, lm.z - T(0.5), kv),
+             trilinear_again(f.w, g.nx, g.ny, g.nz + 1, lm.x - T(0.5), lm.y - T(0.5), lm.z, kw));
+  } else {
+    vm = velocity_at(g, f, mid);
+  }
   V3<T> st = vm * dt;
   const T len = len3(st);
   if (len > g.dx) st = st * (g.dx / len);
