@@ -1571,9 +1571,9 @@ __device__ inline uint32_t wave_append(uint32_t* ctr, bool flag) {
 }
 
 template <class T>
-__global__ void k_redist_apply(Parts<T> p, const uint32_t* __restrict__ act,
+__global__ void __launch_bounds__(256) k_redist_apply(Parts<T> p, const uint32_t* __restrict__ act,
                                const int32_t* __restrict__ dest, int64_t nact, GridD<T> g,
-                               CFacesD<T> f, SdfD<T> sdf) {
+                               CFacesD<T> f, SdfD<T> sdf, int32_t* cellkey, uint32_t* count) {
   int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nact) return;
   int32_t nc = dest[a];
@@ -1591,6 +1591,16 @@ __global__ void k_redist_apply(Parts<T> p, const uint32_t* __restrict__ act,
   p.vx[slot] = v.x;
   p.vy[slot] = v.y;
   p.vz[slot] = v.z;
+  if (count) {
+    // The decision counted the particle in nc; the next sweep's histogram counts it in
+    // the cell its projected position lies in (integer atomics: order-free).
+    const int32_t c2 = (int32_t)g.cell_of(q);
+    if (c2 != nc) {
+      atomicSub(&count[nc], 1u);
+      if (c2 >= 0) atomicAdd(&count[c2], 1u);
+    }
+    cellkey[slot] = c2;
+  }
 }
 
 // ------------------------------------------------------------------ CFL (grid.rs:202-206)
@@ -1971,9 +1981,10 @@ void launch_redist_decide(const uint32_t* act, int64_t nact, const int32_t* cell
 }
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act, const int32_t* dest, int64_t nact,
-                         GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st) {
+                         GridD<T> g, CFacesD<T> f, SdfD<T> sdf, int32_t* cellkey, uint32_t* count,
+                         hipStream_t st) {
   if (nact <= 0) return;
-  k_redist_apply<T><<<grid_for(nact), kBlock, 0, st>>>(p, act, dest, nact, g, f, sdf);
+  k_redist_apply<T><<<grid_for(nact), kBlock, 0, st>>>(p, act, dest, nact, g, f, sdf, cellkey, count);
 }
 template <class T>
 void launch_maxabs_faces(const T* a, int64_t n, unsigned long long* out, hipStream_t st) {
@@ -2107,7 +2118,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
   template void launch_hist<T>(const Parts<T>&, int64_t, GridD<T>, int32_t*, uint32_t*,            \
                                hipStream_t);                                                       \
   template void launch_redist_apply<T>(const Parts<T>&, const uint32_t*, const int32_t*, int64_t,  \
-                                       GridD<T>, CFacesD<T>, SdfD<T>, hipStream_t);                \
+                                       GridD<T>, CFacesD<T>, SdfD<T>, int32_t*, uint32_t*,         \
+                                       hipStream_t);                                               \
   template void launch_maxabs_faces<T>(const T*, int64_t, unsigned long long*, hipStream_t);       \
   template void launch_to_aos<T>(const Parts<T>&, int64_t, double*, double*, hipStream_t);         \
   template void launch_to_aos_f32<T>(const Parts<T>&, int64_t, float*, hipStream_t);               \

@@ -212,9 +212,13 @@ void launch_rd_round2(int64_t max_pending, int parity, bool first, uint32_t* pen
                       int nx, int ny, int nz, uint32_t cap, RedistDirs dn, unsigned long long* resv,
                       unsigned long long* second, uint32_t epoch, int32_t* dest, uint32_t* ctr,
                       hipStream_t st);
+// The moves (solver.rs:546-552).  With count (and cellkey) given, the decision's counts
+// become the next sweep's histogram: a moved particle whose projected position lies
+// outside its destination cell is counted where it lies, and cellkey follows it.
 template <class T>
 void launch_redist_apply(const Parts<T>& p, const uint32_t* act_slot_sorted, const int32_t* dest,
-                         int64_t nact, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, hipStream_t st);
+                         int64_t nact, GridD<T> g, CFacesD<T> f, SdfD<T> sdf, int32_t* cellkey, uint32_t* count,
+                         hipStream_t st);
 template <class T>
 void launch_maxabs_faces(const T* a, int64_t n, unsigned long long* out_bits, hipStream_t st);
 template <class T>

@@ -1319,15 +1319,19 @@ struct Solver : SolverBase {
     pin.ensure(64);
     uint32_t* hn = (uint32_t*)pin.p;
     for (int sweep = 0; sweep < 4; ++sweep) {
-      HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
-      launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
+      // the histogram of the sweep (solver.rs:464-469); after the first, the previous
+      // sweep's decision counts corrected by its moves are that histogram
+      if (sweep == 0 || redist_sequential) {
+        HIPCHK(hipMemsetAsync(count.p, 0, (size_t)ncells() * 4, st));
+        launch_hist<T>(parts(cur), n, grid(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
+      }
       if (!redist_sequential) {
         auto [nact, moved] = decide(capc, dn);
         if (nact == 0) break;  // no cell over the cap: later sweeps see the same state
         redist_active += (uint64_t)nact;
         redist_moved += moved;
         launch_redist_apply<T>(parts(cur), act_slot.as<uint32_t>(), dest.as<int32_t>(), nact, grid(),
-                               cfaces(), sdf(), st);
+                               cfaces(), sdf(), cellkey.as<int32_t>(), count.as<uint32_t>(), st);
         continue;
       }
       // The single-thread reference loop over the active particles in reference order.
@@ -1351,7 +1355,7 @@ struct Solver : SolverBase {
       HIPCHK(hipStreamSynchronize(st));
       redist_moved += ((unsigned long long*)pin.p)[4];
       launch_redist_apply<T>(parts(cur), act_slot_sorted.as<uint32_t>(), dest.as<int32_t>(), nact, grid(),
-                             cfaces(), sdf(), st);
+                             cfaces(), sdf(), nullptr, nullptr, st);
     }
     sorted_valid = false;
   }
