@@ -49,26 +49,38 @@ __global__ void k_gather_parts(Parts<T> s, Parts<T> d, const uint32_t* __restric
 // ------------------------------------------------------------------ classify (solver.rs:342-371)
 // Per cell: the static class (outlet -> Air, !mask -> Solid, else Air) then Fluid if a
 // particle whose `cell_of` is this cell lies outside the outlet.  Also counts the
-// particles whose `cell_of` is this cell (volume_source, solver.rs:400-409).
-template <class T>
-__global__ void k_classify(const uint8_t* __restrict__ static_cls, const uint32_t* __restrict__ cstart,
-                           const uint32_t* __restrict__ cend, Parts<T> p, GridD<T> g, BoxD<T> outlet,
-                           uint8_t* cells, uint32_t* count) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+// particles whose `cell_of` is this cell (volume_source, solver.rs:400-409).  Two passes,
+// so a cell of thousands of particles costs no more than as many ordinary ones: the
+// cells take their static class and a zero count, then every particle adds itself to
+// its cell's count (one atomic per run of equal cells in a wave: the particles are in
+// cell order) and marks an Air cell Fluid when it lies outside the outlet (every such
+// store writes the same value).
+__global__ void k_classify_init(const uint8_t* __restrict__ static_cls, int64_t nc, uint8_t* cells,
+                                uint32_t* count) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  uint8_t cls = static_cls[c];
-  uint32_t s = cstart[c], e = cend[c];
-  uint32_t cnt = 0;
-  bool fluid = false;
-  for (uint32_t t = s; t < e; ++t) {
-    V3<T> q = load_pos(p, t);
-    if (g.cell_of(q) != c) continue;
-    ++cnt;
-    if (!outlet.contains(q)) fluid = true;
+  cells[c] = static_cls[c];
+  count[c] = 0;
+}
+template <class T>
+__global__ void k_classify_parts(const uint8_t* __restrict__ static_cls, Parts<T> p, int64_t n, GridD<T> g,
+                                 BoxD<T> outlet, uint8_t* cells, uint32_t* count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int c = -1;
+  if (t < n) {
+    const V3<T> q = load_pos(p, t);
+    c = (int)g.cell_of(q);
+    if (c >= 0 && static_cls[c] == kAir && !outlet.contains(q)) cells[c] = kFluid;
   }
-  cells[c] = (cls == kAir && fluid) ? (uint8_t)kFluid : cls;
-  count[c] = cnt;
+  const int prev = __shfl_up(c, 1);
+  const bool head = lane == 0 || prev != c;
+  const unsigned long long heads = __ballot(head);
+  if (head && c >= 0) {
+    const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    atomicAdd(&count[c], (uint32_t)(end - lane));
+  }
 }
 
 // ------------------------------------------------------------------ P2G (solver.rs:560-585)
@@ -1431,18 +1443,29 @@ __global__ void k_compact(Parts<T> s, Parts<T> d, const uint32_t* __restrict__ k
 
 // The reference compacts in place (particles.rs:38-49), so a survivor's index drops by
 // the number of removed particles before it.
-__global__ void k_rank_fix(uint32_t* rank, int64_t n, const uint32_t* __restrict__ culled,
-                           int64_t m) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The number of removed ranks below a survivor's rank, from a bitmap of the removed
+// ranks and an exclusive scan of its words' population counts.
+__global__ void k_zero_words(uint32_t* bits, int64_t nw) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) bits[w] = 0;
+}
+__global__ void k_mark_ranks(const uint32_t* __restrict__ culled, int64_t m, uint32_t* bits) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const uint32_t r = culled[t];
+  atomicOr(&bits[r >> 5], 1u << (r & 31));
+}
+__global__ void k_word_popc(const uint32_t* __restrict__ bits, int64_t nw, uint32_t* pop) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) pop[w] = (uint32_t)__popc(bits[w]);
+}
+__global__ void k_rank_fix(uint32_t* rank, int64_t n, const uint32_t* __restrict__ bits,
+                           const uint32_t* __restrict__ below) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  uint32_t r = rank[t];
-  int64_t lo = 0, hi = m;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (culled[mid] < r) lo = mid + 1;
-    else hi = mid;
-  }
-  rank[t] = r - (uint32_t)lo;
+  const uint32_t r = rank[t];
+  const uint32_t w = r >> 5;
+  rank[t] = r - (below[w] + (uint32_t)__popc(bits[w] & ((1u << (r & 31)) - 1u)));
 }
 
 // ------------------------------------------------------------------ redistribute (solver.rs:454-557)
@@ -1717,12 +1740,11 @@ void launch_gather_parts(const Parts<T>& s, const Parts<T>& d, const uint32_t* p
   k_gather_parts<T><<<grid_for(n), kBlock, 0, st>>>(s, d, perm, n);
 }
 template <class T>
-void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
-                     const Parts<T>& p, GridD<T> g, BoxD<T> outlet, uint8_t* cells,
-                     uint32_t* count, hipStream_t st) {
+void launch_classify(const uint8_t* static_cls, const Parts<T>& p, int64_t n, GridD<T> g, BoxD<T> outlet,
+                     uint8_t* cells, uint32_t* count, hipStream_t st) {
   int64_t nc = (int64_t)g.nx * g.ny * g.nz;
-  k_classify<T><<<grid_for(nc), kBlock, 0, st>>>(static_cls, cstart, cend, p, g, outlet, cells,
-                                                  count);
+  k_classify_init<<<grid_for(nc), kBlock, 0, st>>>(static_cls, nc, cells, count);
+  if (n > 0) k_classify_parts<T><<<grid_for(n), kBlock, 0, st>>>(static_cls, p, n, g, outlet, cells, count);
 }
 template <class T>
 void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
@@ -1950,9 +1972,16 @@ void launch_compact(const Parts<T>& s, const Parts<T>& d, const uint32_t* keep, 
   if (n <= 0) return;
   k_compact<T><<<grid_for(n), kBlock, 0, st>>>(s, d, keep, pos, n, culled_ranks);
 }
-void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* culled, int64_t m, hipStream_t st) {
-  if (n <= 0 || m <= 0) return;
-  k_rank_fix<<<grid_for(n), kBlock, 0, st>>>(rank, n, culled, m);
+void launch_rank_bits(const uint32_t* culled, int64_t m, int64_t nranks, uint32_t* bits, uint32_t* pop,
+                      hipStream_t st) {
+  const int64_t nw = (nranks + 31) / 32;
+  k_zero_words<<<grid_for(nw), kBlock, 0, st>>>(bits, nw);
+  if (m > 0) k_mark_ranks<<<grid_for(m), kBlock, 0, st>>>(culled, m, bits);
+  k_word_popc<<<grid_for(nw), kBlock, 0, st>>>(bits, nw, pop);
+}
+void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* bits, const uint32_t* below, hipStream_t st) {
+  if (n <= 0) return;
+  k_rank_fix<<<grid_for(n), kBlock, 0, st>>>(rank, n, bits, below);
 }
 template <class T>
 void launch_hist(const Parts<T>& p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count,
@@ -2069,8 +2098,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
 #define SDR_INST(T)                                                                                \
   template void launch_gather_parts<T>(const Parts<T>&, const Parts<T>&, const uint32_t*, int64_t,  \
                                        hipStream_t);                                               \
-  template void launch_classify<T>(const uint8_t*, const uint32_t*, const uint32_t*, const Parts<T>&, \
-                                   GridD<T>, BoxD<T>, uint8_t*, uint32_t*, hipStream_t);           \
+  template void launch_classify<T>(const uint8_t*, const Parts<T>&, int64_t, GridD<T>, BoxD<T>,      \
+                                   uint8_t*, uint32_t*, hipStream_t);                              \
   template void launch_p2g<T>(int, GridD<T>, const Parts<T>&, const uint32_t*, const uint32_t*, T*, \
                               bool, hipStream_t);                                                  \
   template void launch_axpy_const<T>(T*, int64_t, T, hipStream_t);                                 \

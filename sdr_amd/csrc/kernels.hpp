@@ -56,9 +56,8 @@ template <class T>
 void launch_gather_parts(const Parts<T>& src, const Parts<T>& dst, const uint32_t* perm, int64_t n,
                          hipStream_t st);
 template <class T>
-void launch_classify(const uint8_t* static_cls, const uint32_t* cstart, const uint32_t* cend,
-                     const Parts<T>& p, GridD<T> g, BoxD<T> outlet, uint8_t* cells,
-                     uint32_t* count, hipStream_t st);
+void launch_classify(const uint8_t* static_cls, const Parts<T>& p, int64_t n, GridD<T> g, BoxD<T> outlet,
+                     uint8_t* cells, uint32_t* count, hipStream_t st);
 template <class T>
 void launch_p2g(int comp, GridD<T> g, const Parts<T>& p, const uint32_t* cstart,
                 const uint32_t* cend, T* out, bool exact, hipStream_t st);
@@ -174,8 +173,13 @@ void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, ui
 template <class T>
 void launch_compact(const Parts<T>& src, const Parts<T>& dst, const uint32_t* keep,
                     const uint32_t* pos, int64_t n, uint32_t* culled_ranks, hipStream_t st);
-void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* culled_sorted, int64_t nculled,
-                     hipStream_t st);
+// The reference compacts in place (particles.rs:38-49), so a survivor's rank drops by the
+// number of removed ranks below it: bits = a bitmap of the m removed ranks over
+// [0, nranks), pop = its words' population counts; with below = the exclusive scan of
+// pop, launch_rank_fix subtracts below[r / 32] + the bits under r in its word.
+void launch_rank_bits(const uint32_t* culled, int64_t m, int64_t nranks, uint32_t* bits, uint32_t* pop,
+                      hipStream_t st);
+void launch_rank_fix(uint32_t* rank, int64_t n, const uint32_t* bits, const uint32_t* below, hipStream_t st);
 template <class T>
 void launch_hist(const Parts<T>& p, int64_t n, GridD<T> g, int32_t* cellkey, uint32_t* count,
                  hipStream_t st);

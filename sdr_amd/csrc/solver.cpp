@@ -893,7 +893,7 @@ struct Solver : SolverBase {
   Buf cstart, cend, count, sort_cursor, mask, static_cls, cells;
   Buf fu, fv, fw, su, sv, sw, ftmp, known0, known1, pressure, sdf_data;
   Buf temp;
-  Buf keep, scanpos, cullflag, culled_ranks, culled_sorted;
+  Buf keep, scanpos, cullflag, culled_ranks;
   Buf cellkey, act_rank, act_slot, act_rank_sorted, act_slot_sorted, dest;
   Buf counters, aos_pos, aos_vel, fill_cells, rd_resv, rd_second, rd_ctr;
   Buf rd_sidx, rd_boff, rd_src_cell, rd_src_off, rd_src_head, rd_src_lim, rd_src_fill, rd_pend_a, rd_pend_b,
@@ -1034,7 +1034,7 @@ struct Solver : SolverBase {
         part[w][a].swap(nb);
       }
     for (Buf* b : {&keys, &keys_alt, &vals, &vals_alt, &keep, &scanpos, &cullflag, &culled_ranks,
-                   &culled_sorted, &cellkey, &act_rank, &act_slot, &act_rank_sorted,
+                   &cellkey, &act_rank, &act_slot, &act_rank_sorted,
                    &act_slot_sorted, &dest})
       b->alloc((size_t)nc * 4);
     if (opts.dot_mode == SDR_DOT_EXACT) skey_t.alloc((size_t)nc * 4);
@@ -1111,8 +1111,8 @@ struct Solver : SolverBase {
   void classify() {
     sort_particles();
     Scope sc_(timer, "classify");
-    launch_classify<T>(static_cls.as<uint8_t>(), cstart.as<uint32_t>(), cend.as<uint32_t>(),
-                       parts(cur), grid(), outlet_box(), cells.as<uint8_t>(), count.as<uint32_t>(), st);
+    launch_classify<T>(static_cls.as<uint8_t>(), parts(cur), n, grid(), outlet_box(), cells.as<uint8_t>(),
+                       count.as<uint32_t>(), st);
   }
   void ensure_counts() {
     if (sorted_valid) return;
@@ -1222,9 +1222,12 @@ struct Solver : SolverBase {
                       culled_ranks.as<uint32_t>(), st);
     cur = 1 - cur;
     int64_t kept = n - removed;
-    HIPCHK(sort_keys_u32(culled_ranks.as<uint32_t>(), culled_sorted.as<uint32_t>(), cullflag.as<uint32_t>(),
-                         removed, st));
-    launch_rank_fix(part[cur][6].as<uint32_t>(), kept, culled_sorted.as<uint32_t>(), removed, st);
+    // ranks of the survivors: minus the removed ranks below them (keep, cullflag and
+    // scanpos are free once the compaction has run)
+    const int64_t nw = (n + 31) / 32;
+    launch_rank_bits(culled_ranks.as<uint32_t>(), removed, n, keep.as<uint32_t>(), cullflag.as<uint32_t>(), st);
+    HIPCHK(exclusive_sum_u32(temp.p, temp.bytes, cullflag.as<uint32_t>(), scanpos.as<uint32_t>(), nw, st));
+    launch_rank_fix(part[cur][6].as<uint32_t>(), kept, keep.as<uint32_t>(), scanpos.as<uint32_t>(), st);
     n = kept;
     sorted_valid = false;
   }
@@ -1801,9 +1804,9 @@ int sdr_solver_footprint(uint64_t nx, uint64_t ny, uint64_t nz, uint64_t particl
     const double S = precision == SDR_PRECISION_F32 ? 4.0 : 8.0;
     const double nc = (double)nx * ny * nz;
     const double nsh = (double)nx * (nx + ny - 1) * nz;  // sheared solve layout
-    // per particle: two 7-array particle buffers, 15 u32 work arrays, sort scratch (the
+    // per particle: two 7-array particle buffers, 14 u32 work arrays, sort scratch (the
     // exact mode's merge keys, the long-segment list)
-    const double ppart = 2.0 * (6.0 * S + 4.0) + 60.0 + 5.0;
+    const double ppart = 2.0 * (6.0 * S + 4.0) + 56.0 + 5.0;
     // per cell: cell ranges, counts, sort cursor and classes, faces (current, saved,
     // scratch), pressure, redistribution tables, the P2G heavy-cell table (one entry per
     // 64 cells); per sheared slot: 7 solve vectors,
