@@ -1422,23 +1422,35 @@ __global__ void k_bounds(Parts<T> p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uin
   if (why) atomicAdd(&counters[why - 1], 1ull);
 }
 
+// Compaction in place.  The slots' order is not the reference order (that is the rank,
+// and every order-dependent stage reads ranks), so only the m removed particles' slots
+// change hands: with kept = n - m, the removed slots below kept (holes) and the
+// survivors at or above kept (movers) are equally many, and hole j takes mover j, both
+// numbered in slot order (pos = exclusive scan of keep).  The removed ranks are listed
+// for the rank fix.
 template <class T>
-__global__ void k_compact(Parts<T> s, Parts<T> d, const uint32_t* __restrict__ keep,
-                          const uint32_t* __restrict__ pos, int64_t n, uint32_t* culled_ranks) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  uint32_t q = pos[t];
-  if (keep[t]) {
-    d.px[q] = s.px[t];
-    d.py[q] = s.py[t];
-    d.pz[q] = s.pz[t];
-    d.vx[q] = s.vx[t];
-    d.vy[q] = s.vy[t];
-    d.vz[q] = s.vz[t];
-    d.rank[q] = s.rank[t];
-  } else {
-    culled_ranks[(uint32_t)t - q] = s.rank[t];
-  }
+__global__ void k_compact_holes(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos, int64_t n,
+                                int64_t kept, const uint32_t* __restrict__ rank, uint32_t* hole_slot,
+                                uint32_t* culled_ranks) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || keep[t]) return;
+  const uint32_t j = (uint32_t)t - pos[t];  // removed slots before t
+  culled_ranks[j] = rank[t];
+  if (t < kept) hole_slot[j] = (uint32_t)t;
+}
+template <class T>
+__global__ void k_compact_move(Parts<T> p, const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                               int64_t n, int64_t kept, const uint32_t* __restrict__ hole_slot) {
+  const int64_t t = kept + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !keep[t]) return;
+  const uint32_t d = hole_slot[pos[t] - pos[kept]];  // survivors in [kept, t)
+  p.px[d] = p.px[t];
+  p.py[d] = p.py[t];
+  p.pz[d] = p.pz[t];
+  p.vx[d] = p.vx[t];
+  p.vy[d] = p.vy[t];
+  p.vz[d] = p.vz[t];
+  p.rank[d] = p.rank[t];
 }
 
 // The reference compacts in place (particles.rs:38-49), so a survivor's index drops by
@@ -1967,10 +1979,11 @@ void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, ui
   k_bounds<T><<<grid_for(n), kBlock, 0, st>>>(p, n, sdf, outlet, keep, cull, counters);
 }
 template <class T>
-void launch_compact(const Parts<T>& s, const Parts<T>& d, const uint32_t* keep, const uint32_t* pos,
-                    int64_t n, uint32_t* culled_ranks, hipStream_t st) {
-  if (n <= 0) return;
-  k_compact<T><<<grid_for(n), kBlock, 0, st>>>(s, d, keep, pos, n, culled_ranks);
+void launch_compact(const Parts<T>& p, const uint32_t* keep, const uint32_t* pos, int64_t n, int64_t kept,
+                    uint32_t* hole_slot, uint32_t* culled_ranks, hipStream_t st) {
+  if (n <= 0 || kept >= n) return;
+  k_compact_holes<T><<<grid_for(n), kBlock, 0, st>>>(keep, pos, n, kept, p.rank, hole_slot, culled_ranks);
+  k_compact_move<T><<<grid_for(n - kept), kBlock, 0, st>>>(p, keep, pos, n, kept, hole_slot);
 }
 void launch_rank_bits(const uint32_t* culled, int64_t m, int64_t nranks, uint32_t* bits, uint32_t* pop,
                       hipStream_t st) {
@@ -2142,8 +2155,8 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
                                      SdfD<T>, T, hipStream_t);                                     \
   template void launch_bounds<T>(const Parts<T>&, int64_t, SdfD<T>, BoxD<T>, uint32_t*, uint32_t*, \
                                  unsigned long long*, hipStream_t);                                \
-  template void launch_compact<T>(const Parts<T>&, const Parts<T>&, const uint32_t*,               \
-                                  const uint32_t*, int64_t, uint32_t*, hipStream_t);               \
+  template void launch_compact<T>(const Parts<T>&, const uint32_t*, const uint32_t*, int64_t, int64_t, \
+                                  uint32_t*, uint32_t*, hipStream_t);                              \
   template void launch_hist<T>(const Parts<T>&, int64_t, GridD<T>, int32_t*, uint32_t*,            \
                                hipStream_t);                                                       \
   template void launch_redist_apply<T>(const Parts<T>&, const uint32_t*, const int32_t*, int64_t,  \

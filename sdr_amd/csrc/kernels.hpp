@@ -170,9 +170,12 @@ void launch_g2p_advect(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, C
 template <class T>
 void launch_bounds(const Parts<T>& p, int64_t n, SdfD<T> sdf, BoxD<T> outlet, uint32_t* keep,
                    uint32_t* cull_flag, unsigned long long* counters, hipStream_t st);
+// Removes the particles with keep = 0 in place (pos = exclusive scan of keep, kept = the
+// survivors): the survivors above kept fill the removed slots below it, in slot order;
+// the removed particles' ranks go to culled_ranks.  hole_slot: scratch of n entries.
 template <class T>
-void launch_compact(const Parts<T>& src, const Parts<T>& dst, const uint32_t* keep,
-                    const uint32_t* pos, int64_t n, uint32_t* culled_ranks, hipStream_t st);
+void launch_compact(const Parts<T>& p, const uint32_t* keep, const uint32_t* pos, int64_t n, int64_t kept,
+                    uint32_t* hole_slot, uint32_t* culled_ranks, hipStream_t st);
 // The reference compacts in place (particles.rs:38-49), so a survivor's rank drops by the
 // number of removed ranks below it: bits = a bitmap of the m removed ranks over
 // [0, nranks), pop = its words' population counts; with below = the exclusive scan of

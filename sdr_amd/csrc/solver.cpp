@@ -1218,10 +1218,9 @@ struct Solver : SolverBase {
     culled += cu;
     int64_t removed = (int64_t)(dr + cu);
     if (removed == 0) return;
-    launch_compact<T>(parts(cur), parts(1 - cur), keep.as<uint32_t>(), scanpos.as<uint32_t>(), n,
-                      culled_ranks.as<uint32_t>(), st);
-    cur = 1 - cur;
     int64_t kept = n - removed;
+    launch_compact<T>(parts(cur), keep.as<uint32_t>(), scanpos.as<uint32_t>(), n, kept, cullflag.as<uint32_t>(),
+                      culled_ranks.as<uint32_t>(), st);
     // ranks of the survivors: minus the removed ranks below them (keep, cullflag and
     // scanpos are free once the compaction has run)
     const int64_t nw = (n + 31) / 32;
