@@ -366,6 +366,39 @@ __device__ inline void block_partial(double v, double* partials) {
     partials[blockIdx.x] = OPMAX ? nmax(nmax(nmax(red[0], red[1]), red[2]), red[3])
                                  : ((red[0] + red[1]) + red[2]) + red[3];
 }
+// The sum of n partials in k_dot_finish's fast order (lane-strided sums over 256
+// threads, a butterfly per wave, the four waves in order), by every thread of a
+// 256-thread workgroup, each of which gets it.
+__device__ inline double block_combine_sum(const double* partials, int n) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < n; q += 256) acc += partials[q];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  return ((red[0] + red[1]) + red[2]) + red[3];
+}
+// kOpDenom of the fast mode (pressure.rs:372-373), combined by every workgroup of the
+// axpy in k_dot_finish's order: denom = dot(s, As), alpha = sigma / denom with the sigma
+// the fused product left in sigma_next, which workgroup 0 commits.  False (for every
+// workgroup alike) when the solve stops here: a breakdown (|denom| < 1e-30) or a NaN.
+__device__ inline bool denom_step(PcgScalars* sc, const double* den_partials, int nden, double* alpha) {
+  const double den = block_combine_sum(den_partials, nden);
+  const double sig = sc->sigma_next;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (den != den) {
+    if (lead) sc->nan_seen = 1, sc->done = 1;
+    return false;
+  }
+  if (lead) sc->sigma = sig, sc->denom = den;
+  if (fabs(den) < 1e-30) {
+    if (lead) sc->done = 1;
+    return false;
+  }
+  *alpha = sig / den;
+  if (lead) sc->alpha = *alpha;
+  return true;
+}
 
 // ------------------------------------------------------------------ apply_a (pressure.rs:400-439)
 // Matrix-free: diag and the -1 couplings come from the flags.  In-bounds neighbours
@@ -567,7 +600,8 @@ inline Tile25 tile25(Shear sh) {
 template <class T>
 __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* __restrict__ flags,
                                                      const T* __restrict__ z, T* s0, T* s1, T* __restrict__ out,
-                                                     PcgScalars* sc, double* partials, Tile25 tl) {
+                                                     PcgScalars* sc, double* partials, const double* dot_partials,
+                                                     int ndot, Tile25 tl) {
   if (sc->done) return;
   constexpr int NS = ((kR + 2) * 66 + 255) / 256;  // staged slots per thread
   constexpr int NC = kR / 4;                       // owned cells per thread and plane
@@ -575,12 +609,25 @@ __global__ void __launch_bounds__(256) k_spmv_update(Shear sh, const uint16_t* _
   // four plane slots: the staging of plane k + 1 writes the slot plane k - 3 used, which no
   // thread reads after the barrier of plane k - 1, so one barrier per plane suffices
   __shared__ T ring[4][kR + 2][66];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters += 1;  // one more iteration of a running solve
   const int par = sc->spar;
   const bool copy = sc->started == 0;
+  // kOpSigmaNew of the step before (pressure.rs:383-385), combined here by every
+  // workgroup in the same order: beta = dot(z, r) / sigma; the new sigma waits in
+  // sigma_next for the axpy (this kernel's workgroups still read the old one)
+  const double sig_new = copy ? sc->sigma : block_combine_sum(dot_partials, ndot);
+  if (sig_new != sig_new) {  // a NaN reaches every dot: stop, report it
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->nan_seen = 1, sc->done = 1;
+    return;
+  }
+  const double beta_d = copy ? 0.0 : sig_new / sc->sigma;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->iters += 1;  // one more iteration of a running solve
+    if (!copy) sc->beta = beta_d;
+    sc->sigma_next = sig_new;
+  }
   const T* sin = par ? s1 : s0;
   T* sout = par ? s0 : s1;
-  const T beta = (T)sc->beta;
+  const T beta = (T)beta_d;
   const int nx = sh.nx, ny = sh.ny, nz = sh.nz, NJ = sh.NJ;
   const int64_t plane = (int64_t)NJ * nx;
   // plane k of a buffer of esz-byte elements; no records unless k is in [klo, khi)
@@ -837,12 +884,16 @@ __global__ void __launch_bounds__(256) k_update_s(Shear sh, T* __restrict__ s, c
 template <class T>
 __global__ void __launch_bounds__(256) k_axpy_pr2(Shear sh, T* __restrict__ p, T* __restrict__ r,
                                                   const T* s0, const T* __restrict__ as,
-                                                  const PcgScalars* sc, double* partials,
-                                                  const T* s1 = nullptr) {
+                                                  PcgScalars* sc, double* partials,
+                                                  const T* s1 = nullptr, const double* den_partials = nullptr,
+                                                  int nden = 0) {
   if (sc->done) return;
-  // s1: s is the direction the fused product just wrote (sbuf[1 - spar])
+  // s1: s is the direction the fused product just wrote (sbuf[1 - spar]), and the
+  // product's partials of dot(s, As) are combined here (denom_step)
   const T* __restrict__ s = s1 && !sc->spar ? s1 : s0;
-  const T alpha = (T)sc->alpha;
+  double alpha_d = sc->alpha;
+  if (den_partials && !denom_step(sc, den_partials, nden, &alpha_d)) return;
+  const T alpha = (T)alpha_d;
   double m = 0.0;
   for_row_pairs(sh, [&](bool ao, const RowRun& a, int ai, bool bo, const RowRun& b, int bi) {
     const int64_t ta = a.base + ai, tb = b.base + bi;
@@ -930,14 +981,17 @@ __global__ void __launch_bounds__(256) k_axpy_pr_units(Shear sh, T* __restrict__
 template <class T, int U>
 __global__ void __launch_bounds__(256) k_axpy_pr_par(Shear sh, T* __restrict__ p, T* __restrict__ r,
                                                      const T* s0, const T* s1, const T* __restrict__ as,
-                                                     const PcgScalars* sc, double* partials, int span) {
+                                                     PcgScalars* sc, double* partials, const double* den_partials,
+                                                     int nden, int span) {
   if (sc->done) return;
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t n = (uint32_t)sh.size();
   const T* s = sc->spar ? s0 : s1;
   const rsrc_t rp = make_rsrc(p, n * SZ), rr = make_rsrc(r, n * SZ), rs = make_rsrc(s, n * SZ),
                rq = make_rsrc(as, n * SZ);
-  const T alpha = (T)sc->alpha;
+  double alpha_d;
+  if (!denom_step(sc, den_partials, nden, &alpha_d)) return;
+  const T alpha = (T)alpha_d;
   double m = 0.0;
   int q0, q1;
   unit_span(sh, span, &q0, &q1);
@@ -1842,7 +1896,7 @@ void launch_axpy_pr(Shear sh, T* p, T* r, const T* s, const T* as, const PcgScal
     k_axpy_pr_units<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials, unit_span_len(sh, nblocks, 4));
     return;
   }
-  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, sc, partials);
+  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s, as, const_cast<PcgScalars*>(sc), partials);  // reads sc only
 }
 template <class T>
 void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream_t st) {
@@ -1860,17 +1914,19 @@ int spmv_update_blocks(Shear sh) {
 }
 template <class T>
 void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
-                        double* partials, hipStream_t st) {
-  k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials, tile25(sh));
+                        double* partials, const double* dot_partials, int ndot, hipStream_t st) {
+  k_spmv_update<T><<<spmv_update_blocks(sh), 256, 0, st>>>(sh, flags, z, s0, s1, out, sc, partials, dot_partials,
+                                                           ndot, tile25(sh));
 }
 template <class T>
-void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
-                        double* partials, int nblocks, hipStream_t st) {
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, PcgScalars* sc,
+                        double* partials, const double* den_partials, int nden, int nblocks, hipStream_t st) {
   if (units_fit<T>(sh)) {
-    k_axpy_pr_par<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s0, s1, as, sc, partials, unit_span_len(sh, nblocks, 4));
+    k_axpy_pr_par<T, 4><<<nblocks, 256, 0, st>>>(sh, p, r, s0, s1, as, sc, partials, den_partials, nden,
+                                                  unit_span_len(sh, nblocks, 4));
     return;
   }
-  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s0, as, sc, partials, s1);
+  k_axpy_pr2<T><<<nblocks, 256, 0, st>>>(sh, p, r, s0, as, sc, partials, s1, den_partials, nden);
 }
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
@@ -2130,9 +2186,9 @@ void launch_build_units(int nx, int ny, uint32_t* out, hipStream_t st) {
                                   int, hipStream_t);                                               \
   template void launch_update_s<T>(Shear, T*, const T*, const PcgScalars*, hipStream_t);           \
   template void launch_spmv_update<T>(Shear, const uint16_t*, const T*, T*, T*, T*, PcgScalars*,     \
-                                      double*, hipStream_t);                                       \
-  template void launch_axpy_pr_par<T>(Shear, T*, T*, const T*, const T*, const T*, const PcgScalars*, \
-                                      double*, int, hipStream_t);                                  \
+                                      double*, const double*, int, hipStream_t);                   \
+  template void launch_axpy_pr_par<T>(Shear, T*, T*, const T*, const T*, const T*, PcgScalars*,     \
+                                      double*, const double*, int, int, hipStream_t);              \
   template void launch_mic_plane<T>(int, Shear, const uint16_t*, T*, const T*, T*, int,            \
                                     const PcgScalars*, hipStream_t);                               \
   template void launch_gradient<T>(GridD<T>, Shear, FacesD<T>, const uint8_t*, const T*, T,        \

@@ -34,6 +34,12 @@ struct PcgScalars {
   // holding the current s; started is 0 until the first step, whose product takes s = z.
   int32_t spar;
   int32_t started;
+  int32_t pad2;
+  int32_t pad3;
+  // The fast mode's step sigma, set by the fused product (which combines the backward
+  // sweep's partials of dot(z, r) itself) and committed to sigma by the axpy (which
+  // combines the product's partials of dot(s, As) itself): no finish launches between.
+  double sigma_next;
 };
 
 enum DotOp : int32_t {
@@ -119,14 +125,19 @@ void launch_update_s(Shear sh, T* s, const T* z, const PcgScalars* sc, hipStream
 // The direction update s = z + beta s (s = z on a solve's first step) fused with the
 // product As = A s and the partials of dot(s, As): s is read from sbuf[spar] and written
 // to sbuf[1 - spar] (the residual's finish flips spar).  partials: spmv_update_blocks.
+// Each workgroup first combines the backward sweep's ndot partials of dot(z, r) itself
+// (kOpSigmaNew: beta = that / sigma, kept in sigma_next; not on a solve's first step).
 template <class T>
 void launch_spmv_update(Shear sh, const uint16_t* flags, const T* z, T* s0, T* s1, T* out, PcgScalars* sc,
-                        double* partials, hipStream_t st);
+                        double* partials, const double* dot_partials, int ndot, hipStream_t st);
 int spmv_update_blocks(Shear sh);
-// p += alpha s; r -= alpha As with s the direction just written by launch_spmv_update.
+// p += alpha s; r -= alpha As with s the direction just written by launch_spmv_update,
+// each workgroup first combining the product's nden partials of dot(s, As) itself
+// (kOpDenom: alpha = sigma_next / denom; sigma = sigma_next).  partials: its own, of
+// max|r| (kOpResidual).
 template <class T>
-void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, const PcgScalars* sc,
-                        double* partials, int nblocks, hipStream_t st);
+void launch_axpy_pr_par(Shear sh, T* p, T* r, const T* s0, const T* s1, const T* as, PcgScalars* sc,
+                        double* partials, const double* den_partials, int nden, int nblocks, hipStream_t st);
 template <class T>
 void launch_mic_plane(int op, Shear sh, const uint16_t* flags, T* precon, const T* in, T* out,
                       int sigma, const PcgScalars* sc, hipStream_t st);

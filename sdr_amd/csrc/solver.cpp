@@ -161,6 +161,9 @@ struct Projector {
   hipStream_t st = nullptr;
   Timer* timer = nullptr;
   Buf b, z, s, s2, p, as, q, precon, flags, partials, scal, units;  // s2: the fused product's second direction buffer
+  // the fast mode's partials of the backward sweep's dot(z, r) and of the axpy's max|r|,
+  // read by the next kernel while `partials` holds the fused product's dot(s, As)
+  Buf part_dot, part_res;
   Buf halo_x, halo_z, pipe_err, sink;
   Buf b64, p64, r64, norms;  // f64 refinement of the f32 solve
   uint32_t sweep_tag = 0;
@@ -276,6 +279,8 @@ struct Projector {
     scal.alloc(sizeof(PcgScalars));
     host_scal.ensure(sizeof(PcgScalars) + 16);
     partials.alloc(pl.nb * sizeof(double));
+    part_dot.alloc(pl.nb * sizeof(double));
+    part_res.alloc(pl.nb * sizeof(double));
     halo_x.alloc(pl.hx * sizeof(uint64_t));
     halo_z.alloc(pl.hz * sizeof(uint64_t));
     pipe_err.alloc(sizeof(int));
@@ -299,7 +304,7 @@ struct Projector {
   // Bytes of caller memory init_noalloc carves for a plan (256-byte aligned pieces).
   static size_t up256(size_t b) { return (b + 255) / 256 * 256; }
   static size_t noalloc_bytes(const Plan& pl) {
-    return 8 * up256(pl.n * sizeof(T)) + up256(pl.n * 2) + up256(pl.nb * 8) + up256(sizeof(PcgScalars)) +
+    return 8 * up256(pl.n * sizeof(T)) + up256(pl.n * 2) + 3 * up256(pl.nb * 8) + up256(sizeof(PcgScalars)) +
            up256(pl.nunits * 4) + up256(pl.hx * 8) + up256(pl.hz * 8) + up256(sizeof(int)) + up256(1024 * sizeof(T));
   }
   // The same working set as init, as views of [base, base + cap): no allocation, and the
@@ -321,6 +326,8 @@ struct Projector {
     for (Buf* bb : {&b, &z, &s, &s2, &p, &as, &q, &precon}) take(*bb, pl.n * sizeof(T));
     take(flags, pl.n * 2);
     take(partials, pl.nb * 8);
+    take(part_dot, pl.nb * 8);
+    take(part_res, pl.nb * 8);
     take(scal, sizeof(PcgScalars));
     take(units, pl.nunits * 4);
     take(halo_x, pl.hx * 8);
@@ -374,7 +381,7 @@ struct Projector {
     }
     MicPipeArgs<T> pa{halo_x.as<uint64_t>(), halo_z.as<uint64_t>(), sink.as<T>(), pipe_err.as<int>(),
                       1L << 22, ps.ntile_i, ps.ntile_k, sweep_tag, (uint32_t)halo_x.bytes,
-                      (uint32_t)halo_z.bytes, ncu, dot ? 1 : 0, partials.as<double>(), fused() ? 1 : 0,
+                      (uint32_t)halo_z.bytes, ncu, dot ? 1 : 0, part_dot.as<double>(), fused() ? 1 : 0,
                       dev_tag ? tagbuf.as<uint32_t>() : nullptr};
     launch_mic_pipe<T>(op, variant, sh, flags.as<uint16_t>(), precon.as<T>(), in, out, sc(), pa, st);
   }
@@ -396,13 +403,21 @@ struct Projector {
     launch_dot_finish(partials.as<double>(), nb, exact_dot, op, sc(), tol, st);
   }
 
-  // The fused step head of the fast mode: s = z + beta s (s = z on the first step) and
-  // As = A s with the partials of dot(s, As), then denom and alpha.  The new s goes to
-  // the other direction buffer (sbuf = {s, s2}, PcgScalars::spar).
+  // The fused step head of the fast mode: beta from the last backward sweep's partials,
+  // s = z + beta s (s = z on the first step) and As = A s with the partials of
+  // dot(s, As), which the axpy combines into denom and alpha.  The new s goes to the
+  // other direction buffer (sbuf = {s, s2}, PcgScalars::spar).
   void fused_product() {
     launch_spmv_update<T>(sh, flags.as<uint16_t>(), z.as<T>(), s.as<T>(), s2.as<T>(), as.as<T>(), sc(),
-                          partials.as<double>(), st);
-    launch_dot_finish(partials.as<double>(), spmv_update_blocks(sh), false, kOpDenom, sc(), 0.0, st);
+                          partials.as<double>(), part_dot.as<double>(),
+                          mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)), st);
+  }
+  // p += alpha s, r -= alpha As with max|r| after the fused product (denom and alpha
+  // combined in the kernel), then the stopping test.
+  void fused_axpy(int mb, double tol) {
+    launch_axpy_pr_par<T>(sh, p.as<T>(), b.as<T>(), s.as<T>(), s2.as<T>(), as.as<T>(), sc(), part_res.as<double>(),
+                          partials.as<double>(), spmv_update_blocks(sh), mb, st);
+    launch_dot_finish(part_res.as<double>(), mb, false, kOpResidual, sc(), tol, st);
   }
   // The current direction buffer (fused: by the polled scalars).
   Buf& cur_s(const PcgScalars& h) { return fused() && h.spar ? s2 : s; }
@@ -453,7 +468,6 @@ struct Projector {
     dot(z.as<T>(), r, kOpSigma0, tol);
     const int mb = maxabs_blocks(sh);
     const uint64_t check_every = 8;
-    const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
     // one PCG iteration (pressure.rs:372-396); dev: sweep tags from the device word
     auto iteration = [&](bool dev) {
       if (fz) {
@@ -463,9 +477,7 @@ struct Projector {
         }
         {
           Scope sc1(*timer, "axpy_norm");
-          launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
-                                mb, st);
-          launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), tol, st);
+          fused_axpy(mb, tol);
         }
         {
           Scope sc1(*timer, "mic_fwd");
@@ -473,8 +485,7 @@ struct Projector {
         }
         {
           Scope sc1(*timer, "mic_bwd_dot");
-          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);
-          launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), tol, st);
+          mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);  // its dot is combined by the next product
         }
         return;
       }
@@ -655,7 +666,6 @@ struct Projector {
     const bool fz = fused();
     if (!fz) HIPCHK(hipMemcpyAsync(s.p, z.p, n * sizeof(T), hipMemcpyDeviceToDevice, st));
     dot(z.as<T>(), r, kOpSigma0, 0.0);
-    const int ndot = fz ? mic_pipe_dot_partials(mic_pipe_shape(sh, mic_variant, ncu)) : 0;
     const uint64_t check_every = 8;
     uint64_t it = 0;
     int replacements = 0, restarts = 0;
@@ -669,9 +679,7 @@ struct Projector {
           fused_product();
         }
         Scope sc1(*timer, "axpy_norm");
-        launch_axpy_pr_par<T>(sh, p.as<T>(), r, s.as<T>(), s2.as<T>(), as.as<T>(), sc(), partials.as<double>(),
-                              mb, st);
-        launch_dot_finish(partials.as<double>(), mb, false, kOpResidual, sc(), 0.0, st);
+        fused_axpy(mb, 0.0);
         return;
       }
       {
@@ -692,8 +700,7 @@ struct Projector {
           mic_sweep(kMicFwd, r, q.as<T>(), false, dev);
         }
         Scope sc1(*timer, "mic_bwd_dot");
-        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);
-        launch_dot_finish(partials.as<double>(), ndot, false, kOpSigmaNew, sc(), 0.0, st);
+        mic_sweep(kMicBwd, q.as<T>(), z.as<T>(), true, dev);  // its dot is combined by the next product
       } else {
         {
           Scope sc1(*timer, "mic_fwd");
