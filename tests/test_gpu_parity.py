@@ -227,6 +227,38 @@ def test_p2g_heavy_cells_match_face_gather(dev, precision, monkeypatch):
         assert np.abs(got[reach] - ref[reach]).max() <= tol * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_bounds_removals_match_oracle(dev, orc, mode):
+    """enforce_particle_bounds + retain (solver.rs:736-755, particles.rs:38-49) with many
+    removals: a c1 fill plus particles deep in the wall, outside the SDF box and in the
+    outlet, shuffled.  The device compacts in place (survivors from the end fill the
+    removed slots) and fixes the ranks from a bitmap of the removed ones; the survivors
+    in reference order, the counts and a following classify must equal the oracle's."""
+    sc, o, _ = paired("c1", orc, None, stride=2)
+    d = scenes.make_solver(sc, **(EXACT if mode == "exact" else FAST))
+    g = np.random.default_rng(21)
+    pos, _ = o.particles()
+    lo = np.asarray(sc.sdf.origin)
+    hi = lo + sc.sdf.dx * (np.array([sc.sdf.nx, sc.sdf.ny, sc.sdf.nz]) - 1)
+    extra = [lo + (hi - lo) * g.uniform(-0.2, 1.2, (3000, 3))]  # walls, and outside the box
+    ob = sc.outlet
+    extra.append(np.asarray(ob.min) + (np.asarray(ob.max) - np.asarray(ob.min)) * g.uniform(0, 1, (500, 3)))
+    allp = np.concatenate([pos] + extra)
+    allp = allp[g.permutation(allp.shape[0])]
+    vel = g.standard_normal(allp.shape)
+    for s_ in (o, d):
+        s_.set_particles(allp, vel)
+        s_.run_stage("bounds", sc.dt)
+    so, sd = o.state(), d.state()
+    assert so.culled > 1000 and (so.drained, so.culled) == (sd.drained, sd.culled)
+    po, vo = o.particles()
+    pd, vd = d.particles()
+    assert bits_equal(po, pd) and bits_equal(vo, vd)
+    for s_ in (o, d):
+        s_.run_stage("classify", sc.dt)
+    assert np.array_equal(o.cells(), d.cells())
+
+
 def test_deterministic_run_to_run(dev):
     """No float atomics anywhere: two device runs agree bit for bit, fast mode included."""
     out = []
