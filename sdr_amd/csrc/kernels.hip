@@ -217,17 +217,21 @@ __global__ void k_enforce_solid(GridD<T> g, T* __restrict__ a, const uint8_t* __
 // pressure.rs:152-223 with the volume-control source of solver.rs:392-423, written in
 // the sheared solve layout.  One thread per sheared slot; slots outside the grid get
 // b = 0 and flags = 0.
-template <class T>
-__global__ void k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* __restrict__ cells,
-                            const uint32_t* __restrict__ count, const double* __restrict__ srcarr,
-                            double inv_scale, int use_source, double rest, double k_vol,
-                            double unit, double cap, T* __restrict__ b,
-                            uint16_t* __restrict__ flags, double* __restrict__ b64) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= sh.size()) return;
-  int i = (int)(t % sh.nx);
-  int J = (int)((t / sh.nx) % sh.NJ);
-  int k = (int)(t / ((int64_t)sh.nx * sh.NJ));
+template <class T, class I>
+__global__ void __launch_bounds__(256) k_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f,
+                                                   const uint8_t* __restrict__ cells,
+                                                   const uint32_t* __restrict__ count,
+                                                   const double* __restrict__ srcarr, double inv_scale,
+                                                   int use_source, double rest, double k_vol, double unit,
+                                                   double cap, T* __restrict__ b, uint16_t* __restrict__ flags,
+                                                   double* __restrict__ b64) {
+  // I: the index type (32-bit when the sheared slots number below 2^31)
+  const I t = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  if (t >= (I)sh.size()) return;
+  const I rowi = t / (I)sh.nx;
+  int i = (int)(t - rowi * (I)sh.nx);
+  int J = (int)(rowi % (I)sh.NJ);
+  int k = (int)(rowi / (I)sh.NJ);
   int j = J - i;
   if (j < 0 || j >= sh.ny) {
     b[t] = T(0);
@@ -1327,12 +1331,14 @@ __global__ void k_accum64(Shear sh, double* __restrict__ p, const T* __restrict_
 }
 
 // ------------------------------------------------------------------ extrapolate (solver.rs:608-647, 960-1009)
-template <class T, int COMP>
-__global__ void k_extrap_init(GridD<T> g, const uint8_t* __restrict__ cells, uint8_t* known) {
+template <class T, int COMP, class I>
+__global__ void __launch_bounds__(256) k_extrap_init(GridD<T> g, const uint8_t* __restrict__ cells,
+                                                     uint8_t* known) {
   const int fnx = g.nx + (COMP == 0), fny = g.ny + (COMP == 1), fnz = g.nz + (COMP == 2);
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= (int64_t)fnx * fny * fnz) return;
-  int i = (int)(f % fnx), j = (int)((f / fnx) % fny), k = (int)(f / ((int64_t)fnx * fny));
+  const I f = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  if (f >= (I)fnx * (I)fny * (I)fnz) return;
+  const I row = f / (I)fnx;
+  const int i = (int)(f - row * (I)fnx), j = (int)(row % (I)fny), k = (int)(row / (I)fny);
   auto fluid = [&](int a_, int b_, int c_) {
     return cells[((int64_t)c_ * g.ny + b_) * g.nx + a_] == kFluid;
   };
@@ -1343,18 +1349,21 @@ __global__ void k_extrap_init(GridD<T> g, const uint8_t* __restrict__ cells, uin
   known[f] = v;
 }
 
-template <class T>
-__global__ void k_extrap_iter(int nx, int ny, int nz, const T* __restrict__ src, T* __restrict__ dst,
-                              const uint8_t* __restrict__ ksrc, uint8_t* __restrict__ kdst) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= (int64_t)nx * ny * nz) return;
+// I: the index type (32-bit when the faces number below 2^31: no 64-bit division)
+template <class T, class I>
+__global__ void __launch_bounds__(256) k_extrap_iter(int nx, int ny, int nz, const T* __restrict__ src,
+                                                     T* __restrict__ dst, const uint8_t* __restrict__ ksrc,
+                                                     uint8_t* __restrict__ kdst) {
+  const I c = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  if (c >= (I)nx * (I)ny * (I)nz) return;
   if (ksrc[c]) {
     dst[c] = src[c];
     kdst[c] = 1;
     return;
   }
-  int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((int64_t)nx * ny));
-  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  const I row = c / (I)nx;
+  const int i = (int)(c - row * (I)nx), j = (int)(row % (I)ny), k = (int)(row / (I)ny);
+  const I sy = (I)nx, sz = (I)nx * (I)ny;
   T sum = T(0);
   int cnt = 0;
   if (i > 0 && ksrc[c - 1]) { sum += src[c - 1]; ++cnt; }
@@ -1844,9 +1853,14 @@ void launch_rhs_flags(GridD<T> g, Shear sh, CFacesD<T> f, const uint8_t* cells,
                       const uint32_t* count, const double* srcarr, double inv_scale, int use_source,
                       double rest, double k_vol, double unit, double cap, T* b, uint16_t* flags,
                       double* b64, hipStream_t st) {
-  k_rhs_flags<T><<<grid_for(sh.size()), kBlock, 0, st>>>(g, sh, f, cells, count, srcarr, inv_scale,
-                                                          use_source, rest, k_vol, unit, cap, b,
-                                                          flags, b64);
+  if (sh.size() < ((int64_t)1 << 31))
+    k_rhs_flags<T, uint32_t><<<grid_for(sh.size()), kBlock, 0, st>>>(g, sh, f, cells, count, srcarr, inv_scale,
+                                                                      use_source, rest, k_vol, unit, cap, b,
+                                                                      flags, b64);
+  else
+    k_rhs_flags<T, int64_t><<<grid_for(sh.size()), kBlock, 0, st>>>(g, sh, f, cells, count, srcarr, inv_scale,
+                                                                     use_source, rest, k_vol, unit, cap, b,
+                                                                     flags, b64);
 }
 int cell_blocks(Shear sh) {
   int64_t waves = (int64_t)sh.nz * sh.NJ;  // one wavefront per row at most
@@ -2000,15 +2014,24 @@ void launch_accum64(Shear sh, double* p, const T* e, double scale, hipStream_t s
 template <class T>
 void launch_extrap_init(GridD<T> g, int comp, const uint8_t* cells, uint8_t* known, hipStream_t st) {
   int64_t nf = (int64_t)(g.nx + (comp == 0)) * (g.ny + (comp == 1)) * (g.nz + (comp == 2));
-  if (comp == 0) k_extrap_init<T, 0><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
-  if (comp == 1) k_extrap_init<T, 1><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
-  if (comp == 2) k_extrap_init<T, 2><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+  if (nf < ((int64_t)1 << 31)) {
+    if (comp == 0) k_extrap_init<T, 0, uint32_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+    if (comp == 1) k_extrap_init<T, 1, uint32_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+    if (comp == 2) k_extrap_init<T, 2, uint32_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+    return;
+  }
+  if (comp == 0) k_extrap_init<T, 0, int64_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+  if (comp == 1) k_extrap_init<T, 1, int64_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
+  if (comp == 2) k_extrap_init<T, 2, int64_t><<<grid_for(nf), kBlock, 0, st>>>(g, cells, known);
 }
 template <class T>
 void launch_extrap_iter(int nx, int ny, int nz, const T* src, T* dst, const uint8_t* ksrc,
                         uint8_t* kdst, hipStream_t st) {
-  k_extrap_iter<T><<<grid_for((int64_t)nx * ny * nz), kBlock, 0, st>>>(nx, ny, nz, src, dst, ksrc,
-                                                                        kdst);
+  const int64_t n = (int64_t)nx * ny * nz;
+  if (n < ((int64_t)1 << 31))
+    k_extrap_iter<T, uint32_t><<<grid_for(n), kBlock, 0, st>>>(nx, ny, nz, src, dst, ksrc, kdst);
+  else
+    k_extrap_iter<T, int64_t><<<grid_for(n), kBlock, 0, st>>>(nx, ny, nz, src, dst, ksrc, kdst);
 }
 template <class T>
 void launch_g2p(const Parts<T>& p, int64_t n, GridD<T> g, CFacesD<T> f, CFacesD<T> saved, T flip,
